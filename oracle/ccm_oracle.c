@@ -1,0 +1,505 @@
+/*
+ * CCMpred CPU ORACLE -- TEST INFRASTRUCTURE ONLY (see ccm_oracle.h).
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "ccm_oracle.h"
+
+/* ---- PLL evaluator, FP32 and FP64 instantiations (evaluate_cpu.c) ------ */
+
+#define REAL float
+#define SFX f32
+#define EXPR(v) expf(v)
+#define LOGR(v) logf(v)
+#include "pll_tmpl.h"
+#undef REAL
+#undef SFX
+#undef EXPR
+#undef LOGR
+
+#define REAL double
+#define SFX f64
+#define EXPR(v) exp(v)
+#define LOGR(v) log(v)
+#include "pll_tmpl.h"
+#undef REAL
+#undef SFX
+#undef EXPR
+#undef LOGR
+
+/* ---- reweighting (reweighting.c:9-67) ---------------------------------- */
+
+/* reweighting.c:10: threshold (float) times ncol is a float product; its
+ * ceiling is taken in double and truncated to int. */
+int ora_identity_threshold(int ncol, float threshold)
+{
+	float prod = threshold * (float)(uint64_t)ncol;
+	return (int)ceil((double)prod);
+}
+
+/* counts[i] = #{ j != i : identities(i, j) > idthres } */
+void ora_identity_counts(const unsigned char *msa, int ncol, int nrow, int idthres,
+                         int32_t *counts)
+{
+	memset(counts, 0, sizeof(int32_t) * (size_t)nrow);
+	for (long i = 0; i < nrow; i++) {
+		const unsigned char *si = msa + i * ncol;
+		for (long j = i + 1; j < nrow; j++) {
+			const unsigned char *sj = msa + j * ncol;
+			int ids = 0;
+			for (int k = 0; k < ncol; k++)
+				ids += si[k] == sj[k];
+			if (ids > idthres) {
+				counts[i]++;
+				counts[j]++;
+			}
+		}
+	}
+}
+
+/*
+ * reweighting.c:9-58.  The reference loops over the N(N+1)/2 pairs i <= j,
+ * so the diagonal pair adds 2 to w[i] when ncol > idthres; w accumulates in
+ * float and is finally set to 1./(w - 1) in double, rounded to float.
+ */
+void ora_calculate_weights(const unsigned char *msa, int ncol, int nrow, float threshold,
+                           float *w)
+{
+	int idthres = ora_identity_threshold(ncol, threshold);
+	int32_t *cnt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nrow > 0 ? nrow : 1));
+	ora_identity_counts(msa, ncol, nrow, idthres, cnt);
+	int diag = ncol > idthres ? 2 : 0;
+	for (long i = 0; i < nrow; i++) {
+		float acc = (float)(cnt[i] + diag);
+		w[i] = (float)(1. / (acc - 1));
+	}
+	free(cnt);
+}
+
+void ora_uniform_weights(int nrow, float *w)
+{
+	for (long i = 0; i < nrow; i++)
+		w[i] = 1.0f;
+}
+
+/* ---- init_bias (ccmpred.c:90-131) -------------------------------------- */
+
+void ora_init_bias(const unsigned char *msa, int ncol, int nrow, float *x)
+{
+	memset(x, 0, sizeof(float) * (size_t)ora_nvar_padded(ncol));
+	for (long j = 0; j < ncol; j++) {
+		int cnt[ORA_NA];
+		for (int a = 0; a < ORA_NA; a++)
+			cnt[a] = 1;
+		for (long i = 0; i < nrow; i++)
+			cnt[msa[i * ncol + j]]++;
+		int total = nrow + ORA_NA;
+		float frac[ORA_NA];
+		for (int a = 0; a < ORA_NA; a++)
+			frac[a] = (float)cnt[a] / (float)total;
+		float gapnorm = logf(frac[ORA_NA - 1]);
+		for (int a = 0; a < ORA_NA - 1; a++)
+			x[a * ncol + j] = logf(frac[a]) - gapnorm;
+	}
+}
+
+/* ---- OpenMP evaluator (evaluate_cpu_omp.c:18-189) ---------------------- */
+
+struct ora_omp_ctx {
+	const unsigned char *msa;
+	const float *w;
+	int ncol, nrow;
+	float lambda_single, lambda_pair;
+	float *raw; /* init-time pair scratch, like omp_extra_userdata.g2 */
+};
+
+int ora_omp_max_threads(void)
+{
+#ifdef _OPENMP
+	return omp_get_max_threads();
+#else
+	return 1;
+#endif
+}
+
+ora_omp_ctx *ora_omp_create(const unsigned char *msa, const float *w, int ncol, int nrow,
+                            float lambda_single, float lambda_pair, int nthreads)
+{
+	ora_omp_ctx *c = (ora_omp_ctx *)calloc(1, sizeof(*c));
+	c->msa = msa;
+	c->w = w;
+	c->ncol = ncol;
+	c->nrow = nrow;
+	c->lambda_single = lambda_single;
+	c->lambda_pair = lambda_pair;
+	c->raw = (float *)malloc(sizeof(float) * (size_t)ncol * ncol * ORA_NA * ORA_NAP);
+#ifdef _OPENMP
+	if (nthreads > 0)
+		omp_set_num_threads(nthreads);
+#else
+	(void)nthreads;
+#endif
+	return c;
+}
+
+void ora_omp_destroy(ora_omp_ctx *c)
+{
+	if (!c)
+		return;
+	free(c->raw);
+	free(c);
+}
+
+/*
+ * Same structure as evaluate_cpu_omp.c: per-call buffers of N*32*L floats
+ * (:50-52), row-parallel logits (:57-99), serial f (:101-107), column-parallel
+ * node gradient (:109-121), a serial loop over sequences with a parallel
+ * loop over columns for the pair gradient (:124-144), serial symmetrisation
+ * and self-edge reset (:147-165), parallel single and serial pair L2 (:170-179).
+ */
+float ora_evaluate_omp_f32(ora_omp_ctx *c, const float *x, float *g)
+{
+	const long L = c->ncol, N = c->nrow;
+	const long nsingle = L * (ORA_NA - 1);
+	const long nsp = ora_nsingle_padded(c->ncol);
+	const long nvp = ora_nvar_padded(c->ncol);
+	const unsigned char *msa = c->msa;
+	const float *x1 = x, *x2 = x + nsp;
+	float *g1 = g, *g2 = g + nsp;
+	float *raw = c->raw;
+	const float lam1 = c->lambda_single, lam2 = c->lambda_pair;
+
+	float f = 0;
+	memset(g, 0, sizeof(float) * (size_t)nvp);
+	memset(raw, 0, sizeof(float) * (size_t)(nvp - nsp));
+
+	float *logit = (float *)malloc(sizeof(float) * (size_t)N * ORA_NAP * L);
+	float *lz = (float *)malloc(sizeof(float) * (size_t)N * L);
+	float *prob = (float *)malloc(sizeof(float) * (size_t)N * ORA_NAP * L);
+	memset(lz, 0, sizeof(float) * (size_t)N * L);
+
+#define PCI(i, a, s) (((i) * ORA_NA + (a)) * L + (s))
+
+	#pragma omp parallel for
+	for (long i = 0; i < N; i++) {
+		const unsigned char *seq = msa + i * L;
+		for (long a = 0; a < ORA_NA - 1; a++)
+			for (long s = 0; s < L; s++)
+				logit[PCI(i, a, s)] = x1[a * L + s];
+		for (long s = 0; s < L; s++)
+			logit[PCI(i, ORA_NA - 1, s)] = 0;
+		for (long k = 0; k < L; k++) {
+			float *pp = logit + PCI(i, 0, 0);
+			const float *pw = x2 + ORA_X2(seq[k], k, 0, 0, L);
+			for (long t = 0; t < ORA_NA * L; t++)
+				pp[t] += pw[t];
+		}
+		for (long a = 0; a < ORA_NA; a++)
+			for (long s = 0; s < L; s++)
+				lz[i * L + s] += expf(logit[PCI(i, a, s)]);
+		for (long s = 0; s < L; s++)
+			lz[i * L + s] = logf(lz[i * L + s]);
+		for (long a = 0; a < ORA_NA; a++)
+			for (long s = 0; s < L; s++)
+				prob[PCI(i, a, s)] = expf(logit[PCI(i, a, s)] - lz[i * L + s]);
+	}
+
+	for (long i = 0; i < N; i++) {
+		const float wi = c->w[i];
+		for (long k = 0; k < L; k++) {
+			int xk = msa[i * L + k];
+			f += wi * (lz[i * L + k] - logit[PCI(i, xk, k)]);
+		}
+	}
+
+	#pragma omp parallel for
+	for (long k = 0; k < L; k++) {
+		for (long i = 0; i < N; i++) {
+			const float wi = c->w[i];
+			int xk = msa[i * L + k];
+			if (xk < ORA_NA - 1)
+				g1[xk * L + k] -= wi;
+			for (long a = 0; a < ORA_NA - 1; a++)
+				g1[a * L + k] += wi * prob[PCI(i, a, k)];
+		}
+	}
+
+	for (long i = 0; i < N; i++) {
+		const float wi = c->w[i];
+		const unsigned char *seq = msa + i * L;
+		#pragma omp parallel for
+		for (long k = 0; k < L; k++) {
+			float *pg = raw + ORA_X2(seq[k], k, 0, 0, L);
+			for (long j = 0; j < L; j++)
+				pg[seq[j] * L + j] -= wi;
+			const float *pp = prob + PCI(i, 0, 0);
+			for (long t = 0; t < ORA_NA * L; t++)
+				pg[t] += wi * pp[t];
+		}
+	}
+#undef PCI
+
+	for (long b = 0; b < ORA_NA; b++)
+		for (long k = 0; k < L; k++)
+			for (long a = 0; a < ORA_NA; a++)
+				for (long j = 0; j < L; j++)
+					g2[ORA_X2(b, k, a, j, L)] =
+					    raw[ORA_X2(b, k, a, j, L)] + raw[ORA_X2(a, j, b, k, L)];
+	for (long b = 0; b < ORA_NA; b++)
+		for (long k = 0; k < L; k++)
+			for (long a = 0; a < ORA_NA; a++)
+				g2[ORA_X2(b, k, a, k, L)] = 0;
+
+	float reg = 0;
+	#pragma omp parallel for reduction(+ : reg)
+	for (long v = 0; v < nsingle; v++) {
+		reg += lam1 * x[v] * x[v];
+		g[v] += 2.0f * lam1 * x[v];
+	}
+	for (long v = nsp; v < nvp; v++) {
+		reg += 0.5f * lam2 * x[v] * x[v];
+		g[v] += 2.0f * lam2 * x[v];
+	}
+	f += reg;
+
+	free(logit);
+	free(lz);
+	free(prob);
+	return f;
+}
+
+/* ---- contact scores (util.c:16-139) ------------------------------------ */
+
+void ora_sum_submatrices(const float *x, int ncol, float *out)
+{
+	const long L = ncol;
+	const float *x2 = x + ora_nsingle_padded(ncol);
+	memset(out, 0, sizeof(float) * (size_t)L * L);
+	for (long k = 0; k < L; k++) {
+		for (long j = 0; j < L; j++) {
+			float mean = 0;
+			for (int a = 0; a < ORA_NA; a++)
+				for (int b = 0; b < ORA_NA; b++)
+					mean += x2[ORA_X2(b, k, a, j, L)];
+			mean /= (float)(ORA_NA * ORA_NA);
+			for (int a = 0; a < ORA_NA - 1; a++)
+				for (int b = 0; b < ORA_NA - 1; b++) {
+					float d = x2[ORA_X2(b, k, a, j, L)] - mean;
+					out[k * L + j] += d * d;
+				}
+		}
+	}
+	for (long k = 0; k < L; k++) {
+		for (long j = 0; j < L; j++)
+			out[k * L + j] = (float)sqrt((double)out[k * L + j]);
+		out[k * L + k] = 0;
+	}
+}
+
+void ora_apc(float *mat, int ncol)
+{
+	const long L = ncol;
+	float *means = (float *)calloc((size_t)L, sizeof(float));
+	float meansum = 0;
+	for (long i = 0; i < L; i++)
+		for (long j = 0; j < L; j++) {
+			float v = mat[i * L + j];
+			means[j] += v / (float)L;
+			meansum += v;
+		}
+	meansum /= (float)(L * L);
+	for (long i = 0; i < L; i++)
+		for (long j = 0; j < L; j++)
+			mat[i * L + j] -= (means[i] * means[j]) / meansum;
+	float lo = INFINITY;
+	for (long i = 0; i < L; i++)
+		for (long j = i + 1; j < L; j++)
+			if (mat[i * L + j] < lo)
+				lo = mat[i * L + j];
+	for (long i = 0; i < L; i++) {
+		for (long j = 0; j < L; j++)
+			mat[i * L + j] -= lo;
+		mat[i * L + i] = 0;
+	}
+	free(means);
+}
+
+void ora_normalize(float *mat, int ncol)
+{
+	const long L = ncol;
+	float lo = mat[1], hi = mat[1];
+	for (long i = 0; i < L; i++)
+		for (long j = 0; j < L; j++) {
+			if (i == j)
+				continue;
+			float v = mat[i * L + j];
+			if (v < lo)
+				lo = v;
+			if (v > hi)
+				hi = v;
+		}
+	float range = hi - lo;
+	for (long i = 0; i < L; i++)
+		for (long j = 0; j < L; j++)
+			mat[i * L + j] = i == j ? 0.0f : (mat[i * L + j] - lo) / range;
+}
+
+/* ---- host conjugate gradient ------------------------------------------- */
+
+static double ora_dot(const float *a, const float *b, long n)
+{
+	/* fixed-order pairwise FP64 summation over blocks of 4096 */
+	double total = 0;
+	for (long s = 0; s < n; s += 4096) {
+		long e = s + 4096 < n ? s + 4096 : n;
+		double part = 0;
+		for (long i = s; i < e; i++)
+			part += (double)a[i] * (double)b[i];
+		total += part;
+	}
+	return total;
+}
+
+enum { ORA_CG_SUCCESS = 0, ORA_CG_ALREADY_MINIMIZED = 1, ORA_CG_MAXITER = 2,
+       ORA_CG_ERR_LINESEARCH = -1 };
+
+int ora_cg(int n, float *x, double *fx, ora_eval_fn eval, void *ctx, const ora_cg_param *p,
+           int *iterations, int *evaluations, double *fhist)
+{
+	float *g = (float *)malloc(sizeof(float) * (size_t)n);
+	float *gprev = (float *)malloc(sizeof(float) * (size_t)n);
+	float *s = (float *)malloc(sizeof(float) * (size_t)n);
+	double *hist = (double *)malloc(sizeof(double) * (size_t)(p->max_iterations + 1));
+	int nevals = 0, it = 0, ret = ORA_CG_MAXITER;
+
+	double f = eval(ctx, x, g, n);
+	nevals++;
+	double gg = ora_dot(g, g, n);
+	if (gg == 0.0) {
+		ret = ORA_CG_ALREADY_MINIMIZED;
+		goto done;
+	}
+	for (long i = 0; i < n; i++)
+		s[i] = -g[i];
+	double dg0 = -gg;
+	double alpha = 1.0 / sqrt(gg);
+	int steepest = 1;
+
+	while (it < p->max_iterations) {
+		const double f0 = f;
+		float *tmp = gprev;
+		gprev = g;
+		g = tmp;
+		double moved = 0.0, ft = 0.0;
+		int accepted = 0, ls;
+		for (ls = 1; ls <= p->max_linesearch; ls++) {
+			const float step = (float)(alpha - moved);
+			for (long i = 0; i < n; i++)
+				x[i] += step * s[i];
+			moved = alpha;
+			ft = eval(ctx, x, g, n);
+			nevals++;
+			if (ft <= f0 + p->ftol * alpha * dg0) {
+				double dgt = ora_dot(g, s, n);
+				if (dgt <= -p->wolfe * dg0 || ls == p->max_linesearch) {
+					accepted = 1;
+					break;
+				}
+			}
+			if (ls < p->max_linesearch)
+				alpha *= p->alpha_mul;
+		}
+		if (!accepted) {
+			const float back = (float)(-moved);
+			for (long i = 0; i < n; i++)
+				x[i] += back * s[i];
+			tmp = gprev;
+			gprev = g;
+			g = tmp;
+			f = f0;
+			if (steepest) {
+				ret = ORA_CG_ERR_LINESEARCH;
+				goto done;
+			}
+			for (long i = 0; i < n; i++)
+				s[i] = -g[i];
+			dg0 = -gg;
+			alpha = 1.0 / sqrt(gg);
+			steepest = 1;
+			continue;
+		}
+		f = ft;
+		hist[it] = f;
+		if (fhist)
+			fhist[it] = f;
+		it++;
+		if (p->k > 0 && it > p->k) {
+			double before = hist[it - 1 - p->k];
+			if ((before - f) / fabs(f) < p->epsilon) {
+				ret = ORA_CG_SUCCESS;
+				break;
+			}
+		}
+		double ggn = ora_dot(g, g, n);
+		double ggp = ora_dot(g, gprev, n);
+		double beta = (ggn - ggp) / gg;
+		if (beta < 0)
+			beta = 0;
+		const float fb = (float)beta;
+		for (long i = 0; i < n; i++)
+			s[i] = -g[i] + fb * s[i];
+		double dgn = ora_dot(g, s, n);
+		if (dgn >= 0) {
+			for (long i = 0; i < n; i++)
+				s[i] = -g[i];
+			dgn = -ggn;
+			steepest = 1;
+		} else {
+			steepest = beta == 0.0;
+		}
+		alpha = alpha * dg0 / dgn;
+		dg0 = dgn;
+		gg = ggn;
+	}
+done:
+	*fx = f;
+	if (iterations)
+		*iterations = it;
+	if (evaluations)
+		*evaluations = nevals;
+	free(g);
+	free(gprev);
+	free(s);
+	free(hist);
+	return ret;
+}
+
+typedef struct {
+	const unsigned char *msa;
+	const float *w;
+	int ncol, nrow;
+	float l1, l2;
+} ora_pll_ctx;
+
+static double ora_pll_eval(void *vctx, const float *x, float *g, int n)
+{
+	(void)n;
+	ora_pll_ctx *c = (ora_pll_ctx *)vctx;
+	return ora_evaluate_f32(c->msa, c->w, c->ncol, c->nrow, c->l1, c->l2, x, g);
+}
+
+int ora_cg_pll_f32(const unsigned char *msa, const float *w, int ncol, int nrow,
+                   float lambda_single, float lambda_pair, float *x, const ora_cg_param *p,
+                   double *fx, int *iterations, int *evaluations, double *fhist)
+{
+	ora_pll_ctx c = {msa, w, ncol, nrow, lambda_single, lambda_pair};
+	return ora_cg((int)ora_nvar_padded(ncol), x, fx, ora_pll_eval, &c, p, iterations,
+	              evaluations, fhist);
+}

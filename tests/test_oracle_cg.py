@@ -1,0 +1,73 @@
+"""Host CG of the oracle.  Its parity with the reference optimizer is
+unpinned (libconjugrad is not vendored); these tests check that it descends
+monotonically and reaches the minimiser found by an independent optimizer."""
+import numpy as np
+import pytest
+from scipy.optimize import minimize
+
+from oracle import oracle as ora
+from ccmpred_amd.msa import synthetic_msa
+from tests.pll_bruteforce import nsingle_padded, NA, NAP
+
+
+def _setup(L=8, N=60, seed=13):
+    msa, _ = synthetic_msa(L, N, seed=seed, n_pairs=1)
+    w = ora.calculate_weights(msa, 0.8)
+    return msa, w
+
+
+def test_cg_monotone_and_descends():
+    msa, w = _setup()
+    L = msa.shape[1]
+    x0 = ora.init_bias(msa)
+    f0, _ = ora.evaluate_f32(msa, w, x0, 0.01, 0.2 * (L - 1))
+    r = ora.cg_pll_f32(msa, w, x0, 0.01, 0.2 * (L - 1), max_iterations=30)
+    assert r["ret"] == 2 and r["iterations"] == 30
+    h = r["fhist"]
+    assert np.all(np.diff(h) <= 0)
+    assert h[-1] < f0
+
+
+def test_cg_reaches_minimum():
+    msa, w = _setup(L=6, N=40, seed=17)
+    L = msa.shape[1]
+    lam1, lam2 = 0.01, 0.2 * (L - 1)
+    nsp = nsingle_padded(L)
+    iu = [(k, j) for k in range(L) for j in range(k + 1, L)]
+
+    def to_x(theta):
+        x = np.zeros(ora.nvar_padded(L))
+        x[: 20 * L] = theta[: 20 * L]
+        X2 = x[nsp:].reshape(NA, L, NAP, L)
+        blocks = theta[20 * L:].reshape(len(iu), NA, NA)
+        for t, (k, j) in enumerate(iu):
+            X2[:, k, :NA, j] = blocks[t]          # [b, a]
+            X2[:, j, :NA, k] = blocks[t].T
+        return x
+
+    def fg(theta):
+        x = to_x(theta)
+        f, g = ora.evaluate_f64(msa, w, x, lam1, lam2)
+        G2 = g[nsp:].reshape(NA, L, NAP, L)
+        gt = np.concatenate([g[: 20 * L]] + [G2[:, k, :NA, j].ravel() for k, j in iu])
+        return f, gt
+
+    theta0 = np.zeros(20 * L + len(iu) * NA * NA)
+    res = minimize(fg, theta0, jac=True, method="L-BFGS-B",
+                   options=dict(maxiter=5000, gtol=1e-10, ftol=1e-15))
+    fstar = res.fun
+    r = ora.cg_pll_f32(msa, w, np.zeros(ora.nvar_padded(L), np.float32), lam1, lam2,
+                       max_iterations=400)
+    # at the FP32 resolution of f the line search may stop with -1
+    assert r["ret"] in (0, 2, -1) and r["iterations"] > 100
+    f0, _ = ora.evaluate_f64(msa, w, np.zeros(ora.nvar_padded(L)), lam1, lam2)
+    assert r["fx"] == pytest.approx(fstar, rel=1e-4)
+    assert (f0 - r["fx"]) / (f0 - fstar) > 0.999
+
+
+def test_cg_epsilon_stop():
+    msa, w = _setup()
+    L = msa.shape[1]
+    r = ora.cg_pll_f32(msa, w, ora.init_bias(msa), 0.01, 0.2 * (L - 1),
+                       max_iterations=500, epsilon=1e-3, k=5)
+    assert r["ret"] == 0 and r["iterations"] < 500
