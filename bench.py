@@ -1,0 +1,147 @@
+#!/usr/bin/env python
+"""PLL f+grad evaluations per second at L=250, N=5000 on one MI355X.
+
+A step is one call of the evaluate(instance, x, g, nvar) callback on
+device-resident x and g (f read back to the host), for the seeded
+synthetic MSA with L=250 columns, N=5000 sequences and 60 planted contacts
+(seed 2), reweighting threshold 0.8, lambda_single 0.01,
+lambda_pair 0.2*(L-1), and x = init_bias plus symmetric N(0, 0.01^2)
+couplings.  `value` is the BF16-MFMA path; the plain FP32 path and the
+OpenMP CPU evaluator (the oracle's restatement of evaluate_cpu_omp.c) are
+timed in the same run on the same MSA and x.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+L_COLS, N_SEQS, SEED, N_PAIRS = 250, 5000, 2, 60
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+HBM_PEAK_GBS = 8000.0
+
+
+def make_x(msa):
+    from ccmpred_amd.pll import init_bias, nsingle_padded
+    L = msa.shape[1]
+    x = init_bias(msa).astype(np.float64)
+    rng = np.random.Generator(np.random.PCG64(7))
+    W = rng.normal(0.0, 0.01, (21, L, 21, L))
+    W = W + W.transpose(2, 3, 0, 1)
+    idx = np.arange(L)
+    W[:, idx, :, idx] = 0.0
+    X2 = x[nsingle_padded(L):].reshape(21, L, 32, L)
+    X2[:, :, :21, :] += W
+    return x.astype(np.float32)
+
+
+def time_gpu(msa, x_host, precision, steps, warmup):
+    import torch
+    from ccmpred_amd.pll import PLL
+    with PLL(msa, precision=precision) as p:
+        xd = torch.from_numpy(x_host).cuda()
+        gd = torch.empty_like(xd)
+        torch.cuda.synchronize()
+        for _ in range(warmup):
+            p.evaluate_ptr(xd.data_ptr(), gd.data_ptr())
+        p.set_timing(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f = 0.0
+        for _ in range(steps):
+            f = p.evaluate_ptr(xd.data_ptr(), gd.data_ptr())
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ncalls, steps_ms = p.timing()
+    return dict(evals_per_s=steps / (t1 - t0), ms_per_eval=1e3 * (t1 - t0) / steps, f=f,
+                timed_calls=ncalls, step_ms=steps_ms)
+
+
+def time_cpu(msa, x_host, evals):
+    from oracle import oracle as ora
+    w = ora.calculate_weights(msa, 0.8)
+    L = msa.shape[1]
+    ev = ora.OmpEvaluator(msa, w, 0.01, np.float32(0.2 * (L - 1)))
+    g = np.zeros_like(x_host)
+    ev(x_host, g)  # warm-up (page faults of the per-call buffers)
+    ts = []
+    for _ in range(evals):
+        t0 = time.perf_counter()
+        ev(x_host, g)
+        ts.append(time.perf_counter() - t0)
+    threads = ev.threads
+    ev.close()
+    med = float(np.median(ts))
+    return dict(evals_per_s=1.0 / med, seconds=ts, threads=threads)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-evals", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the evaluate callback works on one device); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from ccmpred_amd.msa import synthetic_msa
+    msa, _ = synthetic_msa(L_COLS, N_SEQS, SEED, N_PAIRS)
+    x = make_x(msa)
+
+    bf16 = time_gpu(msa, x, "bf16", args.steps, args.warmup)
+    fp32 = time_gpu(msa, x, "fp32", max(10, args.steps // 4), max(2, args.warmup // 2))
+    cpu = None if args.no_cpu else time_cpu(msa, x, args.cpu_evals)
+
+    L, N = L_COLS, N_SEQS
+    flops = 2.0 * N * (21 * L) ** 2          # algorithmic FLOPs of one contraction
+    gemm = max(("gemm_fwd", "gemm_grad"), key=lambda k: bf16["step_ms"][k])
+    gemm_ms = bf16["step_ms"][gemm]
+    achieved = flops / (gemm_ms * 1e-3) / 1e12
+    line = {
+        "metric": "PLL f+grad evals/s at L=250,N=5000: BF16-MFMA vs FP32 GPU vs host CPU",
+        "value": round(bf16["evals_per_s"], 2),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(bf16["ms_per_eval"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": "PLL evaluate, synthetic MSA L=250 N=5000 (seed 2, 60 planted pairs)",
+                   "L": L, "N": N, "theta": 0.8, "lambda_single": 0.01,
+                   "lambda_pair": 0.2 * (L - 1), "x": "init_bias + symmetric N(0,0.01^2) couplings"},
+        "roofline": {"bound": "mfma", "kernel": gemm, "achieved": round(achieved, 2),
+                     "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
+                     "algorithmic_flops": flops, "avg_ms": round(gemm_ms, 4)},
+        "fp32_evals_per_s": round(fp32["evals_per_s"], 2),
+        "bf16_over_fp32": round(bf16["evals_per_s"] / fp32["evals_per_s"], 3),
+        "bf16_step_ms": {k: round(v, 4) for k, v in bf16["step_ms"].items()},
+        "fp32_step_ms": {k: round(v, 4) for k, v in fp32["step_ms"].items()},
+        "f_bf16": bf16["f"], "f_fp32": fp32["f"],
+    }
+    if cpu is not None:
+        line["cpu_baseline"] = {"value": round(cpu["evals_per_s"], 4), "unit": "evals/s",
+                                "cores": cpu["threads"], "kind": "port",
+                                "sample": f"{len(cpu['seconds'])} evaluations of the full L=250,N=5000 "
+                                          "workload (median), OpenMP restatement of evaluate_cpu_omp.c"}
+        line["bf16_over_cpu"] = round(bf16["evals_per_s"] / cpu["evals_per_s"], 1)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,123 @@
+"""ctypes binding of libccmpred_amd.so (the HIP library).
+
+The library must have been built (``make -C ccmpred_amd/csrc`` or
+``__graft_entry__.build()``); importing without it raises, there is no
+fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libccmpred_amd.so")
+BIN_PATH = os.path.join(_HERE, "bin", "ccmpred")
+
+_lib = None
+
+
+class UserData(C.Structure):
+    """ccmpred.h:39-92, field for field."""
+    _fields_ = [
+        ("msa", C.POINTER(C.c_ubyte)),
+        ("weights", C.POINTER(C.c_float)),
+        ("ncol", C.c_int),
+        ("nrow", C.c_int),
+        ("nsingle", C.c_int),
+        ("nvar", C.c_int),
+        ("extra", C.c_void_p),
+        ("lambda_single", C.c_float),
+        ("lambda_pair", C.c_float),
+        ("reweighting_threshold", C.c_float),
+        ("meta_steps", C.c_void_p),
+    ]
+
+
+class HipConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("bf16_passes", C.c_int),
+                ("verbose", C.c_int)]
+
+
+class CGParam(C.Structure):
+    _fields_ = [("max_iterations", C.c_int), ("epsilon", C.c_double), ("k", C.c_int),
+                ("max_linesearch", C.c_int), ("alpha_mul", C.c_double), ("ftol", C.c_double),
+                ("wolfe", C.c_double)]
+
+
+EVALUATE_T = C.CFUNCTYPE(C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int)
+PROGRESS_T = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float,
+                         C.c_float, C.c_float, C.c_int, C.c_int, C.c_int)
+
+# every symbol the public headers declare (include/ccmpred_amd.h, include/ccmpred_amd_kernels.h)
+EXPORTED = [
+    "ccm_nsingle_padded", "ccm_nvar_padded", "ccm_hip_default_config", "init_hip", "init_hip_ex",
+    "init_hip_bf16", "evaluate_hip", "destroy_hip", "ccm_hip_evaluate", "ccm_hip_stream",
+    "ccm_hip_set_timing", "ccm_hip_get_timing",
+    "ccm_conjugrad_init", "ccm_conjugrad_gpu", "ccm_conjugrad_gpu_ex", "ccm_read_msa",
+    "ccm_init_bias", "ccm_sum_submatrices", "ccm_apc", "ccm_normalize", "ccm_write_matrix",
+    "ccm_write_raw", "ccm_read_raw",
+    "ccm_reweight_words", "ccm_launch_identity_counts", "ccm_launch_weights_from_counts",
+    "ccm_launch_fill_f32", "ccm_launch_gather_fwd_f32", "ccm_launch_gather_grad_f32",
+    "ccm_launch_neg_weighted_onehot", "ccm_softmax_rows_blocks", "ccm_softmax_rows_nblk",
+    "ccm_softmax_cols_blocks", "ccm_softmax_cols_nblk", "ccm_launch_softmax_rows",
+    "ccm_launch_softmax_cols", "ccm_launch_convert_wt", "ccm_launch_gemm_fwd",
+    "ccm_launch_gemm_grad", "ccm_pair_epilogue_blocks", "ccm_singles_blocks",
+    "ccm_launch_pair_epilogue", "ccm_launch_hsym", "ccm_launch_singles_finalize",
+    "ccm_launch_sum3", "ccm_dots_partial_len", "ccm_launch_dots", "ccm_launch_axpy",
+    "ccm_launch_direction", "ccm_launch_negate",
+]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `make -C ccmpred_amd/csrc` "
+                "(there is no CPU fallback)")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    vp, ip = C.c_void_p, C.c_int
+    L.ccm_nsingle_padded.argtypes = [ip]
+    L.ccm_nsingle_padded.restype = C.c_long
+    L.ccm_nvar_padded.argtypes = [ip]
+    L.ccm_nvar_padded.restype = C.c_long
+    L.ccm_hip_default_config.argtypes = [C.POINTER(HipConfig)]
+    L.init_hip.argtypes = [vp]
+    L.init_hip.restype = ip
+    L.init_hip_ex.argtypes = [vp, C.POINTER(HipConfig)]
+    L.init_hip_ex.restype = ip
+    L.init_hip_bf16.argtypes = [vp]
+    L.init_hip_bf16.restype = ip
+    L.evaluate_hip.argtypes = [vp, vp, vp, ip]
+    L.evaluate_hip.restype = C.c_float
+    L.destroy_hip.argtypes = [vp]
+    L.destroy_hip.restype = ip
+    L.ccm_hip_evaluate.argtypes = [vp, vp, vp, ip, C.POINTER(C.c_double)]
+    L.ccm_hip_evaluate.restype = ip
+    L.ccm_hip_stream.argtypes = [vp]
+    L.ccm_hip_stream.restype = vp
+    L.ccm_hip_set_timing.argtypes = [vp, ip]
+    L.ccm_hip_set_timing.restype = ip
+    L.ccm_hip_get_timing.argtypes = [vp, C.POINTER(C.c_double), ip]
+    L.ccm_hip_get_timing.restype = ip
+    L.ccm_conjugrad_init.argtypes = [C.POINTER(CGParam)]
+    L.ccm_conjugrad_gpu_ex.argtypes = [ip, vp, C.POINTER(C.c_double), EVALUATE_T, vp, vp,
+                                       C.POINTER(CGParam), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_double)]
+    L.ccm_conjugrad_gpu_ex.restype = ip
+    L.ccm_init_bias.argtypes = [vp, vp, ip, ip]
+    L.ccm_sum_submatrices.argtypes = [vp, vp, ip]
+    L.ccm_apc.argtypes = [vp, ip]
+    L.ccm_normalize.argtypes = [vp, ip]
+    L.ccm_read_raw.argtypes = [C.c_char_p, ip, vp]
+    L.ccm_read_raw.restype = ip
+
+
+def evaluate_fnptr():
+    """The library's evaluate_hip as a C function pointer usable as a CG callback."""
+    return C.cast(lib().evaluate_hip, EVALUATE_T)

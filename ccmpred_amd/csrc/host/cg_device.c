@@ -1,0 +1,256 @@
+/*
+ * Nonlinear conjugate gradient on device-resident vectors, with the call
+ * shape of libconjugrad's conjugrad_gpu (ccmpred.c:526-534, 619-648).
+ * libconjugrad is not vendored with the reference, so this is an own
+ * design within the reference's parameter set:
+ *
+ *   s0 = -g0, alpha0 = 1/|g0|
+ *   per iteration, line search along s from x (x moves in place):
+ *     x += (alpha - moved) s ; f_t, g_t = evaluate(x)
+ *     Armijo:    f_t <= f0 + ftol * alpha * <g0, s>
+ *     curvature: <g_t, s> <= wolfe * |<g0, s>|   (not far past the minimum)
+ *     accept when both hold, or when Armijo holds on the last allowed trial;
+ *     otherwise alpha *= alpha_mul (at most max_linesearch trials)
+ *   on failure x returns to the start; a failed steepest-descent search ends
+ *   the run with CONJUGRADERR_MAXIMUMLINESEARCH, otherwise the direction is
+ *   reset to -g once
+ *   beta = max(0, (<g,g> - <g,g_prev>) / <g_prev,g_prev>)   (Polak-Ribiere+)
+ *   s = -g + beta s; if <g, s> >= 0 then s = -g
+ *   next alpha = alpha * <g_prev, s_prev> / <g, s>
+ *   stop with CONJUGRAD_SUCCESS once (f_{it-k} - f_it) / |f_it| < epsilon.
+ * All scalars, dot products and tests are FP64; vectors are FP32.
+ */
+#define __HIP_PLATFORM_AMD__
+#include <hip/hip_runtime_api.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "ccmpred_amd.h"
+#include "ccmpred_amd_kernels.h"
+
+void ccm_conjugrad_init(ccm_conjugrad_parameter *p)
+{
+	p->max_iterations = 250;
+	p->epsilon = 0.01;
+	p->k = 5;
+	p->max_linesearch = 5;
+	p->alpha_mul = 0.5;
+	p->ftol = 1e-4;
+	p->wolfe = 0.2;
+}
+
+typedef struct {
+	hipStream_t s;
+	double *d_part;
+	double *d_out;
+	double *h_out;
+	long n;
+} cg_ctx;
+
+static int dots(cg_ctx *c, const float *a, const float *b0, const float *b1, const float *b2,
+                int nd, double *out)
+{
+	if (ccm_launch_dots(a, b0, b1, b2, c->n, nd, c->d_part, c->d_out, c->s) != 0)
+		return -1;
+	if (hipMemcpyAsync(c->h_out, c->d_out, sizeof(double) * (size_t)nd, hipMemcpyDeviceToHost,
+	                   c->s) != hipSuccess)
+		return -1;
+	if (hipStreamSynchronize(c->s) != hipSuccess)
+		return -1;
+	for (int i = 0; i < nd; i++)
+		out[i] = c->h_out[i];
+	return 0;
+}
+
+static int sync_stream(cg_ctx *c) { return hipStreamSynchronize(c->s) == hipSuccess ? 0 : -1; }
+
+int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_evaluate_t evaluate,
+                         conjugrad_progress_t progress, void *instance,
+                         const ccm_conjugrad_parameter *p, int *iterations, int *evaluations,
+                         double *fhist)
+{
+	cg_ctx c = {0};
+	float *g = NULL, *gprev = NULL, *sdir = NULL;
+	double *hist = NULL;
+	int ret = CONJUGRADERR_DEVICE, it = 0, nevals = 0;
+	double f = NAN;
+	c.n = n;
+	if (hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking) != hipSuccess)
+		return CONJUGRADERR_DEVICE;
+	if (hipMalloc((void **)&g, sizeof(float) * (size_t)n) != hipSuccess ||
+	    hipMalloc((void **)&gprev, sizeof(float) * (size_t)n) != hipSuccess ||
+	    hipMalloc((void **)&sdir, sizeof(float) * (size_t)n) != hipSuccess ||
+	    hipMalloc((void **)&c.d_part, sizeof(double) * (size_t)ccm_dots_partial_len()) !=
+	        hipSuccess ||
+	    hipMalloc((void **)&c.d_out, sizeof(double) * 4) != hipSuccess ||
+	    hipHostMalloc((void **)&c.h_out, sizeof(double) * 4, hipHostMallocDefault) != hipSuccess)
+		goto done;
+	hist = (double *)malloc(sizeof(double) * (size_t)(p->max_iterations + 1));
+	if (!hist)
+		goto done;
+
+	f = evaluate(instance, d_x, g, n);
+	nevals++;
+	if (isnan(f))
+		goto done;
+	double gg;
+	if (dots(&c, g, g, NULL, NULL, 1, &gg))
+		goto done;
+	if (gg == 0.0) {
+		ret = CONJUGRAD_ALREADY_MINIMIZED;
+		goto done;
+	}
+	if (ccm_launch_negate(sdir, g, n, c.s))
+		goto done;
+	double dg0 = -gg;
+	double alpha = 1.0 / sqrt(gg);
+	int steepest = 1;
+	ret = CONJUGRADERR_MAXIMUMITERATION;
+
+	while (it < p->max_iterations) {
+		const double f0 = f;
+		float *tmp = gprev;
+		gprev = g;
+		g = tmp;
+		double moved = 0.0, ft = 0.0;
+		int accepted = 0, ls;
+		for (ls = 1; ls <= p->max_linesearch; ls++) {
+			if (ccm_launch_axpy(d_x, sdir, (float)(alpha - moved), n, c.s) || sync_stream(&c)) {
+				ret = CONJUGRADERR_DEVICE;
+				goto done;
+			}
+			moved = alpha;
+			ft = evaluate(instance, d_x, g, n);
+			nevals++;
+			if (isnan(ft)) {
+				ret = CONJUGRADERR_DEVICE;
+				goto done;
+			}
+			if (ft <= f0 + p->ftol * alpha * dg0) {
+				double dgt;
+				if (dots(&c, g, sdir, NULL, NULL, 1, &dgt)) {
+					ret = CONJUGRADERR_DEVICE;
+					goto done;
+				}
+				if (dgt <= -p->wolfe * dg0 || ls == p->max_linesearch) {
+					accepted = 1;
+					break;
+				}
+			}
+			if (ls < p->max_linesearch)
+				alpha *= p->alpha_mul;
+		}
+		if (!accepted) {
+			if (ccm_launch_axpy(d_x, sdir, (float)(-moved), n, c.s) || sync_stream(&c)) {
+				ret = CONJUGRADERR_DEVICE;
+				goto done;
+			}
+			tmp = gprev;
+			gprev = g;
+			g = tmp;
+			f = f0;
+			if (steepest) {
+				ret = CONJUGRADERR_MAXIMUMLINESEARCH;
+				goto done;
+			}
+			if (ccm_launch_negate(sdir, g, n, c.s)) {
+				ret = CONJUGRADERR_DEVICE;
+				goto done;
+			}
+			dg0 = -gg;
+			alpha = 1.0 / sqrt(gg);
+			steepest = 1;
+			continue;
+		}
+		f = ft;
+		hist[it] = f;
+		if (fhist)
+			fhist[it] = f;
+		it++;
+
+		double d3[3];
+		if (dots(&c, g, g, gprev, NULL, 2, d3)) {
+			ret = CONJUGRADERR_DEVICE;
+			goto done;
+		}
+		const double ggn = d3[0], ggp = d3[1];
+		if (progress) {
+			double xx;
+			if (dots(&c, d_x, d_x, NULL, NULL, 1, &xx)) {
+				ret = CONJUGRADERR_DEVICE;
+				goto done;
+			}
+			progress(instance, d_x, g, (conjugrad_float_t)f, (conjugrad_float_t)sqrt(xx),
+			         (conjugrad_float_t)sqrt(ggn), (conjugrad_float_t)alpha, n, it, ls);
+		}
+		if (p->k > 0 && it > p->k) {
+			const double before = hist[it - 1 - p->k];
+			if ((before - f) / fabs(f) < p->epsilon) {
+				ret = CONJUGRAD_SUCCESS;
+				break;
+			}
+		}
+		double beta = (ggn - ggp) / gg;
+		if (beta < 0)
+			beta = 0;
+		if (ccm_launch_direction(sdir, g, (float)beta, n, c.s)) {
+			ret = CONJUGRADERR_DEVICE;
+			goto done;
+		}
+		double dgn;
+		if (dots(&c, g, sdir, NULL, NULL, 1, &dgn)) {
+			ret = CONJUGRADERR_DEVICE;
+			goto done;
+		}
+		if (dgn >= 0) {
+			if (ccm_launch_negate(sdir, g, n, c.s)) {
+				ret = CONJUGRADERR_DEVICE;
+				goto done;
+			}
+			dgn = -ggn;
+			steepest = 1;
+		} else {
+			steepest = beta == 0.0;
+		}
+		alpha = alpha * dg0 / dgn;
+		dg0 = dgn;
+		gg = ggn;
+	}
+done:
+	if (c.s)
+		(void)hipStreamSynchronize(c.s);
+	*fx = f;
+	if (iterations)
+		*iterations = it;
+	if (evaluations)
+		*evaluations = nevals;
+	if (g)
+		(void)hipFree(g);
+	if (gprev)
+		(void)hipFree(gprev);
+	if (sdir)
+		(void)hipFree(sdir);
+	if (c.d_part)
+		(void)hipFree(c.d_part);
+	if (c.d_out)
+		(void)hipFree(c.d_out);
+	if (c.h_out)
+		(void)hipHostFree(c.h_out);
+	if (c.s)
+		(void)hipStreamDestroy(c.s);
+	free(hist);
+	return ret;
+}
+
+int ccm_conjugrad_gpu(int n, conjugrad_float_t *d_x, conjugrad_float_t *fx,
+                      conjugrad_evaluate_t evaluate, conjugrad_progress_t progress, void *instance,
+                      const ccm_conjugrad_parameter *param)
+{
+	double f = 0;
+	int r = ccm_conjugrad_gpu_ex(n, d_x, &f, evaluate, progress, instance, param, NULL, NULL,
+	                             NULL);
+	*fx = (conjugrad_float_t)f;
+	return r;
+}

@@ -1,0 +1,68 @@
+// Shared device helpers for the gfx950 kernels of the PLL path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define CCM_NA 21   // 20 amino acids + gap (gap index 20)
+#define CCM_NAP 32  // padded alphabet of the reference's default PADDING build
+
+// Status codes returned by every launcher (0 = success).
+#define CCM_OK 0
+#define CCM_ERR_ARG -2
+#define CCM_ERR_LAUNCH -3
+
+#define CCM_LAUNCH_CHECK()                                  \
+  do {                                                      \
+    hipError_t e_ = hipGetLastError();                      \
+    if (e_ != hipSuccess) return CCM_ERR_LAUNCH;            \
+  } while (0)
+
+// 8 XCDs on MI355X: consecutive workgroup ids are dealt round-robin to the
+// XCDs.  Remap a hardware id so that the workgroups one XCD runs back to back
+// get consecutive logical ids (and therefore share operand tiles in that
+// XCD's L2).  `total` must be a multiple of 8; the grid is padded to it.
+__device__ __forceinline__ uint32_t ccm_xcd_remap(uint32_t hw, uint32_t total) {
+  const uint32_t per = total >> 3;
+  return (hw & 7u) * per + (hw >> 3);
+}
+
+__device__ __forceinline__ double ccm_wave_sum_f64(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ float ccm_wave_sum_f32(float v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Block-wide FP64 sum with a fixed reduction order; result valid in thread 0.
+template <int NT>
+__device__ __forceinline__ double ccm_block_sum_f64(double v, double* lds) {
+  v = ccm_wave_sum_f64(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < NT / 64; i++) t += lds[i];
+  }
+  __syncthreads();
+  return t;
+}
+
+// float -> bf16 (round to nearest even), as raw bits
+__device__ __forceinline__ uint16_t ccm_f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  uint32_t r = 0x7FFFu + ((u >> 16) & 1u);
+  if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+  return (uint16_t)((u + r) >> 16);
+}
+
+__device__ __forceinline__ float ccm_bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+static inline long ccm_nsp(int L) {
+  long ns = (long)L * (CCM_NA - 1);
+  return ns + CCM_NAP - (ns % CCM_NAP);
+}

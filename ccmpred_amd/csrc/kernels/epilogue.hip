@@ -1,0 +1,205 @@
+// Gradient assembly and the objective's global sum.
+//
+// Pairs (evaluate_cpu.c:131-163): for every padded pair entry (b,k,a,j)
+//   g2 = M(b,k,a,j) + M(a,j,b,k) [+ hsym(b,k,a,j)] for a < 21 and k != j,
+//   g2 = 0 for self-edges (k == j) and for padding (a >= 21),
+//   then g2 += 2 lambda_pair x and reg += 0.5 lambda_pair x^2 over the whole
+//   padded block.  M is the raw (unsymmetrised) pair gradient in either the
+//   FP32 layout (Lm = L) or the padded BF16 layout (Lm = Lp).
+// Singles (evaluate_cpu.c:102-109,153-156): g1 = sum of per-block partials
+//   + 2 lambda_single V for the 20L real singles, 0 on the padding.
+#include "common.h"
+
+#define EP_T 64
+
+__global__ __launch_bounds__(256) void k_pair_epilogue(const float* __restrict__ M, long ldm, int Lm,
+                                                       const float* __restrict__ hsym,
+                                                       const float* __restrict__ x2,
+                                                       float* __restrict__ g2, int L, float lam2,
+                                                       uint32_t tiles_c, double* __restrict__ regpart) {
+  __shared__ float T[EP_T][EP_T + 1];
+  __shared__ double red[4];
+  const int tr = blockIdx.x / tiles_c, tc = blockIdx.x % tiles_c;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int C21 = CCM_NA * L, ld2 = CCM_NAP * L;
+  const float invL = 1.0f / (float)L;
+
+  // transposed operand M(a,j,b,k): rows of the LDS tile follow the output
+  // column (a,j), columns follow the output row (b,k)
+  {
+    const int R = tr * EP_T + lane;
+    const int b = __float2int_rd(((float)R + 0.5f) * invL);
+    const int k = R - b * L;
+    for (int cc = wv; cc < EP_T; cc += 4) {
+      const int Cp = tc * EP_T + cc;
+      float v = 0.f;
+      if (Cp < C21 && R < C21) {
+        const int a = __float2int_rd(((float)Cp + 0.5f) * invL);
+        const int j = Cp - a * L;
+        v = M[(long)(a * Lm + j) * ldm + b * Lm + k];
+      }
+      T[cc][lane] = v;
+    }
+  }
+  __syncthreads();
+
+  const int Cp = tc * EP_T + lane;
+  const int a = __float2int_rd(((float)Cp + 0.5f) * invL);
+  const int j = Cp - a * L;
+  const bool pair = Cp < C21;
+  const bool inrange = Cp < ld2;
+  const float two_l = 2.0f * lam2, half_l = 0.5f * lam2;
+  double reg = 0.0;
+  for (int rr = wv; rr < EP_T; rr += 4) {
+    const int R = tr * EP_T + rr;
+    if (R >= C21 || !inrange) continue;
+    const int b = __float2int_rd(((float)R + 0.5f) * invL);
+    const int k = R - b * L;
+    const long o = (long)R * ld2 + Cp;
+    const float xv = x2[o];
+    float v = 0.f;
+    if (pair && k != j) {
+      v = M[(long)(b * Lm + k) * ldm + a * Lm + j] + T[lane][rr];
+      if (hsym) v += hsym[o];
+    }
+    g2[o] = v + two_l * xv;
+    reg += (double)(half_l * xv * xv);
+  }
+  reg = ccm_wave_sum_f64(reg);
+  if (lane == 0) red[wv] = reg;
+  __syncthreads();
+  if (threadIdx.x == 0) regpart[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// symmetrised -2 H2 in the x2 layout with zero self-edge blocks and padding:
+// hsym(b,k,a,j) = Mh(b,k,a,j) + Mh(a,j,b,k) with Mh = -H2 (FP32 layout)
+__global__ __launch_bounds__(256) void k_hsym(const float* __restrict__ Mh, long ldm,
+                                              float* __restrict__ hsym, int L, uint32_t tiles_c) {
+  __shared__ float T[EP_T][EP_T + 1];
+  const int tr = blockIdx.x / tiles_c, tc = blockIdx.x % tiles_c;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int C21 = CCM_NA * L, ld2 = CCM_NAP * L;
+  const float invL = 1.0f / (float)L;
+  {
+    const int R = tr * EP_T + lane;
+    for (int cc = wv; cc < EP_T; cc += 4) {
+      const int Cp = tc * EP_T + cc;
+      T[cc][lane] = (Cp < C21 && R < C21) ? Mh[(long)Cp * ldm + R] : 0.f;
+    }
+  }
+  __syncthreads();
+  const int Cp = tc * EP_T + lane;
+  const int a = __float2int_rd(((float)Cp + 0.5f) * invL);
+  const int j = Cp - a * L;
+  for (int rr = wv; rr < EP_T; rr += 4) {
+    const int R = tr * EP_T + rr;
+    if (R >= C21 || Cp >= ld2) continue;
+    const int b = __float2int_rd(((float)R + 0.5f) * invL);
+    const int k = R - b * L;
+    float v = 0.f;
+    if (Cp < C21 && k != j) v = Mh[(long)R * ldm + Cp] + T[lane][rr];
+    hsym[(long)R * ld2 + Cp] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_singles_finalize(const double* __restrict__ g1part, int nblk,
+                                                          const float* __restrict__ x1,
+                                                          float* __restrict__ g1, int L, long nsp,
+                                                          float lam1, double* __restrict__ regpart) {
+  __shared__ double red[4];
+  const long v = (long)blockIdx.x * 256 + threadIdx.x;
+  const long ns = 20L * L;
+  double reg = 0.0;
+  if (v < ns) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += g1part[(long)b * ns + v];
+    const float xv = x1[v];
+    g1[v] = (float)s + (2.0f * lam1) * xv;
+    reg = (double)(lam1 * xv * xv);
+  } else if (v < nsp) {
+    g1[v] = 0.f;
+  }
+  reg = ccm_wave_sum_f64(reg);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = reg;
+  __syncthreads();
+  if (threadIdx.x == 0) regpart[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Neumaier-compensated FP64 sum of up to three partial arrays, fixed order.
+// out[0] = total (double)
+__global__ __launch_bounds__(256) void k_sum3(const double* __restrict__ a, int na,
+                                              const double* __restrict__ b, int nb,
+                                              const double* __restrict__ c, int nc,
+                                              double* __restrict__ out) {
+  __shared__ double sh[256], shc[256];
+  double s = 0.0, comp = 0.0;
+  auto add = [&](double v) {
+    const double t = s + v;
+    if (fabs(s) >= fabs(v))
+      comp += (s - t) + v;
+    else
+      comp += (v - t) + s;
+    s = t;
+  };
+  for (int i = threadIdx.x; i < na; i += 256) add(a[i]);
+  for (int i = threadIdx.x; i < nb; i += 256) add(b[i]);
+  for (int i = threadIdx.x; i < nc; i += 256) add(c[i]);
+  sh[threadIdx.x] = s;
+  shc[threadIdx.x] = comp;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s = 0.0;
+    comp = 0.0;
+    for (int i = 0; i < 256; i++) {
+      add(sh[i]);
+      comp += shc[i];
+    }
+    out[0] = s + comp;
+  }
+}
+
+extern "C" {
+
+int ccm_pair_epilogue_blocks(int L) {
+  const int tr = (CCM_NA * L + EP_T - 1) / EP_T, tc = (CCM_NAP * L + EP_T - 1) / EP_T;
+  return tr * tc;
+}
+
+int ccm_singles_blocks(int L) { return (int)((ccm_nsp(L) + 255) / 256); }
+
+int ccm_launch_pair_epilogue(const float* d_m, long ldm, int Lm, const float* d_hsym, const float* d_x,
+                             float* d_g, int L, float lam2, double* d_regpart, hipStream_t s) {
+  const long nsp = ccm_nsp(L);
+  const uint32_t tr = (CCM_NA * L + EP_T - 1) / EP_T, tc = (CCM_NAP * L + EP_T - 1) / EP_T;
+  hipLaunchKernelGGL(k_pair_epilogue, dim3(tr * tc), dim3(256), 0, s, d_m, ldm, Lm, d_hsym, d_x + nsp,
+                     d_g + nsp, L, lam2, tc, d_regpart);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+// d_hsym: pair block (21L x 32L floats) in the x2 layout
+int ccm_launch_hsym(const float* d_mh, long ldm, float* d_hsym, int L, hipStream_t s) {
+  const uint32_t tr = (CCM_NA * L + EP_T - 1) / EP_T, tc = (CCM_NAP * L + EP_T - 1) / EP_T;
+  hipLaunchKernelGGL(k_hsym, dim3(tr * tc), dim3(256), 0, s, d_mh, ldm, d_hsym, L, tc);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_singles_finalize(const double* d_g1part, int nblk, const float* d_x, float* d_g, int L,
+                                float lam1, double* d_regpart, hipStream_t s) {
+  const long nsp = ccm_nsp(L);
+  hipLaunchKernelGGL(k_singles_finalize, dim3((unsigned)((nsp + 255) / 256)), dim3(256), 0, s, d_g1part,
+                     nblk, d_x, d_g, L, nsp, lam1, d_regpart);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_sum3(const double* a, int na, const double* b, int nb, const double* c, int nc,
+                    double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum3, dim3(1), dim3(256), 0, s, a, na, b, nb, c, nc, out);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,225 @@
+// Plain FP32 one-hot x coupling contractions of the PLL path.
+//
+// Forward (evaluate_cpu.c:55-75, d_compute_pc cu:30-107):
+//   Z[n][a*L+j] = [a<20] V(j,a) + sum_k W(x_nk, k, a, j)
+// Every x_nk selects one coupling row (x_nk, k) of 21L floats; the kernel
+// stages, per chunk of 4 columns k, all 21 candidate rows of a 128-column
+// slab in LDS and lets each wave gather them for 64 sequences.
+//
+// Pair gradient (evaluate_cpu.c:112-126, d_compute_edge_gradients cu:279-312):
+//   G[(b*L+k)][c] = sum_n [x_nk = b] D[n][c],  D = w_n (P_n - onehot_n)
+// Each wave owns 4 columns k and scatters D rows into wave-private LDS
+// accumulators of 21 x 64 columns per k, so the result is deterministic.
+#include "common.h"
+
+#define GF_SLAB 128  // forward: columns per workgroup (2 per lane)
+#define GF_TK 4      // forward: columns k per LDS chunk
+#define GF_SEQ 64    // forward: sequences per wave
+
+__global__ __launch_bounds__(256) void k_gather_fwd_f32(const uint8_t* __restrict__ msap, int Lp,
+                                                        const float* __restrict__ x1,
+                                                        const float* __restrict__ x2, int L, int N,
+                                                        float* __restrict__ Z, long ldz,
+                                                        uint32_t tiles_n, uint32_t tiles,
+                                                        uint32_t total) {
+  __shared__ float2 wt[GF_TK * CCM_NA * 64];  // [kk][b][lane]
+  __shared__ uint4 offs[256];                 // per sequence: byte offsets for kk = 0..3
+  const uint32_t q = ccm_xcd_remap(blockIdx.x, total);
+  if (q >= tiles) return;
+  const int ct = q / tiles_n, nt = q % tiles_n;
+  const int c0 = ct * GF_SLAB, n0 = nt * 256;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const long C21 = (long)CCM_NA * L, ld2 = (long)CCM_NAP * L;
+  const int c = c0 + 2 * lane;
+
+  // seed with the singles (a < 20), gap block starts at 0
+  float2 init = make_float2(0.f, 0.f);
+  if (c < 20L * L) init.x = x1[c];
+  if (c + 1 < 20L * L) init.y = x1[c + 1];
+  float2 acc[GF_SEQ];
+#pragma unroll
+  for (int i = 0; i < GF_SEQ; i++) acc[i] = init;
+
+  for (int k0 = 0; k0 < L; k0 += GF_TK) {
+    __syncthreads();
+    for (int e = tid; e < GF_TK * CCM_NA * 64; e += 256) {
+      const int kk = e / (CCM_NA * 64), r = e % (CCM_NA * 64);
+      const int b = r / 64, cc = c0 + 2 * (r % 64);
+      const int k = k0 + kk;
+      float2 v = make_float2(0.f, 0.f);
+      if (k < L) {
+        const float* src = x2 + (long)(b * L + k) * ld2 + cc;
+        if (cc + 1 < C21) {
+          v = *reinterpret_cast<const float2*>(src);
+        } else if (cc < C21) {
+          v.x = src[0];
+        }
+      }
+      wt[e] = v;
+    }
+    {
+      const int n = n0 + tid;
+      uint32_t o[GF_TK];
+#pragma unroll
+      for (int kk = 0; kk < GF_TK; kk++) {
+        const int k = k0 + kk;
+        uint32_t x = (n < N && k < L) ? (uint32_t)msap[(long)n * Lp + k] : 0u;
+        o[kk] = (uint32_t)((kk * CCM_NA + x) * 64) * 8u;
+      }
+      offs[tid] = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+    __syncthreads();
+    const char* base = reinterpret_cast<const char*>(wt) + lane * 8;
+#pragma unroll
+    for (int i = 0; i < GF_SEQ; i++) {
+      const uint4 o = offs[wv * GF_SEQ + i];
+      const float2 v0 = *reinterpret_cast<const float2*>(base + o.x);
+      const float2 v1 = *reinterpret_cast<const float2*>(base + o.y);
+      const float2 v2 = *reinterpret_cast<const float2*>(base + o.z);
+      const float2 v3 = *reinterpret_cast<const float2*>(base + o.w);
+      float2 a = acc[i];
+      a.x += v0.x; a.y += v0.y;
+      a.x += v1.x; a.y += v1.y;
+      a.x += v2.x; a.y += v2.y;
+      a.x += v3.x; a.y += v3.y;
+      acc[i] = a;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < GF_SEQ; i++) {
+    const int n = n0 + wv * GF_SEQ + i;
+    if (n < N) {
+      float* dst = Z + (long)n * ldz + c;
+      if (c + 1 < C21) {
+        *reinterpret_cast<float2*>(dst) = acc[i];
+      } else if (c < C21) {
+        dst[0] = acc[i].x;
+      }
+    }
+  }
+}
+
+#define GG_KW 4  // gradient: columns k per wave (16 per workgroup)
+
+__global__ __launch_bounds__(256) void k_gather_grad_f32(const float* __restrict__ D, long ldd,
+                                                         const uint8_t* __restrict__ msap, int Lp,
+                                                         int L, int N, float* __restrict__ G,
+                                                         long ldg, uint32_t tiles_k, uint32_t tiles,
+                                                         uint32_t total) {
+  // wave-private accumulators, one array per k slot so the compiler may
+  // overlap their read-modify-write chains
+  __shared__ float a0[4][CCM_NA * 64];
+  __shared__ float a1[4][CCM_NA * 64];
+  __shared__ float a2[4][CCM_NA * 64];
+  __shared__ float a3[4][CCM_NA * 64];
+  const uint32_t q = ccm_xcd_remap(blockIdx.x, total);
+  if (q >= tiles) return;
+  const int cs = q / tiles_k, kt = q % tiles_k;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const long C21 = (long)CCM_NA * L;
+  const int c = cs * 64 + lane;
+  const int kb = kt * 16 + wv * GG_KW;
+  const bool cok = c < C21;
+
+  float* s0 = a0[wv];
+  float* s1 = a1[wv];
+  float* s2 = a2[wv];
+  float* s3 = a3[wv];
+  for (int b = 0; b < CCM_NA; b++) {
+    s0[b * 64 + lane] = 0.f;
+    s1[b * 64 + lane] = 0.f;
+    s2[b * 64 + lane] = 0.f;
+    s3[b * 64 + lane] = 0.f;
+  }
+  // the wave's 4 columns k are consecutive; msap rows are 32-byte aligned
+  const bool k0ok = kb < L, k1ok = kb + 1 < L, k2ok = kb + 2 < L, k3ok = kb + 3 < L;
+  if (k0ok) {
+    for (int n0 = 0; n0 < N; n0 += 8) {
+      float d[8];
+      uint32_t xs[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const int n = n0 + i;
+        d[i] = (n < N && cok) ? D[(long)n * ldd + c] : 0.f;
+        xs[i] = (n < N) ? *reinterpret_cast<const uint32_t*>(msap + (long)n * Lp + kb) : 0u;
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const uint32_t x = xs[i];
+        const int o0 = (int)(x & 0xffu) * 64 + lane;
+        const int o1 = (int)((x >> 8) & 0xffu) * 64 + lane;
+        const int o2 = (int)((x >> 16) & 0xffu) * 64 + lane;
+        const int o3 = (int)(x >> 24) * 64 + lane;
+        s0[o0] += d[i];
+        if (k1ok) s1[o1] += d[i];
+        if (k2ok) s2[o2] += d[i];
+        if (k3ok) s3[o3] += d[i];
+      }
+    }
+  }
+  if (cok) {
+    for (int b = 0; b < CCM_NA; b++) {
+      if (k0ok) G[(long)(b * L + kb) * ldg + c] = s0[b * 64 + lane];
+      if (k1ok) G[(long)(b * L + kb + 1) * ldg + c] = s1[b * 64 + lane];
+      if (k2ok) G[(long)(b * L + kb + 2) * ldg + c] = s2[b * 64 + lane];
+      if (k3ok) G[(long)(b * L + kb + 3) * ldg + c] = s3[b * 64 + lane];
+    }
+  }
+}
+
+// D = -w_n * onehot_n: run through the gradient kernel this yields
+// G = -H2 with H2[(b,k)][(a,j)] = sum_n w_n [x_nk=b][x_nj=a], the data term
+// of the reference's init-time histograms (cu:350-387).
+__global__ void k_neg_weighted_onehot(const uint8_t* __restrict__ msap, int Lp, const float* __restrict__ w,
+                                      int L, int N, float* __restrict__ D, long ldd) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long C21 = (long)CCM_NA * L;
+  if (t >= (long)N * C21) return;
+  const int n = (int)(t / C21);
+  const int cidx = (int)(t % C21);
+  const int a = cidx / L, j = cidx % L;
+  D[(long)n * ldd + cidx] = (msap[(long)n * Lp + j] == a) ? -w[n] : 0.f;
+}
+
+static inline uint32_t round8(uint32_t v) { return (v + 7u) & ~7u; }
+
+extern "C" {
+
+// Z (N x ldz floats, ldz even and >= 21L) <- V + sum_k W rows
+int ccm_launch_gather_fwd_f32(const uint8_t* d_msap, int Lp, const float* d_x, int L, int N,
+                              float* d_z, long ldz, hipStream_t s) {
+  if (L <= 0 || N <= 0 || (ldz & 1) || ldz < 21L * L || (Lp % 32) || Lp < L) return CCM_ERR_ARG;
+  const float* x1 = d_x;
+  const float* x2 = d_x + ccm_nsp(L);
+  const uint32_t tiles_c = (uint32_t)((21L * L + GF_SLAB - 1) / GF_SLAB);
+  const uint32_t tiles_n = (uint32_t)((N + 255) / 256);
+  const uint32_t tiles = tiles_c * tiles_n, total = round8(tiles);
+  hipLaunchKernelGGL(k_gather_fwd_f32, dim3(total), dim3(256), 0, s, d_msap, Lp, x1, x2, L, N, d_z,
+                     ldz, tiles_n, tiles, total);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+// G (21L x ldg floats) <- sum_n onehot_n^T D_n ; D is N x ldd
+int ccm_launch_gather_grad_f32(const float* d_d, long ldd, const uint8_t* d_msap, int Lp, int L,
+                               int N, float* d_g, long ldg, hipStream_t s) {
+  if (L <= 0 || N <= 0 || ldg < 21L * L || ldd < 21L * L || (Lp % 32) || Lp < L) return CCM_ERR_ARG;
+  const uint32_t tiles_c = (uint32_t)((21L * L + 63) / 64);
+  const uint32_t tiles_k = (uint32_t)((L + 15) / 16);
+  const uint32_t tiles = tiles_c * tiles_k, total = round8(tiles);
+  hipLaunchKernelGGL(k_gather_grad_f32, dim3(total), dim3(256), 0, s, d_d, ldd, d_msap, Lp, L, N, d_g,
+                     ldg, tiles_k, tiles, total);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_neg_weighted_onehot(const uint8_t* d_msap, int Lp, const float* d_w, int L, int N,
+                                   float* d_d, long ldd, hipStream_t s) {
+  const long tot = (long)N * 21L * L;
+  hipLaunchKernelGGL(k_neg_weighted_onehot, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, d_msap,
+                     Lp, d_w, L, N, d_d, ldd);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,177 @@
+"""Python front end of the HIP PLL backend.
+
+Mirrors the reference's backend lifecycle (ccmpred.c:537-558 and
+evaluate_cuda.c): :class:`PLL` builds a ``userdata`` instance, ``init``
+computes the sequence weights on the device, :meth:`PLL.evaluate` is one call
+of the ``evaluate(instance, x, g, nvar)`` callback on device tensors, and
+:meth:`PLL.conjugrad` runs the device CG.  torch only provides device memory.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+PRECISIONS = {"fp32": (0, 1), "bf16": (1, 1), "bf16x2": (1, 2)}
+FP32_STEPS = ["gather_fwd", "softmax_rows", "gather_grad", "pair_epilogue", "singles_sum"]
+BF16_STEPS = ["convert_wt", "gemm_fwd", "softmax_cols", "gemm_grad", "pair_epilogue", "singles_sum"]
+
+
+def nsingle_padded(ncol: int) -> int:
+    return int(_lib.lib().ccm_nsingle_padded(int(ncol)))
+
+
+def nvar_padded(ncol: int) -> int:
+    return int(_lib.lib().ccm_nvar_padded(int(ncol)))
+
+
+def init_bias(msa: np.ndarray) -> np.ndarray:
+    """Host init_bias (ccmpred.c:90-131) from the library."""
+    msa = np.ascontiguousarray(msa, dtype=np.uint8)
+    nrow, ncol = msa.shape
+    x = np.zeros(nvar_padded(ncol), dtype=np.float32)
+    _lib.lib().ccm_init_bias(x.ctypes.data, msa.ctypes.data, ncol, nrow)
+    return x
+
+
+def contact_map(x: np.ndarray, ncol: int, apc: bool = True, normalize: bool = False) -> np.ndarray:
+    """sum_submatrices -> apc -> normalize (util.c:16-139) on a host x."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.zeros((ncol, ncol), dtype=np.float32)
+    L = _lib.lib()
+    L.ccm_sum_submatrices(x.ctypes.data, out.ctypes.data, int(ncol))
+    if apc:
+        L.ccm_apc(out.ctypes.data, int(ncol))
+    if normalize:
+        L.ccm_normalize(out.ctypes.data, int(ncol))
+    return out
+
+
+def top_pairs(mat: np.ndarray, num: int, min_sep: int = 6):
+    """Highest-scoring (i, j), j - i >= min_sep, ties by (i, j) ascending
+    (top_couplings.py:28-39 selection rule)."""
+    L = mat.shape[0]
+    ii, jj = np.triu_indices(L, k=min_sep)
+    vals = mat[ii, jj]
+    order = np.lexsort((jj, ii, -vals))[:num]
+    return [(int(ii[o]), int(jj[o])) for o in order]
+
+
+class PLL:
+    """One MSA on one GPU: weights, f+grad evaluation and the device CG."""
+
+    def __init__(self, msa, theta=0.8, lambda_single=0.01, lambda_pair=None,
+                 lambda_pair_factor=0.2, precision="fp32", device=0, verbose=False):
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {list(PRECISIONS)}")
+        self.msa = np.ascontiguousarray(msa, dtype=np.uint8)
+        self.nrow, self.ncol = self.msa.shape
+        if lambda_pair is None:
+            lambda_pair = lambda_pair_factor * (self.ncol - 1)
+        self.lambda_single = float(np.float32(lambda_single))
+        self.lambda_pair = float(np.float32(lambda_pair))
+        self.precision = precision
+        self.weights = np.zeros(self.nrow, dtype=np.float32)
+        self.nvar = nvar_padded(self.ncol)
+        self.nsingle_padded = nsingle_padded(self.ncol)
+        ud = _lib.UserData()
+        ud.msa = self.msa.ctypes.data_as(C.POINTER(C.c_ubyte))
+        ud.weights = self.weights.ctypes.data_as(C.POINTER(C.c_float))
+        ud.ncol = self.ncol
+        ud.nrow = self.nrow
+        ud.nsingle = self.ncol * 20
+        ud.nvar = ud.nsingle + self.ncol * self.ncol * 21 * 21
+        ud.lambda_single = self.lambda_single
+        ud.lambda_pair = self.lambda_pair
+        ud.reweighting_threshold = float(theta)
+        self._ud = ud
+        self._uptr = C.addressof(ud)
+        cfg = _lib.HipConfig()
+        _lib.lib().ccm_hip_default_config(C.byref(cfg))
+        cfg.device = int(device)
+        cfg.precision, cfg.bf16_passes = PRECISIONS[precision]
+        cfg.verbose = 1 if verbose else 0
+        rc = _lib.lib().init_hip_ex(self._uptr, C.byref(cfg))
+        if rc != 0:
+            raise RuntimeError("init_hip_ex failed")
+        self._open = True
+
+    # -- instance pointer for C callbacks
+    @property
+    def instance(self):
+        return C.cast(self._uptr, C.c_void_p)
+
+    @property
+    def stream_ptr(self) -> int:
+        return int(_lib.lib().ccm_hip_stream(self._uptr) or 0)
+
+    def evaluate(self, x, g=None):
+        """f (FP64) and the tied gradient for device tensor x (nvar float32)."""
+        import torch
+        if not (x.is_cuda and x.dtype == torch.float32 and x.numel() == self.nvar and x.is_contiguous()):
+            raise ValueError("x must be a contiguous CUDA float32 tensor of nvar_padded entries")
+        if g is None:
+            g = torch.empty_like(x)
+        torch.cuda.current_stream(x.device).synchronize()
+        f = C.c_double()
+        rc = _lib.lib().ccm_hip_evaluate(self._uptr, C.c_void_p(x.data_ptr()),
+                                         C.c_void_p(g.data_ptr()), self.nvar, C.byref(f))
+        if rc != 0:
+            raise RuntimeError("ccm_hip_evaluate failed")
+        return f.value, g
+
+    def evaluate_ptr(self, x_ptr: int, g_ptr: int) -> float:
+        """Raw device-pointer evaluate (no stream synchronisation before the call)."""
+        f = C.c_double()
+        rc = _lib.lib().ccm_hip_evaluate(self._uptr, C.c_void_p(x_ptr), C.c_void_p(g_ptr),
+                                         self.nvar, C.byref(f))
+        if rc != 0:
+            raise RuntimeError("ccm_hip_evaluate failed")
+        return f.value
+
+    def set_timing(self, on=True):
+        _lib.lib().ccm_hip_set_timing(self._uptr, 1 if on else 0)
+
+    def timing(self):
+        out = (C.c_double * 6)()
+        n = _lib.lib().ccm_hip_get_timing(self._uptr, out, 6)
+        names = FP32_STEPS if self.precision == "fp32" else BF16_STEPS
+        return n, {k: out[i] for i, k in enumerate(names)}
+
+    def conjugrad(self, x, max_iterations=50, epsilon=0.0, k=5, max_linesearch=5, alpha_mul=0.5,
+                  ftol=1e-4, wolfe=0.2):
+        """Device CG from x (modified in place); returns status and counters."""
+        import torch
+        p = _lib.CGParam()
+        _lib.lib().ccm_conjugrad_init(C.byref(p))
+        p.max_iterations, p.epsilon, p.k = int(max_iterations), float(epsilon), int(k)
+        p.max_linesearch, p.alpha_mul, p.ftol, p.wolfe = int(max_linesearch), alpha_mul, ftol, wolfe
+        torch.cuda.current_stream(x.device).synchronize()
+        fx = C.c_double()
+        it = C.c_int()
+        ne = C.c_int()
+        hist = (C.c_double * max(1, int(max_iterations)))()
+        ret = _lib.lib().ccm_conjugrad_gpu_ex(self.nvar, C.c_void_p(x.data_ptr()), C.byref(fx),
+                                              _lib.evaluate_fnptr(), None, self._uptr,
+                                              C.byref(p), C.byref(it), C.byref(ne), hist)
+        return dict(ret=ret, fx=fx.value, iterations=it.value, evaluations=ne.value,
+                    fhist=np.array(hist[: it.value]))
+
+    def close(self):
+        if getattr(self, "_open", False):
+            _lib.lib().destroy_hip(self._uptr)
+            self._open = False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

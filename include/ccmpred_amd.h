@@ -1,0 +1,172 @@
+/*
+ * ccmpred_amd -- MI355X (gfx950) implementation of CCMpred's
+ * pseudo-likelihood objective-and-gradient path, as a C ABI.
+ *
+ * The entry points mirror the reference's backend interface so that the
+ * reference driver can select this backend the way it selects its CUDA one
+ * (ccmpred.c:537-558): init / evaluate / destroy on a `userdata` instance,
+ * device-resident x and g in the reference's padded layout, and a
+ * conjugate-gradient driver with the libconjugrad call shape.
+ *
+ * Layout of x and g (ccmpred.h:11-17, ccmpred.c:387-390), nvar_padded floats:
+ *   singles  V(j,a) = x[a*L + j], a < 20, padded to nsingle_padded entries;
+ *   pairs    W(b,k,a,j) = x[nsingle_padded + ((b*L + k)*32 + a)*L + j].
+ */
+#ifndef CCMPRED_AMD_H
+#define CCMPRED_AMD_H
+
+#include <stdint.h>
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* conjugrad_float_t of the reference's default build (CONJUGRAD_FLOAT=32,
+ * CMakeLists.txt:57-63) */
+typedef float conjugrad_float_t;
+
+#define CCM_N_ALPHA 21
+#define CCM_N_ALPHA_PAD 32
+
+/* ccmpred.h:39-92, field for field.  `weights` is a host array of nrow
+ * floats that init_hip fills. */
+typedef struct userdata {
+	unsigned char *msa;
+	conjugrad_float_t *weights;
+	int ncol;
+	int nrow;
+	int nsingle;
+	int nvar;
+	void *extra;
+	conjugrad_float_t lambda_single;
+	conjugrad_float_t lambda_pair;
+	conjugrad_float_t reweighting_threshold;
+	void *meta_steps;
+} userdata;
+
+/* libconjugrad callback types (call sites ccmpred.c:50-61, 632) */
+typedef conjugrad_float_t (*conjugrad_evaluate_t)(void *instance, const conjugrad_float_t *x,
+                                                  conjugrad_float_t *g, const int nvar);
+typedef int (*conjugrad_progress_t)(void *instance, const conjugrad_float_t *x,
+                                    const conjugrad_float_t *g, const conjugrad_float_t fx,
+                                    const conjugrad_float_t xnorm, const conjugrad_float_t gnorm,
+                                    const conjugrad_float_t step, int n, int k, int ls);
+
+/* ---- sizes (ccmpred.c:385-390) ---------------------------------------- */
+long ccm_nsingle_padded(int ncol);
+long ccm_nvar_padded(int ncol);
+
+/* ---- backend lifecycle ------------------------------------------------ */
+
+enum { CCM_PRECISION_FP32 = 0, CCM_PRECISION_BF16 = 1 };
+
+typedef struct ccm_hip_config {
+	int device;      /* HIP device ordinal */
+	int precision;   /* CCM_PRECISION_FP32 (plain) or CCM_PRECISION_BF16 (MFMA) */
+	int bf16_passes; /* 1: BF16 operands; 2: hi+lo BF16 split of the dense operand */
+	int verbose;     /* 1: print the reweighting summary like the reference */
+} ccm_hip_config;
+
+void ccm_hip_default_config(ccm_hip_config *cfg);
+
+/* Replaces init_cuda (evaluate_cuda.c:77-139 / evaluate_cuda.h:16).
+ * Computes sequence weights on the device (reweighting.c semantics, bit
+ * exact), fills ud->weights, and allocates per-instance device state in
+ * ud->extra.  Returns 0 on success. */
+int init_hip(void *instance);
+int init_hip_ex(void *instance, const ccm_hip_config *cfg);
+int init_hip_bf16(void *instance);
+
+/* Replaces evaluate_cuda (evaluate_cuda.c:23-74 / evaluate_cuda.h:8-13).
+ * d_x, d_g: device pointers to nvar_padded floats.  Overwrites all of d_g
+ * with the tied gradient and returns f.  Returns NaN on error. */
+conjugrad_float_t evaluate_hip(void *instance, const conjugrad_float_t *d_x,
+                               conjugrad_float_t *d_g, const int nvar);
+
+/* Replaces destroy_cuda (evaluate_cuda.c:141-151). */
+int destroy_hip(void *instance);
+
+/* Same as evaluate_hip with the FP64 objective and a status code. */
+int ccm_hip_evaluate(void *instance, const float *d_x, float *d_g, int nvar, double *f);
+
+/* HIP stream the instance runs on (hipStream_t). */
+void *ccm_hip_stream(void *instance);
+
+/* Per-step device timing with HIP events on the instance stream (no
+ * counterpart in the reference).  set_timing(on=1) creates the events and
+ * clears the accumulators; get_timing fills up to n averages in ms and
+ * returns the number of timed evaluations.  Steps, FP32 path: gather_fwd,
+ * softmax, gather_grad, pair epilogue, singles+sum; BF16 path: convert_wt,
+ * gemm_fwd, softmax, gemm_grad, pair epilogue, singles+sum. */
+int ccm_hip_set_timing(void *instance, int on);
+int ccm_hip_get_timing(void *instance, double *avg_ms, int n);
+
+/* ---- device conjugate gradient (libconjugrad's conjugrad_gpu shape) --- */
+
+/* Status codes; same names as libconjugrad's, own values (the library is
+ * not vendored with the reference). */
+#define CONJUGRAD_SUCCESS 0
+#define CONJUGRAD_ALREADY_MINIMIZED 1
+#define CONJUGRADERR_MAXIMUMITERATION 2
+#define CONJUGRADERR_MAXIMUMLINESEARCH -1
+#define CONJUGRADERR_DEVICE -2
+
+typedef struct ccm_conjugrad_parameter {
+	int max_iterations;
+	double epsilon;
+	int k;
+	int max_linesearch;
+	double alpha_mul;
+	double ftol;
+	double wolfe;
+} ccm_conjugrad_parameter;
+
+/* ccmpred.c:526-534 defaults */
+void ccm_conjugrad_init(ccm_conjugrad_parameter *param);
+
+/* Nonlinear CG on device vectors (conjugrad_gpu at ccmpred.c:632).
+ * Polak-Ribiere+ direction, backtracking line search with an Armijo test
+ * (ftol) and a curvature test (wolfe), FP64 scalars and dot products.
+ * d_x: device pointer, n floats; *fx receives the final objective. */
+int ccm_conjugrad_gpu(int n, conjugrad_float_t *d_x, conjugrad_float_t *fx,
+                      conjugrad_evaluate_t evaluate, conjugrad_progress_t progress, void *instance,
+                      const ccm_conjugrad_parameter *param);
+
+/* Same with counters and the per-iteration objective history (fhist: at
+ * least max_iterations doubles, may be NULL). */
+int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_evaluate_t evaluate,
+                         conjugrad_progress_t progress, void *instance,
+                         const ccm_conjugrad_parameter *param, int *iterations, int *evaluations,
+                         double *fhist);
+
+/* ---- host model set-up and contact-map output ------------------------- */
+
+/* sequence.c:48-80: PSICOV reader; returns a malloc'd N x L index matrix,
+ * NULL on a FASTA/a3m header (the reference exits with status 100). */
+unsigned char *ccm_read_msa(FILE *f, int *ncol, int *nrow);
+
+/* ccmpred.c:90-131: singles from pseudo-counted column frequencies relative
+ * to the gap; couplings and padding zero.  x: nvar_padded floats (host). */
+void ccm_init_bias(float *x, const unsigned char *msa, int ncol, int nrow);
+
+/* util.c:16-64: Frobenius norm of each mean-centred 21x21 block over the
+ * 20x20 residue part, zero diagonal.  x: host, out: L x L. */
+void ccm_sum_submatrices(const float *x, float *out, int ncol);
+/* util.c:66-108 */
+void ccm_apc(float *mat, int ncol);
+/* util.c:110-139 */
+void ccm_normalize(float *mat, int ncol);
+/* io.c:20-28 */
+void ccm_write_matrix(FILE *out, const float *mat, int ncol, int nrow);
+/* io.c:31-60 */
+void ccm_write_raw(FILE *out, const float *x, int ncol);
+/* io.c:62-103; fills both mirrored couplings.  Returns 0, or -1 on a
+ * malformed file (the reference asserts). */
+int ccm_read_raw(const char *filename, int ncol, float *x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

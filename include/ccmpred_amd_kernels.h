@@ -1,0 +1,81 @@
+/*
+ * ccmpred_amd kernel launchers: one extern "C" function per device step of
+ * the PLL path, replacing the reference's gpu_* wrappers
+ * (evaluate_cuda_kernels.h:16-82, evaluate_cuda_kernels.cu:556-830).
+ *
+ * Unlike the reference wrappers, these take a stream, never synchronise,
+ * never allocate and never exit: they return 0 or a negative status, and
+ * every buffer is owned by the caller.  hipStream_t is passed as void*.
+ */
+#ifndef CCMPRED_AMD_KERNELS_H
+#define CCMPRED_AMD_KERNELS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *ccm_stream_t;
+
+/* reweighting (gpu_compute_weights_simple, cu:804-825) */
+int ccm_reweight_words(int L);
+int ccm_launch_identity_counts(const uint8_t *d_msa, int L, int N, int idthres, uint32_t *d_planes,
+                               int32_t *d_counts, ccm_stream_t s);
+int ccm_launch_weights_from_counts(const int32_t *d_counts, int N, int L, int idthres, float *d_w,
+                                   ccm_stream_t s);
+int ccm_launch_fill_f32(float *d_p, long n, float v, ccm_stream_t s);
+
+/* plain FP32 contractions (gpu_pc cu:560-590; gpu_compute_edge_gradients cu:643-666) */
+int ccm_launch_gather_fwd_f32(const uint8_t *d_msap, int Lp, const float *d_x, int L, int N,
+                              float *d_z, long ldz, ccm_stream_t s);
+int ccm_launch_gather_grad_f32(const float *d_d, long ldd, const uint8_t *d_msap, int Lp, int L,
+                               int N, float *d_g, long ldg, ccm_stream_t s);
+int ccm_launch_neg_weighted_onehot(const uint8_t *d_msap, int Lp, const float *d_w, int L, int N,
+                                   float *d_d, long ldd, ccm_stream_t s);
+
+/* log-sum-exp / objective terms (cu:80-103, cu:592-641) */
+int ccm_softmax_rows_blocks(int L, int N);
+int ccm_softmax_rows_nblk(int N);
+int ccm_softmax_cols_blocks(int L, int N);
+int ccm_softmax_cols_nblk(int N);
+int ccm_launch_softmax_rows(float *d_zd, long ldz, const uint8_t *d_msap, int Lp, const float *d_w,
+                            int L, int N, double *d_fpart, double *d_g1part, ccm_stream_t s);
+int ccm_launch_softmax_cols(const float *d_zt, long ldzt, const float *d_x1, const uint8_t *d_msapt,
+                            int Np, const float *d_w, int L, int Lp, int N, uint16_t *d_dt,
+                            long lddt, int passes, double *d_fpart, double *d_g1part,
+                            ccm_stream_t s);
+
+/* BF16 MFMA contractions */
+int ccm_launch_convert_wt(const float *d_x, int L, int Lp, uint16_t *d_wt, long lda, int passes,
+                          ccm_stream_t s);
+int ccm_launch_gemm_fwd(const uint16_t *d_wt, long lda, int Lp, int passes, const uint8_t *d_msap,
+                        int Np, float *d_zt, long ldc, ccm_stream_t s);
+int ccm_launch_gemm_grad(const uint16_t *d_dt, long lda, int Lp, int passes,
+                         const uint8_t *d_msapt, int Np, float *d_gt, long ldc, ccm_stream_t s);
+
+/* gradient assembly (cu:150-219, cu:390-461) and objective reduction */
+int ccm_pair_epilogue_blocks(int L);
+int ccm_singles_blocks(int L);
+int ccm_launch_pair_epilogue(const float *d_m, long ldm, int Lm, const float *d_hsym,
+                             const float *d_x, float *d_g, int L, float lam2, double *d_regpart,
+                             ccm_stream_t s);
+int ccm_launch_hsym(const float *d_mh, long ldm, float *d_hsym, int L, ccm_stream_t s);
+int ccm_launch_singles_finalize(const double *d_g1part, int nblk, const float *d_x, float *d_g,
+                                int L, float lam1, double *d_regpart, ccm_stream_t s);
+int ccm_launch_sum3(const double *a, int na, const double *b, int nb, const double *c, int nc,
+                    double *out, ccm_stream_t s);
+
+/* conjugate-gradient vector arithmetic */
+int ccm_dots_partial_len(void);
+int ccm_launch_dots(const float *a, const float *b0, const float *b1, const float *b2, long n,
+                    int nd, double *part, double *out, ccm_stream_t s);
+int ccm_launch_axpy(float *x, const float *sdir, float alpha, long n, ccm_stream_t s);
+int ccm_launch_direction(float *sdir, const float *g, float beta, long n, ccm_stream_t s);
+int ccm_launch_negate(float *sdir, const float *g, long n, ccm_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

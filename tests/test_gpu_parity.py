@@ -1,0 +1,222 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Tolerances: FP32 path f within 1e-5 relative and gradient within 1e-4
+relative L2 of the FP64 oracle; BF16 path gradient within 1e-3 relative L2;
+reweighting bit-exact; after 50 CG iterations the APC maps agree with
+Spearman >= 0.999 and identical top-L/5 pairs.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as ora  # noqa: E402
+from ccmpred_amd import pll as P  # noqa: E402
+from ccmpred_amd.msa import synthetic_msa  # noqa: E402
+from tests.pll_bruteforce import nsingle_padded, NA, NAP  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def sym_couplings(L, rng, scale=0.01):
+    nsp = nsingle_padded(L)
+    W = rng.normal(0, scale, (NA, L, NA, L))
+    W = W + W.transpose(2, 3, 0, 1)
+    for k in range(L):
+        W[:, k, :, k] = 0.0
+    x = np.zeros(nsp + NA * L * NAP * L)
+    X2 = x[nsp:].reshape(NA, L, NAP, L)
+    X2[:, :, :NA, :] = W
+    return x
+
+
+def make_x(msa, seed, scale=0.01):
+    L = msa.shape[1]
+    x = ora.init_bias(msa).astype(np.float64)
+    if scale:
+        x += sym_couplings(L, np.random.default_rng(seed), scale)
+    return x.astype(np.float32)
+
+
+# ---- reweighting ---------------------------------------------------------
+
+@pytest.mark.parametrize("L,N,seed,theta", [(75, 1000, 1, 0.8), (75, 1000, 1, 0.3),
+                                            (40, 300, 7, 0.8), (33, 257, 9, 0.6),
+                                            (75, 400, 2, 1.0), (20, 50, 3, 1.5),
+                                            (250, 5000, 2, 0.8)])
+def test_weights_bit_exact(L, N, seed, theta):
+    say(f"weights L={L} N={N} seed={seed} theta={theta}")
+    msa, _ = synthetic_msa(L, N, seed)
+    msa[N // 3: N // 3 + N // 10] = msa[0]          # force identity clusters
+    msa[N // 2: N // 2 + 5, : L // 2] = msa[1, : L // 2]
+    w_ref = ora.calculate_weights(msa, theta)
+    with P.PLL(msa, theta=theta) as p:
+        assert p.weights.tobytes() == w_ref.tobytes()
+
+
+# ---- FP32 plain path -----------------------------------------------------
+
+@pytest.mark.parametrize("L,N,seed,scale", [(75, 1000, 1, 0.0), (75, 1000, 1, 0.01),
+                                            (120, 2000, 5, 0.01), (13, 37, 4, 0.1),
+                                            (64, 129, 6, 0.05)])
+def test_fp32_evaluate_parity(L, N, seed, scale):
+    say(f"fp32 evaluate L={L} N={N} seed={seed} scale={scale}")
+    msa, _ = synthetic_msa(L, N, seed)
+    x = make_x(msa, seed + 100, scale)
+    with P.PLL(msa, precision="fp32") as p:
+        lam1, lam2 = p.lambda_single, p.lambda_pair
+        f64, g64 = ora.evaluate_f64(msa, p.weights.astype(np.float64), x.astype(np.float64), lam1, lam2)
+        f32, g32 = ora.evaluate_f32(msa, p.weights, x, lam1, lam2)
+        xd = torch.from_numpy(x).cuda()
+        f, g = p.evaluate(xd)
+        g = g.cpu().numpy()
+    say(f"  f={f:.10g} f64={f64:.10g} f32={f32:.10g} rel={abs(f - f64) / abs(f64):.2e} "
+        f"g_rel_l2={rel_l2(g, g64):.2e} oracle32_g_rel={rel_l2(g32, g64):.2e}")
+    assert abs(f - f64) / abs(f64) <= 1e-5
+    assert rel_l2(g, g64) <= 1e-4
+    assert rel_l2(g, g32.astype(np.float64)) <= 1e-4
+
+
+def test_fp32_deterministic_and_structured():
+    """Full-size workload L=250, N=5000: two evaluations are bitwise equal, the
+    pair gradient is symmetric, padding and self-edge gradients are 2*lam2*x."""
+    L, N = 250, 5000
+    say(f"fp32 structure L={L} N={N}")
+    msa, _ = synthetic_msa(L, N, 2, 60)
+    x = make_x(msa, 7, 0.01)
+    nsp = nsingle_padded(L)
+    with P.PLL(msa, precision="fp32") as p:
+        xd = torch.from_numpy(x).cuda()
+        f1, g1 = p.evaluate(xd)
+        f2, g2 = p.evaluate(xd)
+        assert f1 == f2 and torch.equal(g1, g2)
+        G = g1[nsp:].view(NA, L, NAP, L)
+        T = G[:, :, :NA, :]
+        lam2 = p.lambda_pair
+        assert torch.equal(T, T.permute(2, 3, 0, 1))
+        X = xd[nsp:].view(NA, L, NAP, L)
+        assert torch.equal(G[:, :, NA:, :], (2.0 * np.float32(lam2)) * X[:, :, NA:, :])
+        idx = torch.arange(L, device=G.device)
+        assert torch.all(G[:, idx, :, idx] == 0)
+        assert torch.all(g1[20 * L: nsp] == 0)
+
+
+def test_x_zero_closed_form_full_size():
+    """At x = 0: f = L ln21 sum(w), g1 = sum(w)/21 - C, g2 = (C_kb + C_ja)/21 - 2 H2
+    with H2 = A^T diag(w) A formed independently from the one-hot MSA."""
+    L, N = 250, 5000
+    say(f"x=0 closed form L={L} N={N}")
+    msa, _ = synthetic_msa(L, N, 2, 60)
+    nsp = nsingle_padded(L)
+    for prec in ("fp32", "bf16"):
+        with P.PLL(msa, precision=prec) as p:
+            w = p.weights.astype(np.float64)
+            xd = torch.zeros(p.nvar, dtype=torch.float32, device="cuda")
+            f, g = p.evaluate(xd)
+            W = w.sum()
+            assert abs(f - L * np.log(21.0) * W) / (L * np.log(21.0) * W) < 1e-6
+            A = torch.nn.functional.one_hot(torch.from_numpy(msa.astype(np.int64)), NA).double()
+            C = torch.einsum("n,nkb->kb", torch.from_numpy(w), A)     # (k, b)
+            g1 = g[: 20 * L].view(20, L).double().cpu()
+            assert torch.allclose(g1, (W / 21.0 - C[:, :20]).T, atol=1e-3, rtol=1e-5)
+            Ad = A.cuda().reshape(N, L * NA).float()                     # column (k, b)
+            H = (Ad * torch.from_numpy(w).float().cuda()[:, None]).T @ Ad   # (k,b) x (j,a)
+            H = H.view(L, NA, L, NA).permute(1, 0, 3, 2).double()        # (b,k,a,j)
+            Cd = C.cuda().T                                              # (b, k)
+            expect = (Cd[:, :, None, None] + Cd[None, None, :, :]) / 21.0 - 2 * H
+            idx = torch.arange(L, device="cuda")
+            expect[:, idx, :, idx] = 0
+            G = g[nsp:].view(NA, L, NAP, L)[:, :, :NA, :].double()
+            err = (G - expect).norm() / expect.norm()
+            say(f"  {prec}: f rel={abs(f - L * np.log(21.0) * W) / (L * np.log(21.0) * W):.2e} g2 rel={err:.2e}")
+            assert err < (1e-5 if prec == "fp32" else 1e-3)
+
+
+# ---- BF16 MFMA path ------------------------------------------------------
+
+@pytest.mark.parametrize("prec,tol", [("bf16", 1e-3), ("bf16x2", 1e-4)])
+@pytest.mark.parametrize("L,N,seed", [(75, 1000, 1), (120, 2000, 5), (37, 300, 3)])
+def test_bf16_evaluate_error(prec, tol, L, N, seed):
+    say(f"{prec} evaluate L={L} N={N} seed={seed}")
+    msa, _ = synthetic_msa(L, N, seed)
+    x = make_x(msa, seed + 100, 0.01)
+    with P.PLL(msa, precision=prec) as p:
+        f64, g64 = ora.evaluate_f64(msa, p.weights.astype(np.float64), x.astype(np.float64),
+                                    p.lambda_single, p.lambda_pair)
+        f, g = p.evaluate(torch.from_numpy(x).cuda())
+        g = g.cpu().numpy()
+    e = rel_l2(g, g64)
+    say(f"  f rel={abs(f - f64) / abs(f64):.2e} g rel_l2={e:.2e}")
+    assert abs(f - f64) / abs(f64) <= 1e-4
+    assert e <= tol
+
+
+# ---- device CG end to end ------------------------------------------------
+
+def _spearman(a, b):
+    from scipy.stats import spearmanr
+    iu = np.triu_indices(a.shape[0], 1)
+    return spearmanr(a[iu], b[iu]).correlation
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_cg_50_iterations_contacts(prec):
+    L, N = 75, 1000
+    say(f"cg50 {prec} L={L} N={N}")
+    msa, pairs = synthetic_msa(L, N, 1, 15)
+    x0 = ora.init_bias(msa)
+    with P.PLL(msa, precision=prec) as p:
+        host = ora.cg_pll_f32(msa, p.weights, x0, p.lambda_single, p.lambda_pair, max_iterations=50,
+                              epsilon=0.0)
+        xd = torch.from_numpy(x0.copy()).cuda()
+        dev = p.conjugrad(xd, max_iterations=50, epsilon=0.0)
+        xg = xd.cpu().numpy()
+    say(f"  host ret={host['ret']} it={host['iterations']} f={host['fx']:.6f} | "
+        f"dev ret={dev['ret']} it={dev['iterations']} f={dev['fx']:.6f}")
+    assert dev["iterations"] == 50 and host["iterations"] == 50
+    assert abs(dev["fx"] - host["fx"]) / host["fx"] < 1e-3
+    m_dev = P.contact_map(xg, L)
+    m_host = ora.apc(ora.sum_submatrices(host["x"], L))
+    rho = _spearman(m_dev, m_host)
+    top_dev = P.top_pairs(m_dev, L // 5, 6)
+    top_host = P.top_pairs(m_host, L // 5, 6)
+    say(f"  spearman={rho:.6f} top-L/5 equal={top_dev == top_host}")
+    assert rho >= 0.999
+    assert top_dev == top_host
+
+
+def test_cli_1atz(tmp_path):
+    """The reference's only test: the contact matrix of example/1atzA.aln has 75 lines."""
+    out = tmp_path / "1atzA.mat"
+    raw = tmp_path / "1atzA.raw"
+    exe = os.path.join(ROOT, "ccmpred_amd", "bin", "ccmpred")
+    say("cli 1atzA")
+    r = subprocess.run([exe, "-n", "20", "-r", str(raw), os.path.join(ROOT, "tests", "golden", "1atzA.aln"),
+                        str(out)], capture_output=True, text=True, timeout=300)
+    sys.stdout.write(r.stdout[-2000:])
+    assert r.returncode == 0, r.stderr
+    lines = out.read_text().splitlines()
+    assert len(lines) == 75
+    mat = np.array([[float(v) for v in ln.split()] for ln in lines])
+    assert mat.shape == (75, 75) and np.all(np.diag(mat) == 0)
+    assert np.allclose(mat, mat.T, atol=1e-5)
