@@ -35,7 +35,7 @@ class UserData(C.Structure):
 
 class HipConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("bf16_passes", C.c_int),
-                ("verbose", C.c_int)]
+                ("verbose", C.c_int), ("chunk_rows", C.c_int)]
 
 
 class CGParam(C.Structure):
@@ -52,7 +52,8 @@ PROGRESS_T = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
 EXPORTED = [
     "ccm_nsingle_padded", "ccm_nvar_padded", "ccm_hip_default_config", "init_hip", "init_hip_ex",
     "init_hip_bf16", "evaluate_hip", "destroy_hip", "ccm_hip_evaluate", "ccm_hip_stream",
-    "ccm_hip_set_timing", "ccm_hip_get_timing",
+    "ccm_hip_set_timing", "ccm_hip_get_timing", "ccm_hip_objective", "ccm_hip_forward",
+    "ccm_hip_backward", "ccm_hip_device_bytes", "ccm_reg_only_blocks", "ccm_launch_reg_only",
     "ccm_conjugrad_init", "ccm_conjugrad_gpu", "ccm_conjugrad_gpu_ex", "ccm_read_msa",
     "ccm_init_bias", "ccm_sum_submatrices", "ccm_apc", "ccm_normalize", "ccm_write_matrix",
     "ccm_write_raw", "ccm_read_raw",
@@ -101,6 +102,14 @@ def _declare(L):
     L.ccm_hip_evaluate.restype = ip
     L.ccm_hip_stream.argtypes = [vp]
     L.ccm_hip_stream.restype = vp
+    L.ccm_hip_objective.argtypes = [vp, vp, ip, C.POINTER(C.c_double)]
+    L.ccm_hip_objective.restype = ip
+    L.ccm_hip_forward.argtypes = [vp, vp, ip, C.POINTER(C.c_double)]
+    L.ccm_hip_forward.restype = ip
+    L.ccm_hip_backward.argtypes = [vp, vp, vp, ip]
+    L.ccm_hip_backward.restype = ip
+    L.ccm_hip_device_bytes.argtypes = [vp]
+    L.ccm_hip_device_bytes.restype = C.c_size_t
     L.ccm_hip_set_timing.argtypes = [vp, ip]
     L.ccm_hip_set_timing.restype = ip
     L.ccm_hip_get_timing.argtypes = [vp, C.POINTER(C.c_double), ip]

@@ -8,17 +8,20 @@
  * stream with a single synchronisation for the f read-back, and init fills
  * the host weight array.
  *
- * Plain FP32 path per evaluation:
+ * Sequences are processed in chunks (all of them by default): per chunk the
+ * logits and probabilities are formed and the pair gradient accumulated, so
+ * the N x 21L intermediate never exceeds one chunk.
+ *
+ * Plain FP32 path, per chunk:
  *   gather_fwd   Z[n][a*L+j] = V + sum_k W(x_nk,k,a,j)
  *   softmax_rows D = w (P - onehot) in place; f and g1 partials
- *   gather_grad  G[(b,k)][(a,j)] = sum_n [x_nk=b] D[n][(a,j)]
- *   epilogue     g2 = G + G^T, self-edges 0, + 2 lambda_pair x
- * BF16 path per evaluation:
- *   convert_wt   Wt = bf16(W)^T in the padded (21*Lp)^2 index space
+ *   gather_grad  G[(b,k)][(a,j)] (+)= sum_n [x_nk=b] D[n][(a,j)]
+ * then epilogue  g2 = G + G^T, self-edges 0, + 2 lambda_pair x.
+ * BF16 path: convert_wt Wt = bf16(W)^T once, then per chunk
  *   gemm_fwd     Zt[c][n] on MFMA with the one-hot operand built in registers
  *   softmax_cols Dt = bf16(w P) (+ lo half); f and g1 partials
- *   gemm_grad    Gt[c][r] = sum_n Dt[c][n] [x_nk=b] on MFMA
- *   epilogue     g2 = Gt + Gt^T + hsym + 2 lambda_pair x, hsym = -2 H2 exact
+ *   gemm_grad    Gt[c][r] (+)= sum_n Dt[c][n] [x_nk=b] on MFMA
+ * then epilogue  g2 = Gt + Gt^T + hsym + 2 lambda_pair x, hsym = -2 H2 exact.
  */
 #define __HIP_PLATFORM_AMD__
 #include <hip/hip_runtime_api.h>
@@ -37,21 +40,24 @@ typedef struct ccm_hip_state {
 	ccm_hip_config cfg;
 	hipStream_t stream;
 	int L, N, Lp, P, Np;
+	int chunk;  /* sequences per chunk, multiple of 256 (>= N when not chunked) */
+	int ccols;  /* padded width of one chunk (multiple of 128) */
 	long nsp, nvp;
 	float lam1, lam2;
+	size_t bytes; /* device bytes held by this instance */
 	unsigned char *d_msap;  /* Np x Lp, 255 past the real rows/columns */
 	unsigned char *d_msapt; /* Lp x Np, 255 past the real rows/columns */
 	float *d_w;
 	/* plain FP32 path */
-	float *d_zd;
+	float *d_zd; /* chunk x ldz */
 	long ldz;
 	float *d_graw;
 	long ldg;
 	/* BF16 path */
 	uint16_t *d_wt;
 	long ldwt;
-	float *d_zt;
-	uint16_t *d_dt;
+	float *d_zt;    /* P x ccols */
+	uint16_t *d_dt; /* P x passes*ccols */
 	long lddt;
 	float *d_gt;
 	float *d_hsym;
@@ -66,6 +72,7 @@ typedef struct ccm_hip_state {
 	int n_regsingle;
 	double *d_f;
 	double *h_f;
+	int forward_valid; /* ccm_hip_forward left probabilities for ccm_hip_backward */
 	/* optional per-step timing with HIP events on the instance stream */
 	int timing;
 	hipEvent_t ev[CCM_TIMING_STEPS + 1];
@@ -110,6 +117,15 @@ void ccm_hip_default_config(ccm_hip_config *cfg)
 	cfg->precision = CCM_PRECISION_FP32;
 	cfg->bf16_passes = 1;
 	cfg->verbose = 1;
+	cfg->chunk_rows = 0;
+}
+
+static hipError_t dalloc(ccm_hip_state *st, void **p, size_t bytes)
+{
+	hipError_t e = hipMalloc(p, bytes);
+	if (e == hipSuccess)
+		st->bytes += bytes;
+	return e;
 }
 
 static void free_state(ccm_hip_state *st)
@@ -149,12 +165,13 @@ int init_hip_ex(void *instance, const ccm_hip_config *cfg_in)
 		cfg = *cfg_in;
 	else
 		ccm_hip_default_config(&cfg);
-	if (cfg.bf16_passes < 1 || cfg.bf16_passes > 2 || ud->ncol <= 0 || ud->nrow <= 0) {
+	if (cfg.bf16_passes < 1 || cfg.bf16_passes > 2 || cfg.chunk_rows < 0 || ud->ncol <= 0 ||
+	    ud->nrow <= 0) {
 		fprintf(stderr, "ccmpred_amd: invalid configuration\n");
 		return 1;
 	}
 	ccm_hip_state *st = (ccm_hip_state *)calloc(1, sizeof(*st));
-	unsigned char *h_msap = NULL, *h_msapt = NULL;
+	unsigned char *h_msap = NULL, *h_msapt = NULL, *d_msa_rows = NULL;
 	uint32_t *d_planes = NULL;
 	int32_t *d_counts = NULL;
 	float *d_tmp_d = NULL, *d_tmp_g = NULL;
@@ -171,7 +188,12 @@ int init_hip_ex(void *instance, const ccm_hip_config *cfg_in)
 	st->nvp = ccm_nvar_padded(L);
 	st->lam1 = ud->lambda_single;
 	st->lam2 = ud->lambda_pair;
+	st->chunk = cfg.chunk_rows > 0 ? (int)round_up(cfg.chunk_rows, 256) : (int)round_up(N, 256);
+	if (st->chunk >= N)
+		st->chunk = (int)round_up(N, 256);
+	st->ccols = (int)round_up(st->chunk < N ? st->chunk : N, 128);
 	const int Lp = st->Lp, Np = st->Np, P = st->P;
+	const int chunk_rows = st->chunk < N ? st->chunk : N;
 
 	HIPCHK(hipSetDevice(cfg.device));
 	HIPCHK(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
@@ -190,31 +212,27 @@ int init_hip_ex(void *instance, const ccm_hip_config *cfg_in)
 			h_msap[n * Lp + k] = v;
 			h_msapt[k * Np + n] = v;
 		}
-	HIPCHK(hipMalloc((void **)&st->d_msap, (size_t)Np * Lp));
-	HIPCHK(hipMalloc((void **)&st->d_msapt, (size_t)Lp * Np));
+	HIPCHK(dalloc(st, (void **)&st->d_msap, (size_t)Np * Lp));
+	HIPCHK(dalloc(st, (void **)&st->d_msapt, (size_t)Lp * Np));
 	HIPCHK(hipMemcpyAsync(st->d_msap, h_msap, (size_t)Np * Lp, hipMemcpyHostToDevice, s));
 	HIPCHK(hipMemcpyAsync(st->d_msapt, h_msapt, (size_t)Lp * Np, hipMemcpyHostToDevice, s));
 
-	/* sequence weights */
-	HIPCHK(hipMalloc((void **)&st->d_w, sizeof(float) * (size_t)N));
+	/* sequence weights (reweighting.c:9-67, bit exact) */
+	HIPCHK(dalloc(st, (void **)&st->d_w, sizeof(float) * (size_t)N));
 	if (ud->reweighting_threshold != 1.0f) {
 		int idthres = identity_threshold(L, ud->reweighting_threshold);
-		unsigned char *d_msa_rows = NULL;
 		/* the identity kernel reads an unpadded N x L matrix */
 		HIPCHK(hipMalloc((void **)&d_msa_rows, (size_t)N * L));
 		HIPCHK(hipMemcpyAsync(d_msa_rows, ud->msa, (size_t)N * L, hipMemcpyHostToDevice, s));
 		HIPCHK(hipMalloc((void **)&d_planes,
 		                 sizeof(uint32_t) * (size_t)N * 5 * ccm_reweight_words(L)));
 		HIPCHK(hipMalloc((void **)&d_counts, sizeof(int32_t) * (size_t)N));
-		int rc = ccm_launch_identity_counts(d_msa_rows, L, N, idthres, d_planes, d_counts, s);
-		if (rc == 0)
-			rc = ccm_launch_weights_from_counts(d_counts, N, L, idthres, st->d_w, s);
-		HIPCHK(hipStreamSynchronize(s));
-		(void)hipFree(d_msa_rows);
-		KCHK(rc);
+		KCHK(ccm_launch_identity_counts(d_msa_rows, L, N, idthres, d_planes, d_counts, s));
+		KCHK(ccm_launch_weights_from_counts(d_counts, N, L, idthres, st->d_w, s));
 	} else {
 		KCHK(ccm_launch_fill_f32(st->d_w, N, 1.0f, s));
 	}
+	HIPCHK(hipStreamSynchronize(s));
 	if (ud->weights) {
 		HIPCHK(hipMemcpyAsync(ud->weights, st->d_w, sizeof(float) * (size_t)N,
 		                      hipMemcpyDeviceToHost, s));
@@ -240,71 +258,74 @@ int init_hip_ex(void *instance, const ccm_hip_config *cfg_in)
 
 	/* reduction scratch */
 	st->n_regpair = ccm_pair_epilogue_blocks(L);
+	if (st->n_regpair < ccm_reg_only_blocks())
+		st->n_regpair = ccm_reg_only_blocks();
 	st->n_regsingle = ccm_singles_blocks(L);
-	HIPCHK(hipMalloc((void **)&st->d_regpair, sizeof(double) * (size_t)st->n_regpair));
-	HIPCHK(hipMalloc((void **)&st->d_regsingle, sizeof(double) * (size_t)st->n_regsingle));
-	HIPCHK(hipMalloc((void **)&st->d_f, sizeof(double) * 4));
+	HIPCHK(dalloc(st, (void **)&st->d_regpair, sizeof(double) * (size_t)st->n_regpair));
+	HIPCHK(dalloc(st, (void **)&st->d_regsingle, sizeof(double) * (size_t)st->n_regsingle));
+	HIPCHK(dalloc(st, (void **)&st->d_f, sizeof(double) * 4));
 	HIPCHK(hipHostMalloc((void **)&st->h_f, sizeof(double) * 4, hipHostMallocDefault));
 
 	if (cfg.precision == CCM_PRECISION_FP32) {
 		st->ldz = round_up((long)CCM_N_ALPHA * L, 64);
 		st->ldg = st->ldz;
-		HIPCHK(hipMalloc((void **)&st->d_zd, sizeof(float) * (size_t)N * st->ldz));
-		HIPCHK(hipMalloc((void **)&st->d_graw,
-		                 sizeof(float) * (size_t)CCM_N_ALPHA * L * st->ldg));
+		HIPCHK(dalloc(st, (void **)&st->d_zd, sizeof(float) * (size_t)chunk_rows * st->ldz));
+		HIPCHK(dalloc(st, (void **)&st->d_graw,
+		              sizeof(float) * (size_t)CCM_N_ALPHA * L * st->ldg));
 		st->n_fpart = ccm_softmax_rows_blocks(L, N);
 		st->n_g1blk = ccm_softmax_rows_nblk(N);
 	} else {
 		const int passes = cfg.bf16_passes;
 		st->ldwt = (long)passes * P;
-		st->lddt = (long)passes * Np;
-		HIPCHK(hipMalloc((void **)&st->d_wt, sizeof(uint16_t) * (size_t)P * st->ldwt));
-		HIPCHK(hipMalloc((void **)&st->d_zt, sizeof(float) * (size_t)P * Np));
-		HIPCHK(hipMalloc((void **)&st->d_dt, sizeof(uint16_t) * (size_t)P * st->lddt));
+		st->lddt = (long)passes * st->ccols;
+		HIPCHK(dalloc(st, (void **)&st->d_wt, sizeof(uint16_t) * (size_t)P * st->ldwt));
+		HIPCHK(dalloc(st, (void **)&st->d_zt, sizeof(float) * (size_t)P * st->ccols));
+		HIPCHK(dalloc(st, (void **)&st->d_dt, sizeof(uint16_t) * (size_t)P * st->lddt));
 		HIPCHK(hipMemsetAsync(st->d_dt, 0, sizeof(uint16_t) * (size_t)P * st->lddt, s));
-		HIPCHK(hipMalloc((void **)&st->d_gt, sizeof(float) * (size_t)P * P));
-		HIPCHK(hipMalloc((void **)&st->d_hsym,
-		                 sizeof(float) * (size_t)CCM_N_ALPHA * L * CCM_N_ALPHA_PAD * L));
+		HIPCHK(dalloc(st, (void **)&st->d_gt, sizeof(float) * (size_t)P * P));
+		HIPCHK(dalloc(st, (void **)&st->d_hsym,
+		              sizeof(float) * (size_t)CCM_N_ALPHA * L * CCM_N_ALPHA_PAD * L));
 		st->n_fpart = ccm_softmax_cols_blocks(L, N);
 		st->n_g1blk = ccm_softmax_cols_nblk(N);
 
 		/* exact data term: hsym = -(H2 + H2^T), H2 = A^T diag(w) A, computed
-		 * once with the FP32 scatter kernel (cu:350-387 analogue) */
-		long ld = round_up((long)CCM_N_ALPHA * L, 64);
-		HIPCHK(hipMalloc((void **)&d_tmp_d, sizeof(float) * (size_t)N * ld));
+		 * once with the FP32 scatter kernel over sequence chunks (the
+		 * reference's init-time histograms, cu:350-387) */
+		const long ld = round_up((long)CCM_N_ALPHA * L, 64);
+		HIPCHK(hipMalloc((void **)&d_tmp_d, sizeof(float) * (size_t)chunk_rows * ld));
 		HIPCHK(hipMalloc((void **)&d_tmp_g, sizeof(float) * (size_t)CCM_N_ALPHA * L * ld));
-		KCHK(ccm_launch_neg_weighted_onehot(st->d_msap, Lp, st->d_w, L, N, d_tmp_d, ld, s));
-		KCHK(ccm_launch_gather_grad_f32(d_tmp_d, ld, st->d_msap, Lp, L, N, d_tmp_g, ld, s));
+		for (int n0 = 0; n0 < N; n0 += st->chunk) {
+			const int nc = N - n0 < st->chunk ? N - n0 : st->chunk;
+			KCHK(ccm_launch_neg_weighted_onehot(st->d_msap + (size_t)n0 * Lp, Lp, st->d_w + n0,
+			                                    L, nc, d_tmp_d, ld, s));
+			KCHK(ccm_launch_gather_grad_f32(d_tmp_d, ld, st->d_msap + (size_t)n0 * Lp, Lp, L, nc,
+			                                d_tmp_g, ld, n0 > 0, s));
+		}
 		KCHK(ccm_launch_hsym(d_tmp_g, ld, st->d_hsym, L, s));
-		HIPCHK(hipStreamSynchronize(s));
 	}
-	HIPCHK(hipMalloc((void **)&st->d_fpart, sizeof(double) * (size_t)st->n_fpart));
-	HIPCHK(hipMalloc((void **)&st->d_g1part,
-	                 sizeof(double) * (size_t)st->n_g1blk * 20 * (size_t)L));
+	HIPCHK(dalloc(st, (void **)&st->d_fpart, sizeof(double) * (size_t)st->n_fpart));
+	HIPCHK(dalloc(st, (void **)&st->d_g1part,
+	              sizeof(double) * (size_t)st->n_g1blk * 20 * (size_t)L));
 	HIPCHK(hipStreamSynchronize(s));
 
-	if (d_planes)
-		(void)hipFree(d_planes);
-	if (d_counts)
-		(void)hipFree(d_counts);
-	if (d_tmp_d)
-		(void)hipFree(d_tmp_d);
-	if (d_tmp_g)
-		(void)hipFree(d_tmp_g);
+	(void)hipFree(d_msa_rows);
+	(void)hipFree(d_planes);
+	(void)hipFree(d_counts);
+	(void)hipFree(d_tmp_d);
+	(void)hipFree(d_tmp_g);
 	free(h_msap);
 	free(h_msapt);
 	ud->extra = st;
 	return 0;
 
 fail:
-	if (d_planes)
-		(void)hipFree(d_planes);
-	if (d_counts)
-		(void)hipFree(d_counts);
-	if (d_tmp_d)
-		(void)hipFree(d_tmp_d);
-	if (d_tmp_g)
-		(void)hipFree(d_tmp_g);
+	if (st->stream)
+		(void)hipStreamSynchronize(st->stream);
+	(void)hipFree(d_msa_rows);
+	(void)hipFree(d_planes);
+	(void)hipFree(d_counts);
+	(void)hipFree(d_tmp_d);
+	(void)hipFree(d_tmp_g);
 	free(h_msap);
 	free(h_msapt);
 	free_state(st);
@@ -337,58 +358,156 @@ void *ccm_hip_stream(void *instance)
 	return st ? (void *)st->stream : NULL;
 }
 
+size_t ccm_hip_device_bytes(void *instance)
+{
+	userdata *ud = (userdata *)instance;
+	ccm_hip_state *st = (ccm_hip_state *)ud->extra;
+	return st ? st->bytes : 0;
+}
+
 #define MARK(i)                                                            \
 	do {                                                               \
 		if (st->timing)                                            \
 			HIPCHK(hipEventRecord(st->ev[(i)], s));            \
 	} while (0)
 
-int ccm_hip_evaluate(void *instance, const float *d_x, float *d_g, int nvar, double *f)
+enum { RUN_OBJECTIVE = 0, RUN_FORWARD = 1, RUN_FULL = 2 };
+
+static ccm_hip_state *get_state(void *instance, int nvar)
 {
 	userdata *ud = (userdata *)instance;
 	ccm_hip_state *st = (ccm_hip_state *)ud->extra;
 	if (!st || nvar != st->nvp) {
 		fprintf(stderr, "ccmpred_amd: evaluate called with nvar=%d, expected %ld\n", nvar,
 		        st ? st->nvp : -1L);
-		return -1;
+		return NULL;
 	}
+	return st;
+}
+
+/* Logits, softmax and (RUN_FULL) pair-gradient accumulation over all
+ * sequence chunks.  *step advances through the timing marks: one mark
+ * before each of the three per-chunk kernels of the first chunk. */
+static int run_chunks(ccm_hip_state *st, const float *d_x, int what, int *step)
+{
 	hipStream_t s = st->stream;
-	const int L = st->L, N = st->N, Lp = st->Lp, Np = st->Np;
-	int step = 0;
-	if (st->cfg.precision == CCM_PRECISION_FP32) {
-		MARK(step++);
-		KCHK(ccm_launch_gather_fwd_f32(st->d_msap, Lp, d_x, L, N, st->d_zd, st->ldz, s));
-		MARK(step++);
-		KCHK(ccm_launch_softmax_rows(st->d_zd, st->ldz, st->d_msap, Lp, st->d_w, L, N,
-		                             st->d_fpart, st->d_g1part, s));
-		MARK(step++);
-		KCHK(ccm_launch_gather_grad_f32(st->d_zd, st->ldz, st->d_msap, Lp, L, N, st->d_graw,
-		                                st->ldg, s));
-		MARK(step++);
+	const int L = st->L, N = st->N, Lp = st->Lp, Np = st->Np, P = st->P;
+	const int mode = what == RUN_OBJECTIVE ? 0 : 1;
+	for (int n0 = 0; n0 < N; n0 += st->chunk) {
+		const int nc = N - n0 < st->chunk ? N - n0 : st->chunk;
+		const int first = n0 == 0;
+		const unsigned char *msap = st->d_msap + (size_t)n0 * Lp;
+		if (st->cfg.precision == CCM_PRECISION_FP32) {
+			if (first)
+				MARK((*step)++);
+			KCHK(ccm_launch_gather_fwd_f32(msap, Lp, d_x, L, nc, st->d_zd, st->ldz, s));
+			if (first)
+				MARK((*step)++);
+			KCHK(ccm_launch_softmax_rows(st->d_zd, st->ldz, msap, Lp, st->d_w + n0, L, nc, n0,
+			                             mode, st->d_fpart, st->d_g1part, s));
+			if (first)
+				MARK((*step)++);
+			if (what == RUN_FULL)
+				KCHK(ccm_launch_gather_grad_f32(st->d_zd, st->ldz, msap, Lp, L, nc, st->d_graw,
+				                                st->ldg, !first, s));
+		} else {
+			const int passes = st->cfg.bf16_passes;
+			const int ncols = (int)round_up(nc, 128);
+			if (first)
+				MARK((*step)++);
+			KCHK(ccm_launch_gemm_fwd(st->d_wt, st->ldwt, Lp, passes, msap, ncols, st->d_zt,
+			                         (long)ncols, s));
+			if (first)
+				MARK((*step)++);
+			KCHK(ccm_launch_softmax_cols(st->d_zt, (long)ncols, d_x, st->d_msapt + n0, (long)Np,
+			                             st->d_w + n0, L, Lp, nc, ncols, n0, st->d_dt,
+			                             (long)passes * ncols, (long)ncols, passes, mode,
+			                             st->d_fpart, st->d_g1part, s));
+			if (first)
+				MARK((*step)++);
+			if (what == RUN_FULL)
+				KCHK(ccm_launch_gemm_grad(st->d_dt, (long)passes * ncols, Lp, passes,
+				                          st->d_msapt + n0, (long)Np, ncols, st->d_gt, (long)P,
+				                          !first, s));
+		}
+	}
+	return 0;
+fail:
+	return -1;
+}
+
+static int pair_gradient_from_forward(ccm_hip_state *st)
+{
+	/* the single-chunk forward left D / Dt in place */
+	hipStream_t s = st->stream;
+	if (st->cfg.precision == CCM_PRECISION_FP32)
+		KCHK(ccm_launch_gather_grad_f32(st->d_zd, st->ldz, st->d_msap, st->Lp, st->L, st->N,
+		                                st->d_graw, st->ldg, 0, s));
+	else
+		KCHK(ccm_launch_gemm_grad(st->d_dt, st->lddt, st->Lp, st->cfg.bf16_passes, st->d_msapt,
+		                          (long)st->Np, st->ccols, st->d_gt, (long)st->P, 0, s));
+	return 0;
+fail:
+	return -1;
+}
+
+static int assemble_gradient(ccm_hip_state *st, const float *d_x, float *d_g)
+{
+	hipStream_t s = st->stream;
+	const int L = st->L;
+	if (st->cfg.precision == CCM_PRECISION_FP32)
 		KCHK(ccm_launch_pair_epilogue(st->d_graw, st->ldg, L, NULL, d_x, d_g, L, st->lam2,
 		                              st->d_regpair, s));
-	} else {
-		const int passes = st->cfg.bf16_passes;
-		MARK(step++);
-		KCHK(ccm_launch_convert_wt(d_x, L, Lp, st->d_wt, st->ldwt, passes, s));
-		MARK(step++);
-		KCHK(ccm_launch_gemm_fwd(st->d_wt, st->ldwt, Lp, passes, st->d_msap, Np, st->d_zt,
-		                         (long)Np, s));
-		MARK(step++);
-		KCHK(ccm_launch_softmax_cols(st->d_zt, (long)Np, d_x, st->d_msapt, Np, st->d_w, L, Lp,
-		                             N, st->d_dt, st->lddt, passes, st->d_fpart, st->d_g1part,
-		                             s));
-		MARK(step++);
-		KCHK(ccm_launch_gemm_grad(st->d_dt, st->lddt, Lp, passes, st->d_msapt, Np, st->d_gt,
-		                          (long)st->P, s));
-		MARK(step++);
-		KCHK(ccm_launch_pair_epilogue(st->d_gt, (long)st->P, Lp, st->d_hsym, d_x, d_g, L,
+	else
+		KCHK(ccm_launch_pair_epilogue(st->d_gt, (long)st->P, st->Lp, st->d_hsym, d_x, d_g, L,
 		                              st->lam2, st->d_regpair, s));
-	}
-	MARK(step++);
 	KCHK(ccm_launch_singles_finalize(st->d_g1part, st->n_g1blk, d_x, d_g, L, st->lam1,
 	                                 st->d_regsingle, s));
-	KCHK(ccm_launch_sum3(st->d_fpart, st->n_fpart, st->d_regpair, st->n_regpair,
+	return 0;
+fail:
+	return -1;
+}
+
+static int read_f(ccm_hip_state *st, int nreg_pair, int nreg_single, double *f)
+{
+	hipStream_t s = st->stream;
+	KCHK(ccm_launch_sum3(st->d_fpart, st->n_fpart, st->d_regpair, nreg_pair, st->d_regsingle,
+	                     nreg_single, st->d_f, s));
+	HIPCHK(hipMemcpyAsync(st->h_f, st->d_f, sizeof(double), hipMemcpyDeviceToHost, s));
+	HIPCHK(hipStreamSynchronize(s));
+	*f = st->h_f[0];
+	return 0;
+fail:
+	return -1;
+}
+
+int ccm_hip_evaluate(void *instance, const float *d_x, float *d_g, int nvar, double *f)
+{
+	ccm_hip_state *st = get_state(instance, nvar);
+	if (!st)
+		return -1;
+	hipStream_t s = st->stream;
+	int step = 0;
+	st->forward_valid = 0;
+	if (st->cfg.precision != CCM_PRECISION_FP32) {
+		MARK(step++);
+		KCHK(ccm_launch_convert_wt(d_x, st->L, st->Lp, st->d_wt, st->ldwt, st->cfg.bf16_passes,
+		                           s));
+	}
+	if (run_chunks(st, d_x, RUN_FULL, &step))
+		goto fail;
+	MARK(step++);
+	if (st->cfg.precision == CCM_PRECISION_FP32) {
+		KCHK(ccm_launch_pair_epilogue(st->d_graw, st->ldg, st->L, NULL, d_x, d_g, st->L,
+		                              st->lam2, st->d_regpair, s));
+	} else {
+		KCHK(ccm_launch_pair_epilogue(st->d_gt, (long)st->P, st->Lp, st->d_hsym, d_x, d_g,
+		                              st->L, st->lam2, st->d_regpair, s));
+	}
+	MARK(step++);
+	KCHK(ccm_launch_singles_finalize(st->d_g1part, st->n_g1blk, d_x, d_g, st->L, st->lam1,
+	                                 st->d_regsingle, s));
+	KCHK(ccm_launch_sum3(st->d_fpart, st->n_fpart, st->d_regpair, ccm_pair_epilogue_blocks(st->L),
 	                     st->d_regsingle, st->n_regsingle, st->d_f, s));
 	MARK(step);
 	HIPCHK(hipMemcpyAsync(st->h_f, st->d_f, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -405,6 +524,75 @@ int ccm_hip_evaluate(void *instance, const float *d_x, float *d_g, int nvar, dou
 	return 0;
 fail:
 	return -1;
+}
+
+int ccm_hip_objective(void *instance, const float *d_x, int nvar, double *f)
+{
+	ccm_hip_state *st = get_state(instance, nvar);
+	if (!st)
+		return -1;
+	hipStream_t s = st->stream;
+	int step = 0, timing = st->timing;
+	st->timing = 0;
+	st->forward_valid = 0;
+	if (st->cfg.precision != CCM_PRECISION_FP32)
+		KCHK(ccm_launch_convert_wt(d_x, st->L, st->Lp, st->d_wt, st->ldwt, st->cfg.bf16_passes,
+		                           s));
+	if (run_chunks(st, d_x, RUN_OBJECTIVE, &step))
+		goto fail;
+	KCHK(ccm_launch_reg_only(d_x, st->L, st->lam1, st->lam2, st->d_regpair, s));
+	if (read_f(st, ccm_reg_only_blocks(), 0, f))
+		goto fail;
+	st->timing = timing;
+	return 0;
+fail:
+	st->timing = timing;
+	return -1;
+}
+
+int ccm_hip_forward(void *instance, const float *d_x, int nvar, double *f)
+{
+	ccm_hip_state *st = get_state(instance, nvar);
+	if (!st)
+		return -1;
+	if (st->chunk < st->N) {
+		fprintf(stderr, "ccmpred_amd: ccm_hip_forward needs an unchunked instance\n");
+		return -1;
+	}
+	hipStream_t s = st->stream;
+	int step = 0, timing = st->timing;
+	st->timing = 0;
+	st->forward_valid = 0;
+	if (st->cfg.precision != CCM_PRECISION_FP32)
+		KCHK(ccm_launch_convert_wt(d_x, st->L, st->Lp, st->d_wt, st->ldwt, st->cfg.bf16_passes,
+		                           s));
+	if (run_chunks(st, d_x, RUN_FORWARD, &step))
+		goto fail;
+	KCHK(ccm_launch_reg_only(d_x, st->L, st->lam1, st->lam2, st->d_regpair, s));
+	if (read_f(st, ccm_reg_only_blocks(), 0, f))
+		goto fail;
+	st->forward_valid = 1;
+	st->timing = timing;
+	return 0;
+fail:
+	st->timing = timing;
+	return -1;
+}
+
+int ccm_hip_backward(void *instance, const float *d_x, float *d_g, int nvar)
+{
+	ccm_hip_state *st = get_state(instance, nvar);
+	if (!st)
+		return -1;
+	if (!st->forward_valid) {
+		fprintf(stderr, "ccmpred_amd: ccm_hip_backward without a preceding ccm_hip_forward\n");
+		return -1;
+	}
+	if (pair_gradient_from_forward(st) || assemble_gradient(st, d_x, d_g))
+		return -1;
+	if (hipStreamSynchronize(st->stream) != hipSuccess)
+		return -1;
+	return 0;
 }
 
 int ccm_hip_set_timing(void *instance, int on)

@@ -126,6 +126,27 @@ __global__ __launch_bounds__(256) void k_singles_finalize(const double* __restri
   if (threadIdx.x == 0) regpart[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
+// regularisation terms alone: lam1 sum V^2 + 0.5 lam2 sum x2^2 (padded block)
+__global__ __launch_bounds__(256) void k_reg_only(const float* __restrict__ x, long ns, long nsp,
+                                                  long nvp, float lam1, float lam2,
+                                                  double* __restrict__ part) {
+  __shared__ double red[4];
+  double r = 0.0;
+  const float half_l = 0.5f * lam2;
+  for (long v = (long)blockIdx.x * 256 + threadIdx.x; v < nvp; v += (long)gridDim.x * 256) {
+    const float xv = x[v];
+    if (v < ns)
+      r += (double)(lam1 * xv * xv);
+    else if (v >= nsp)
+      r += (double)(half_l * xv * xv);
+  }
+  r = ccm_wave_sum_f64(r);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
 // Neumaier-compensated FP64 sum of up to three partial arrays, fixed order.
 // out[0] = total (double)
 __global__ __launch_bounds__(256) void k_sum3(const double* __restrict__ a, int na,
@@ -191,6 +212,16 @@ int ccm_launch_singles_finalize(const double* d_g1part, int nblk, const float* d
   const long nsp = ccm_nsp(L);
   hipLaunchKernelGGL(k_singles_finalize, dim3((unsigned)((nsp + 255) / 256)), dim3(256), 0, s, d_g1part,
                      nblk, d_x, d_g, L, nsp, lam1, d_regpart);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_reg_only_blocks(void) { return 1024; }
+
+int ccm_launch_reg_only(const float* d_x, int L, float lam1, float lam2, double* d_part, hipStream_t s) {
+  const long nsp = ccm_nsp(L);
+  const long nvp = nsp + (long)L * L * CCM_NA * CCM_NAP;
+  hipLaunchKernelGGL(k_reg_only, dim3(1024), dim3(256), 0, s, d_x, 20L * L, nsp, nvp, lam1, lam2, d_part);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }

@@ -104,8 +104,8 @@ __global__ __launch_bounds__(256) void k_gather_fwd_f32(const uint8_t* __restric
 __global__ __launch_bounds__(256) void k_gather_grad_f32(const float* __restrict__ D, long ldd,
                                                          const uint8_t* __restrict__ msap, int Lp,
                                                          int L, int N, float* __restrict__ G,
-                                                         long ldg, uint32_t tiles_k, uint32_t tiles,
-                                                         uint32_t total) {
+                                                         long ldg, int accumulate, uint32_t tiles_k,
+                                                         uint32_t tiles, uint32_t total) {
   // wave-private accumulators, one array per k slot so the compiler may
   // overlap their read-modify-write chains
   __shared__ float a0[4][CCM_NA * 64];
@@ -159,10 +159,11 @@ __global__ __launch_bounds__(256) void k_gather_grad_f32(const float* __restrict
   }
   if (cok) {
     for (int b = 0; b < CCM_NA; b++) {
-      if (k0ok) G[(long)(b * L + kb) * ldg + c] = s0[b * 64 + lane];
-      if (k1ok) G[(long)(b * L + kb + 1) * ldg + c] = s1[b * 64 + lane];
-      if (k2ok) G[(long)(b * L + kb + 2) * ldg + c] = s2[b * 64 + lane];
-      if (k3ok) G[(long)(b * L + kb + 3) * ldg + c] = s3[b * 64 + lane];
+      float* g0 = G + (long)(b * L + kb) * ldg + c;
+      if (k0ok) g0[0] = accumulate ? g0[0] + s0[b * 64 + lane] : s0[b * 64 + lane];
+      if (k1ok) g0[ldg] = accumulate ? g0[ldg] + s1[b * 64 + lane] : s1[b * 64 + lane];
+      if (k2ok) g0[2 * ldg] = accumulate ? g0[2 * ldg] + s2[b * 64 + lane] : s2[b * 64 + lane];
+      if (k3ok) g0[3 * ldg] = accumulate ? g0[3 * ldg] + s3[b * 64 + lane] : s3[b * 64 + lane];
     }
   }
 }
@@ -200,15 +201,15 @@ int ccm_launch_gather_fwd_f32(const uint8_t* d_msap, int Lp, const float* d_x, i
   return CCM_OK;
 }
 
-// G (21L x ldg floats) <- sum_n onehot_n^T D_n ; D is N x ldd
+// G (21L x ldg floats) (+)= sum_n onehot_n^T D_n ; D is N x ldd
 int ccm_launch_gather_grad_f32(const float* d_d, long ldd, const uint8_t* d_msap, int Lp, int L,
-                               int N, float* d_g, long ldg, hipStream_t s) {
+                               int N, float* d_g, long ldg, int accumulate, hipStream_t s) {
   if (L <= 0 || N <= 0 || ldg < 21L * L || ldd < 21L * L || (Lp % 32) || Lp < L) return CCM_ERR_ARG;
   const uint32_t tiles_c = (uint32_t)((21L * L + 63) / 64);
   const uint32_t tiles_k = (uint32_t)((L + 15) / 16);
   const uint32_t tiles = tiles_c * tiles_k, total = round8(tiles);
   hipLaunchKernelGGL(k_gather_grad_f32, dim3(total), dim3(256), 0, s, d_d, ldd, d_msap, Lp, L, N, d_g,
-                     ldg, tiles_k, tiles, total);
+                     ldg, accumulate, tiles_k, tiles, total);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }

@@ -81,8 +81,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_onehot(const uint16_t* __restri
                                                         const uint8_t* __restrict__ bytes,
                                                         long ldbytes, int Lp, int Ncol,
                                                         float* __restrict__ C, long ldc,
-                                                        uint32_t tiles_n, uint32_t tiles,
-                                                        uint32_t total) {
+                                                        int accumulate, uint32_t tiles_n,
+                                                        uint32_t tiles, uint32_t total) {
   __shared__ __attribute__((aligned(16))) uint16_t As[2][GM_TM * GM_LDA];
   const uint32_t qid = ccm_xcd_remap(blockIdx.x, total);
   if (qid >= tiles) return;
@@ -190,7 +190,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_onehot(const uint16_t* __restri
 #pragma unroll
       for (int i = 0; i < 16; i++) {
         const int row = m0 + wm * 128 + rb * 32 + (i >> 2) * 8 + 4 * q + (i & 3);
-        if (row < M) C[(long)row * ldc + col] = acc[rb][cb][i];
+        if (row < M) {
+          float* dst = C + (long)row * ldc + col;
+          *dst = accumulate ? *dst + acc[rb][cb][i] : acc[rb][cb][i];
+        }
       }
     }
   }
@@ -211,28 +214,33 @@ int ccm_launch_convert_wt(const float* d_x, int L, int Lp, uint16_t* d_wt, long 
   return CCM_OK;
 }
 
-// Zt[c][n] (P x ldc floats, ldc >= Np) = sum_r Wt[c][r] [x_nk = b]
+// Zt[c][n] (P x ldc floats, ldc >= ncols) = sum_r Wt[c][r] [x_nk = b] for the
+// ncols (multiple of 128) sequences whose padded MSA rows start at d_msap
 int ccm_launch_gemm_fwd(const uint16_t* d_wt, long lda, int Lp, int passes, const uint8_t* d_msap,
-                        int Np, float* d_zt, long ldc, hipStream_t s) {
+                        int ncols, float* d_zt, long ldc, hipStream_t s) {
   const int P = CCM_NA * Lp;
-  if (Lp % 32 || Np % GM_TN || lda < (long)passes * P || ldc < Np || (lda % 8)) return CCM_ERR_ARG;
-  const uint32_t tm = (P + GM_TM - 1) / GM_TM, tn = Np / GM_TN;
+  if (Lp % 32 || ncols % GM_TN || ncols <= 0 || lda < (long)passes * P || ldc < ncols || (lda % 8))
+    return CCM_ERR_ARG;
+  const uint32_t tm = (P + GM_TM - 1) / GM_TM, tn = ncols / GM_TN;
   const uint32_t tiles = tm * tn, total = round8(tiles);
   hipLaunchKernelGGL(k_gemm_onehot<0>, dim3(total), dim3(256), 0, s, d_wt, lda, P, P, passes, d_msap,
-                     (long)Lp, Lp, Np, d_zt, ldc, tn, tiles, total);
+                     (long)Lp, Lp, ncols, d_zt, ldc, 0, tn, tiles, total);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }
 
-// Gt[c][r] (P x ldc floats, ldc >= P) = sum_n Dt[c][n] [x_nk = b]
+// Gt[c][r] (P x ldc floats, ldc >= P) (+)= sum_n Dt[c][n] [x_nk = b] over the
+// kcols (multiple of 32) sequences of Dt; d_msapt points at the chunk's first
+// column of the transposed MSA, whose rows are ldmt bytes apart
 int ccm_launch_gemm_grad(const uint16_t* d_dt, long lda, int Lp, int passes, const uint8_t* d_msapt,
-                         int Np, float* d_gt, long ldc, hipStream_t s) {
+                         long ldmt, int kcols, float* d_gt, long ldc, int accumulate, hipStream_t s) {
   const int P = CCM_NA * Lp;
-  if (Lp % 32 || Np % GM_TK || lda < (long)passes * Np || ldc < P || (lda % 8)) return CCM_ERR_ARG;
+  if (Lp % 32 || kcols % GM_TK || kcols <= 0 || lda < (long)passes * kcols || ldc < P || (lda % 8))
+    return CCM_ERR_ARG;
   const uint32_t tm = (P + GM_TM - 1) / GM_TM, tn = (P + GM_TN - 1) / GM_TN;
   const uint32_t tiles = tm * tn, total = round8(tiles);
-  hipLaunchKernelGGL(k_gemm_onehot<1>, dim3(total), dim3(256), 0, s, d_dt, lda, P, Np, passes, d_msapt,
-                     (long)Np, Lp, P, d_gt, ldc, tn, tiles, total);
+  hipLaunchKernelGGL(k_gemm_onehot<1>, dim3(total), dim3(256), 0, s, d_dt, lda, P, kcols, passes,
+                     d_msapt, ldmt, Lp, P, d_gt, ldc, accumulate, tn, tiles, total);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }

@@ -63,7 +63,7 @@ class PLL:
     """One MSA on one GPU: weights, f+grad evaluation and the device CG."""
 
     def __init__(self, msa, theta=0.8, lambda_single=0.01, lambda_pair=None,
-                 lambda_pair_factor=0.2, precision="fp32", device=0, verbose=False):
+                 lambda_pair_factor=0.2, precision="fp32", device=0, verbose=False, chunk_rows=0):
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {list(PRECISIONS)}")
         self.msa = np.ascontiguousarray(msa, dtype=np.uint8)
@@ -93,6 +93,7 @@ class PLL:
         cfg.device = int(device)
         cfg.precision, cfg.bf16_passes = PRECISIONS[precision]
         cfg.verbose = 1 if verbose else 0
+        cfg.chunk_rows = int(chunk_rows)
         rc = _lib.lib().init_hip_ex(self._uptr, C.byref(cfg))
         if rc != 0:
             raise RuntimeError("init_hip_ex failed")
@@ -130,6 +131,43 @@ class PLL:
         if rc != 0:
             raise RuntimeError("ccm_hip_evaluate failed")
         return f.value
+
+    def _check(self, x):
+        import torch
+        if not (x.is_cuda and x.dtype == torch.float32 and x.numel() == self.nvar and x.is_contiguous()):
+            raise ValueError("x must be a contiguous CUDA float32 tensor of nvar_padded entries")
+        torch.cuda.current_stream(x.device).synchronize()
+
+    def objective(self, x) -> float:
+        """f alone (FP64); per-sequence probabilities are not kept."""
+        self._check(x)
+        f = C.c_double()
+        if _lib.lib().ccm_hip_objective(self._uptr, C.c_void_p(x.data_ptr()), self.nvar, C.byref(f)):
+            raise RuntimeError("ccm_hip_objective failed")
+        return f.value
+
+    def forward(self, x) -> float:
+        """f (FP64), keeping the probability operand on the device for :meth:`backward`."""
+        self._check(x)
+        f = C.c_double()
+        if _lib.lib().ccm_hip_forward(self._uptr, C.c_void_p(x.data_ptr()), self.nvar, C.byref(f)):
+            raise RuntimeError("ccm_hip_forward failed")
+        return f.value
+
+    def backward(self, x, g=None):
+        """Tied gradient from the probabilities the last :meth:`forward` kept."""
+        import torch
+        self._check(x)
+        if g is None:
+            g = torch.empty_like(x)
+        if _lib.lib().ccm_hip_backward(self._uptr, C.c_void_p(x.data_ptr()), C.c_void_p(g.data_ptr()),
+                                       self.nvar):
+            raise RuntimeError("ccm_hip_backward failed")
+        return g
+
+    @property
+    def device_bytes(self) -> int:
+        return int(_lib.lib().ccm_hip_device_bytes(self._uptr))
 
     def set_timing(self, on=True):
         _lib.lib().ccm_hip_set_timing(self._uptr, 1 if on else 0)

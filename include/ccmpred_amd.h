@@ -15,6 +15,7 @@
 #ifndef CCMPRED_AMD_H
 #define CCMPRED_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 #include <stdio.h>
 
@@ -66,6 +67,7 @@ typedef struct ccm_hip_config {
 	int precision;   /* CCM_PRECISION_FP32 (plain) or CCM_PRECISION_BF16 (MFMA) */
 	int bf16_passes; /* 1: BF16 operands; 2: hi+lo BF16 split of the dense operand */
 	int verbose;     /* 1: print the reweighting summary like the reference */
+	int chunk_rows;  /* sequences per chunk (rounded up to 256); 0 = all at once */
 } ccm_hip_config;
 
 void ccm_hip_default_config(ccm_hip_config *cfg);
@@ -89,6 +91,19 @@ int destroy_hip(void *instance);
 
 /* Same as evaluate_hip with the FP64 objective and a status code. */
 int ccm_hip_evaluate(void *instance, const float *d_x, float *d_g, int nvar, double *f);
+
+/* The objective alone (no gradient, no stored probabilities); sequences are
+ * processed in the instance's chunks.  No counterpart in the reference. */
+int ccm_hip_objective(void *instance, const float *d_x, int nvar, double *f);
+
+/* The evaluation split in two for an unchunked instance: forward computes f
+ * and keeps the N x 21L probability operand on the device, backward turns it
+ * into the tied gradient.  No counterpart in the reference. */
+int ccm_hip_forward(void *instance, const float *d_x, int nvar, double *f);
+int ccm_hip_backward(void *instance, const float *d_x, float *d_g, int nvar);
+
+/* Device bytes the instance holds (all buffers allocated at init). */
+size_t ccm_hip_device_bytes(void *instance);
 
 /* HIP stream the instance runs on (hipStream_t). */
 void *ccm_hip_stream(void *instance);

@@ -30,7 +30,7 @@ int ccm_launch_fill_f32(float *d_p, long n, float v, ccm_stream_t s);
 int ccm_launch_gather_fwd_f32(const uint8_t *d_msap, int Lp, const float *d_x, int L, int N,
                               float *d_z, long ldz, ccm_stream_t s);
 int ccm_launch_gather_grad_f32(const float *d_d, long ldd, const uint8_t *d_msap, int Lp, int L,
-                               int N, float *d_g, long ldg, ccm_stream_t s);
+                               int N, float *d_g, long ldg, int accumulate, ccm_stream_t s);
 int ccm_launch_neg_weighted_onehot(const uint8_t *d_msap, int Lp, const float *d_w, int L, int N,
                                    float *d_d, long ldd, ccm_stream_t s);
 
@@ -40,19 +40,21 @@ int ccm_softmax_rows_nblk(int N);
 int ccm_softmax_cols_blocks(int L, int N);
 int ccm_softmax_cols_nblk(int N);
 int ccm_launch_softmax_rows(float *d_zd, long ldz, const uint8_t *d_msap, int Lp, const float *d_w,
-                            int L, int N, double *d_fpart, double *d_g1part, ccm_stream_t s);
-int ccm_launch_softmax_cols(const float *d_zt, long ldzt, const float *d_x1, const uint8_t *d_msapt,
-                            int Np, const float *d_w, int L, int Lp, int N, uint16_t *d_dt,
-                            long lddt, int passes, double *d_fpart, double *d_g1part,
+                            int L, int Nc, int n0, int mode, double *d_fpart, double *d_g1part,
                             ccm_stream_t s);
+int ccm_launch_softmax_cols(const float *d_zt, long ldzt, const float *d_x1, const uint8_t *d_msapt,
+                            long ldmt, const float *d_w, int L, int Lp, int Nc, int ncols, int n0,
+                            uint16_t *d_dt, long lddt, long lo_off, int passes, int mode,
+                            double *d_fpart, double *d_g1part, ccm_stream_t s);
 
 /* BF16 MFMA contractions */
 int ccm_launch_convert_wt(const float *d_x, int L, int Lp, uint16_t *d_wt, long lda, int passes,
                           ccm_stream_t s);
 int ccm_launch_gemm_fwd(const uint16_t *d_wt, long lda, int Lp, int passes, const uint8_t *d_msap,
-                        int Np, float *d_zt, long ldc, ccm_stream_t s);
+                        int ncols, float *d_zt, long ldc, ccm_stream_t s);
 int ccm_launch_gemm_grad(const uint16_t *d_dt, long lda, int Lp, int passes,
-                         const uint8_t *d_msapt, int Np, float *d_gt, long ldc, ccm_stream_t s);
+                         const uint8_t *d_msapt, long ldmt, int kcols, float *d_gt, long ldc,
+                         int accumulate, ccm_stream_t s);
 
 /* gradient assembly (cu:150-219, cu:390-461) and objective reduction */
 int ccm_pair_epilogue_blocks(int L);
@@ -63,6 +65,9 @@ int ccm_launch_pair_epilogue(const float *d_m, long ldm, int Lm, const float *d_
 int ccm_launch_hsym(const float *d_mh, long ldm, float *d_hsym, int L, ccm_stream_t s);
 int ccm_launch_singles_finalize(const double *d_g1part, int nblk, const float *d_x, float *d_g,
                                 int L, float lam1, double *d_regpart, ccm_stream_t s);
+int ccm_reg_only_blocks(void);
+int ccm_launch_reg_only(const float *d_x, int L, float lam1, float lam2, double *d_part,
+                        ccm_stream_t s);
 int ccm_launch_sum3(const double *a, int na, const double *b, int nb, const double *c, int nc,
                     double *out, ccm_stream_t s);
 

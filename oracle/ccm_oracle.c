@@ -503,3 +503,138 @@ int ora_cg_pll_f32(const unsigned char *msa, const float *w, int ncol, int nrow,
 	return ora_cg((int)ora_nvar_padded(ncol), x, fx, ora_pll_eval, &c, p, iterations,
 	              evaluations, fhist);
 }
+
+/* ---- tied central differences (numdif.c:5-57 methodology) -------------- */
+
+/* affected logits of one sequence: column, residue row, shift */
+typedef struct {
+	long col;
+	int res;
+	double delta;
+} ora_shift;
+
+static double ora_term_f64(const double *z21, int xobs)
+{
+	double m = z21[0];
+	for (int a = 1; a < ORA_NA; a++)
+		if (z21[a] > m)
+			m = z21[a];
+	double s = 0;
+	for (int a = 0; a < ORA_NA; a++)
+		s += exp(z21[a] - m);
+	return m + log(s) - z21[xobs];
+}
+
+/*
+ * out[i] = (f(x + h u) - f(x - h u)) / (2h) in FP64 with u = e_{idx_a[i]} +
+ * e_{idx_b[i]} (a single unit vector when idx_a[i] == idx_b[i]), the
+ * perturbation numdif.c applies to a coupling and its mirror.  Only the
+ * pseudo-likelihood terms whose logits move are recomputed, so each probe
+ * costs O(N * 21) after one O(N * L * 21L) logit pass.
+ */
+void ora_tied_fd_f64(const unsigned char *msa, const double *w, int ncol, int nrow,
+                     double lambda_single, double lambda_pair, const double *x, const long *idx_a,
+                     const long *idx_b, int count, double h, double *out)
+{
+	const long L = ncol, N = nrow;
+	const long nsingle = L * (ORA_NA - 1);
+	const long nsp = ora_nsingle_padded(ncol);
+	const double *x1 = x, *x2 = x + nsp;
+	double *Z = (double *)malloc(sizeof(double) * (size_t)N * ORA_NA * L); /* [n][a][j] */
+	for (long n = 0; n < N; n++) {
+		double *zn = Z + n * ORA_NA * L;
+		const unsigned char *seq = msa + n * L;
+		for (long a = 0; a < ORA_NA; a++)
+			for (long j = 0; j < L; j++)
+				zn[a * L + j] = a < ORA_NA - 1 ? x1[a * L + j] : 0.0;
+		for (long k = 0; k < L; k++) {
+			const double *row = x2 + ORA_X2(seq[k], k, 0, 0, L);
+			for (long c = 0; c < ORA_NA * L; c++)
+				zn[c] += row[c];
+		}
+	}
+	for (int p = 0; p < count; p++) {
+		long ids[2] = {idx_a[p], idx_b[p]};
+		int nid = idx_a[p] == idx_b[p] ? 1 : 2;
+		double total = 0.0;
+		/* regularisation change */
+		for (int q = 0; q < nid; q++) {
+			const long v = ids[q];
+			const double xv = x[v];
+			const double dplus = (xv + h) * (xv + h) - xv * xv;
+			const double dminus = (xv - h) * (xv - h) - xv * xv;
+			if (v < nsingle)
+				total += lambda_single * (dplus - dminus);
+			else if (v >= nsp)
+				total += 0.5 * lambda_pair * (dplus - dminus);
+		}
+		/* decode the likelihood effect of each index */
+		int kind[2];
+		long colv[2], kcol[2];
+		int resv[2], bres[2];
+		for (int q = 0; q < nid; q++) {
+			const long v = ids[q];
+			kind[q] = 0;
+			if (v < nsingle) {
+				kind[q] = 1;
+				resv[q] = (int)(v / L);
+				colv[q] = v % L;
+			} else if (v >= nsp) {
+				long off = v - nsp;
+				const long j = off % L;
+				off /= L;
+				const long a = off % ORA_NAP;
+				off /= ORA_NAP;
+				const long k = off % L;
+				const long b = off / L;
+				if (a < ORA_NA) {
+					kind[q] = 2;
+					resv[q] = (int)a;
+					colv[q] = j;
+					kcol[q] = k;
+					bres[q] = (int)b;
+				}
+			}
+		}
+		for (long n = 0; n < N; n++) {
+			const unsigned char *seq = msa + n * L;
+			ora_shift sh[2];
+			int ns = 0;
+			for (int q = 0; q < nid; q++) {
+				if (kind[q] == 1 || (kind[q] == 2 && seq[kcol[q]] == bres[q])) {
+					sh[ns].col = colv[q];
+					sh[ns].res = resv[q];
+					sh[ns].delta = 1.0;
+					ns++;
+				}
+			}
+			if (!ns)
+				continue;
+			const double *zn = Z + n * ORA_NA * L;
+			/* distinct affected columns */
+			long cols[2];
+			int nc = 0;
+			for (int s = 0; s < ns; s++) {
+				int dup = 0;
+				for (int c = 0; c < nc; c++)
+					dup |= cols[c] == sh[s].col;
+				if (!dup)
+					cols[nc++] = sh[s].col;
+			}
+			for (int c = 0; c < nc; c++) {
+				double zp[ORA_NA], zm[ORA_NA];
+				for (int a = 0; a < ORA_NA; a++)
+					zp[a] = zm[a] = zn[a * L + cols[c]];
+				for (int s = 0; s < ns; s++)
+					if (sh[s].col == cols[c]) {
+						zp[sh[s].res] += h * sh[s].delta;
+						zm[sh[s].res] -= h * sh[s].delta;
+					}
+				const int xo = seq[cols[c]];
+				total += w[n] * (ora_term_f64(zp, xo) - ora_term_f64(zm, xo));
+			}
+		}
+		out[p] = total / (2.0 * h);
+	}
+	free(Z);
+}

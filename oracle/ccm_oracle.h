@@ -13,6 +13,7 @@
  *   ora_evaluate_f32/_f64   brute-force FP64, x=0 closed forms, tied-pair
  *                           central differences, invariants  (tests/test_oracle_pll.py)
  *   ora_evaluate_omp_f32    agreement with ora_evaluate_f32  (tests/test_oracle_pll.py)
+ *   ora_tied_fd_f64         full re-evaluation central differences (tests/test_oracle_pll.py)
  *   ora_sum_submatrices, ora_apc, ora_normalize
  *                           hand-computed cases              (tests/test_oracle_misc.py)
  *   ora_cg                  parity unpinned: the reference optimizer
@@ -72,6 +73,12 @@ void ora_forward_lse_f32(const unsigned char *msa, int ncol, int nrow, const flo
                          float *lse);
 void ora_forward_lse_f64(const unsigned char *msa, int ncol, int nrow, const double *x,
                          double *lse);
+
+/* numdif.c:5-57 methodology: FP64 central differences of f along
+ * e_{idx_a} + e_{idx_b} (one unit vector when idx_a == idx_b) */
+void ora_tied_fd_f64(const unsigned char *msa, const double *w, int ncol, int nrow,
+                     double lambda_single, double lambda_pair, const double *x, const long *idx_a,
+                     const long *idx_b, int count, double h, double *out);
 
 /* evaluate_cpu_omp.c:18-189, the timed CPU baseline */
 typedef struct ora_omp_ctx ora_omp_ctx;

@@ -197,3 +197,44 @@ def cg_pll_f32(msa, w, x0, lambda_single, lambda_pair, max_iterations=50, epsilo
                                C.byref(p), C.byref(fx), C.byref(it), C.byref(ne), hist)
     return dict(ret=ret, x=x, fx=fx.value, iterations=it.value, evaluations=ne.value,
                 fhist=hist[: it.value])
+
+
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+def tied_fd_f64(msa, w, x, lambda_single, lambda_pair, idx_a, idx_b, h=1e-5):
+    """FP64 central differences of f along e_a + e_b (e_a alone when a == b)."""
+    L_ = lib()
+    if not getattr(L_, "_fd_declared", False):
+        L_.ora_tied_fd_f64.argtypes = [_u8p, _f64p, C.c_int, C.c_int, C.c_double, C.c_double, _f64p,
+                                       _i64p, _i64p, C.c_int, C.c_double, _f64p]
+        L_._fd_declared = True
+    msa = _msa(msa)
+    ia = np.ascontiguousarray(idx_a, dtype=np.int64)
+    ib = np.ascontiguousarray(idx_b, dtype=np.int64)
+    out = np.zeros(len(ia), dtype=np.float64)
+    L_.ora_tied_fd_f64(msa, np.ascontiguousarray(w, dtype=np.float64), msa.shape[1], msa.shape[0],
+                       float(lambda_single), float(lambda_pair), np.ascontiguousarray(x, dtype=np.float64),
+                       ia, ib, len(ia), float(h), out)
+    return out
+
+
+def random_tied_coordinates(ncol, count, rng, singles_frac=0.2):
+    """Random (idx_a, idx_b) pairs: non-self, non-padding couplings with their
+    mirror, and single potentials (idx_b == idx_a)."""
+    L = ncol
+    nsp = nsingle_padded(L)
+    ia, ib = [], []
+    while len(ia) < count:
+        if rng.random() < singles_frac:
+            v = int(rng.integers(0, 20 * L))
+            ia.append(v)
+            ib.append(v)
+            continue
+        b, a = (int(t) for t in rng.integers(0, NA, 2))
+        k, j = (int(t) for t in rng.integers(0, L, 2))
+        if k == j:
+            continue
+        ia.append(nsp + ((b * L + k) * NAP + a) * L + j)
+        ib.append(nsp + ((a * L + j) * NAP + b) * L + k)
+    return np.array(ia, dtype=np.int64), np.array(ib, dtype=np.int64)

@@ -156,3 +156,24 @@ def test_omp_matches_serial():
     ev.close()
     assert f1 == pytest.approx(f0, rel=1e-6)
     np.testing.assert_allclose(g1, g0, rtol=1e-5, atol=1e-6)
+
+
+def test_fast_tied_fd_matches_full_reevaluation():
+    """ora_tied_fd_f64 (incremental) against central differences by full
+    re-evaluation, and against the oracle's tied gradient."""
+    rng = np.random.default_rng(21)
+    msa, _ = synthetic_msa(11, 40, seed=21, n_pairs=2)
+    N, L = msa.shape
+    w = ora.calculate_weights(msa, 0.8).astype(np.float64)
+    x = random_symmetric_x(L, rng, scale=0.05)
+    lam1, lam2 = 0.01, 0.2 * (L - 1)
+    ia, ib = ora.random_tied_coordinates(L, 40, rng)
+    # include a self-edge coupling and a padding entry: no likelihood term
+    nsp = nsingle_padded(L)
+    ia = np.append(ia, [nsp + ((3 * L + 2) * NAP + 5) * L + 2, nsp + ((1 * L + 0) * NAP + 25) * L + 4])
+    ib = np.append(ib, [nsp + ((5 * L + 2) * NAP + 3) * L + 2, nsp + ((1 * L + 0) * NAP + 25) * L + 4])
+    fast = ora.tied_fd_f64(msa, w, x, lam1, lam2, ia, ib)
+    slow = _tied_fd(msa, w, x, lam1, lam2, list(zip(ia.tolist(), ib.tolist())))
+    np.testing.assert_allclose(fast, slow, rtol=1e-6, atol=1e-6)
+    f, g = ora.evaluate_f64(msa, w, x, lam1, lam2)
+    np.testing.assert_allclose(fast[:40], g[ia[:40]], rtol=2e-6, atol=2e-6)
