@@ -72,6 +72,13 @@ EXPORTED = [
 def lib():
     global _lib
     if _lib is None:
+        # torch ships its own HIP runtime under the same soname as the system
+        # one; load it first so that this library binds to the same runtime
+        # instead of pulling in a second copy that torch then fails to use.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with `make -C ccmpred_amd/csrc` "

@@ -5,15 +5,19 @@
  * design within the reference's parameter set:
  *
  *   s0 = -g0, alpha0 = 1/|g0|
- *   per iteration, line search along s from x (x moves in place):
+ *   per iteration, line search along s from x (x moves in place), at most
+ *   max_linesearch trials:
  *     x += (alpha - moved) s ; f_t, g_t = evaluate(x)
- *     Armijo:    f_t <= f0 + ftol * alpha * <g0, s>
- *     curvature: <g_t, s> <= wolfe * |<g0, s>|   (not far past the minimum)
- *     accept when both hold, or when Armijo holds on the last allowed trial;
- *     otherwise alpha *= alpha_mul (at most max_linesearch trials)
- *   on failure x returns to the start; a failed steepest-descent search ends
- *   the run with CONJUGRADERR_MAXIMUMLINESEARCH, otherwise the direction is
- *   reset to -g once
+ *     Armijo fails (f_t > f0 + ftol alpha <g0,s>): hi = alpha, shrink
+ *       (alpha *= alpha_mul, or bisect [lo, hi] once a lower bound exists)
+ *     strong Wolfe (|<g_t,s>| <= wolfe |<g0,s>|), or Armijo on the last
+ *       trial: accept
+ *     otherwise <g_t,s> > 0 sets hi, < 0 sets lo; extend (alpha /= alpha_mul)
+ *       while unbracketed, else bisect
+ *   if the last trial fails Armijo the step returns to the longest one that
+ *   passed (one more evaluation); with none, x returns to the start, and a
+ *   failed steepest-descent search ends the run with
+ *   CONJUGRADERR_MAXIMUMLINESEARCH, otherwise the direction is reset to -g
  *   beta = max(0, (<g,g> - <g,g_prev>) / <g_prev,g_prev>)   (Polak-Ribiere+)
  *   s = -g + beta s; if <g, s> >= 0 then s = -g
  *   next alpha = alpha * <g_prev, s_prev> / <g, s>
@@ -114,7 +118,8 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 		float *tmp = gprev;
 		gprev = g;
 		g = tmp;
-		double moved = 0.0, ft = 0.0;
+		/* line search along s; x moves in place, `moved` is its offset */
+		double moved = 0.0, ft = 0.0, lo = 0.0, hi = INFINITY;
 		int accepted = 0, ls;
 		for (ls = 1; ls <= p->max_linesearch; ls++) {
 			if (ccm_launch_axpy(d_x, sdir, (float)(alpha - moved), n, c.s) || sync_stream(&c)) {
@@ -128,19 +133,42 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 				ret = CONJUGRADERR_DEVICE;
 				goto done;
 			}
-			if (ft <= f0 + p->ftol * alpha * dg0) {
-				double dgt;
-				if (dots(&c, g, sdir, NULL, NULL, 1, &dgt)) {
-					ret = CONJUGRADERR_DEVICE;
-					goto done;
-				}
-				if (dgt <= -p->wolfe * dg0 || ls == p->max_linesearch) {
-					accepted = 1;
-					break;
-				}
+			if (!(ft <= f0 + p->ftol * alpha * dg0)) {
+				/* not enough decrease: shrink, bisecting once bracketed */
+				hi = alpha;
+				alpha = lo > 0.0 ? 0.5 * (lo + hi) : alpha * p->alpha_mul;
+				continue;
 			}
-			if (ls < p->max_linesearch)
-				alpha *= p->alpha_mul;
+			double dgt;
+			if (dots(&c, g, sdir, NULL, NULL, 1, &dgt)) {
+				ret = CONJUGRADERR_DEVICE;
+				goto done;
+			}
+			if (fabs(dgt) <= -p->wolfe * dg0 || ls == p->max_linesearch) {
+				accepted = 1;
+				break;
+			}
+			if (dgt > 0.0)
+				hi = alpha; /* past the minimum along s */
+			else
+				lo = alpha; /* still descending steeply: extend */
+			alpha = isinf(hi) ? alpha / p->alpha_mul : 0.5 * (lo + hi);
+		}
+		if (!accepted && lo > 0.0) {
+			/* last trial failed the decrease test: return to the longest
+			 * step that passed it */
+			if (ccm_launch_axpy(d_x, sdir, (float)(lo - moved), n, c.s) || sync_stream(&c)) {
+				ret = CONJUGRADERR_DEVICE;
+				goto done;
+			}
+			moved = alpha = lo;
+			ft = evaluate(instance, d_x, g, n);
+			nevals++;
+			if (isnan(ft)) {
+				ret = CONJUGRADERR_DEVICE;
+				goto done;
+			}
+			accepted = 1;
 		}
 		if (!accepted) {
 			if (ccm_launch_axpy(d_x, sdir, (float)(-moved), n, c.s) || sync_stream(&c)) {

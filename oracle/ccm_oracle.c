@@ -397,7 +397,8 @@ int ora_cg(int n, float *x, double *fx, ora_eval_fn eval, void *ctx, const ora_c
 		float *tmp = gprev;
 		gprev = g;
 		g = tmp;
-		double moved = 0.0, ft = 0.0;
+		/* line search along s; x moves in place, `moved` is its offset */
+		double moved = 0.0, ft = 0.0, lo = 0.0, hi = INFINITY;
 		int accepted = 0, ls;
 		for (ls = 1; ls <= p->max_linesearch; ls++) {
 			const float step = (float)(alpha - moved);
@@ -406,15 +407,34 @@ int ora_cg(int n, float *x, double *fx, ora_eval_fn eval, void *ctx, const ora_c
 			moved = alpha;
 			ft = eval(ctx, x, g, n);
 			nevals++;
-			if (ft <= f0 + p->ftol * alpha * dg0) {
-				double dgt = ora_dot(g, s, n);
-				if (dgt <= -p->wolfe * dg0 || ls == p->max_linesearch) {
-					accepted = 1;
-					break;
-				}
+			if (!(ft <= f0 + p->ftol * alpha * dg0)) {
+				/* not enough decrease: shrink, bisecting once bracketed */
+				hi = alpha;
+				alpha = lo > 0.0 ? 0.5 * (lo + hi) : alpha * p->alpha_mul;
+				continue;
 			}
-			if (ls < p->max_linesearch)
-				alpha *= p->alpha_mul;
+			const double dgt = ora_dot(g, s, n);
+			if (fabs(dgt) <= -p->wolfe * dg0 || ls == p->max_linesearch) {
+				accepted = 1;
+				break;
+			}
+			if (dgt > 0.0) {
+				hi = alpha; /* past the minimum along s */
+			} else {
+				lo = alpha; /* still descending steeply: extend */
+			}
+			alpha = isinf(hi) ? alpha / p->alpha_mul : 0.5 * (lo + hi);
+		}
+		if (!accepted && lo > 0.0) {
+			/* last trial failed the decrease test: return to the longest
+			 * step that passed it */
+			const float step = (float)(lo - moved);
+			for (long i = 0; i < n; i++)
+				x[i] += step * s[i];
+			moved = alpha = lo;
+			ft = eval(ctx, x, g, n);
+			nevals++;
+			accepted = 1;
 		}
 		if (!accepted) {
 			const float back = (float)(-moved);
@@ -637,4 +657,41 @@ void ora_tied_fd_f64(const unsigned char *msa, const double *w, int ncol, int nr
 		out[p] = total / (2.0 * h);
 	}
 	free(Z);
+}
+
+typedef struct {
+	const unsigned char *msa;
+	const double *w;
+	int ncol, nrow;
+	double l1, l2;
+	double *xd, *gd;
+} ora_pll64_ctx;
+
+static double ora_pll_eval64(void *vctx, const float *x, float *g, int n)
+{
+	ora_pll64_ctx *c = (ora_pll64_ctx *)vctx;
+	for (long i = 0; i < n; i++)
+		c->xd[i] = x[i];
+	double f = ora_evaluate_f64(c->msa, c->w, c->ncol, c->nrow, c->l1, c->l2, c->xd, c->gd);
+	for (long i = 0; i < n; i++)
+		g[i] = (float)c->gd[i];
+	return f;
+}
+
+int ora_cg_pll_f64(const unsigned char *msa, const float *w, int ncol, int nrow,
+                   float lambda_single, float lambda_pair, float *x, const ora_cg_param *p,
+                   double *fx, int *iterations, int *evaluations, double *fhist)
+{
+	const long n = ora_nvar_padded(ncol);
+	double *wd = (double *)malloc(sizeof(double) * (size_t)nrow);
+	for (long i = 0; i < nrow; i++)
+		wd[i] = w[i];
+	ora_pll64_ctx c = {msa, wd, ncol, nrow, lambda_single, lambda_pair,
+	                   (double *)malloc(sizeof(double) * (size_t)n),
+	                   (double *)malloc(sizeof(double) * (size_t)n)};
+	int r = ora_cg((int)n, x, fx, ora_pll_eval64, &c, p, iterations, evaluations, fhist);
+	free(c.xd);
+	free(c.gd);
+	free(wd);
+	return r;
 }

@@ -95,8 +95,9 @@ void ora_normalize(float *mat, int ncol);
 
 /*
  * Host nonlinear conjugate gradient with the same algorithm as the device
- * CG in ccmpred_amd/csrc/host/cg_device.c (Polak-Ribiere+ beta, backtracking
- * line search with Armijo and curvature tests, FP64 scalars, FP32 vectors).
+ * CG in ccmpred_amd/csrc/host/cg_device.c (Polak-Ribiere+ beta, line search
+ * with an Armijo test, a strong-Wolfe curvature test, extension of short
+ * steps and bisection once bracketed; FP64 scalars, FP32 vectors).
  */
 typedef double (*ora_eval_fn)(void *ctx, const float *x, float *g, int n);
 typedef struct ora_cg_param {
@@ -110,6 +111,12 @@ typedef struct ora_cg_param {
 } ora_cg_param;
 int ora_cg(int n, float *x, double *fx, ora_eval_fn eval, void *ctx, const ora_cg_param *p,
            int *iterations, int *evaluations, double *fhist);
+
+/* Drives ora_cg with the FP64 oracle evaluator (FP32 vectors as on the
+ * device, f and g evaluated in FP64). */
+int ora_cg_pll_f64(const unsigned char *msa, const float *w, int ncol, int nrow,
+                   float lambda_single, float lambda_pair, float *x, const ora_cg_param *p,
+                   double *fx, int *iterations, int *evaluations, double *fhist);
 
 /* Drives ora_cg with the FP32 oracle evaluator. */
 int ora_cg_pll_f32(const unsigned char *msa, const float *w, int ncol, int nrow,

@@ -71,6 +71,8 @@ def _declare(L):
                                  C.POINTER(CGParam), C.POINTER(C.c_double), C.POINTER(C.c_int),
                                  C.POINTER(C.c_int), _f64p]
     L.ora_cg_pll_f32.restype = C.c_int
+    L.ora_cg_pll_f64.argtypes = L.ora_cg_pll_f32.argtypes
+    L.ora_cg_pll_f64.restype = C.c_int
 
 
 def nsingle_padded(ncol: int) -> int:
@@ -184,7 +186,7 @@ def normalize(mat):
 
 
 def cg_pll_f32(msa, w, x0, lambda_single, lambda_pair, max_iterations=50, epsilon=0.0, k=5,
-               max_linesearch=5, alpha_mul=0.5, ftol=1e-4, wolfe=0.2):
+               max_linesearch=5, alpha_mul=0.5, ftol=1e-4, wolfe=0.2, evaluator="f32"):
     msa = _msa(msa)
     x = np.array(x0, dtype=np.float32, copy=True)
     p = CGParam(max_iterations, epsilon, k, max_linesearch, alpha_mul, ftol, wolfe)
@@ -192,7 +194,8 @@ def cg_pll_f32(msa, w, x0, lambda_single, lambda_pair, max_iterations=50, epsilo
     it = C.c_int()
     ne = C.c_int()
     hist = np.zeros(max_iterations + 1, dtype=np.float64)
-    ret = lib().ora_cg_pll_f32(msa, np.ascontiguousarray(w, dtype=np.float32), msa.shape[1],
+    fn = lib().ora_cg_pll_f32 if evaluator == "f32" else lib().ora_cg_pll_f64
+    ret = fn(msa, np.ascontiguousarray(w, dtype=np.float32), msa.shape[1],
                                msa.shape[0], float(lambda_single), float(lambda_pair), x,
                                C.byref(p), C.byref(fx), C.byref(it), C.byref(ne), hist)
     return dict(ret=ret, x=x, fx=fx.value, iterations=it.value, evaluations=ne.value,

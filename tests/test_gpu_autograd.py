@@ -62,16 +62,17 @@ def test_autograd_against_fd_and_reference(prec, tol):
         f64, g64 = ora.evaluate_f64(msa, w64, x.astype(np.float64), p.lambda_single, p.lambda_pair)
         f32, g32 = ora.evaluate_f32(msa, p.weights, x, p.lambda_single, p.lambda_pair)
     scale = np.sqrt(np.mean(fd ** 2))
-    err_fd = np.max(np.abs(g[ia] - fd)) / scale
+    err_fd = np.linalg.norm(g[ia] - fd) / np.linalg.norm(fd)
     err_64 = np.linalg.norm(g - g64) / np.linalg.norm(g64)
     err_32 = np.linalg.norm(g - g32) / np.linalg.norm(g32)
-    say(f"  f={float(f):.9g} f64={f64:.9g}  max|g-fd|/rms(fd)={err_fd:.2e}  "
+    say(f"  f={float(f.detach()):.9g} f64={f64:.9g}  |g-fd|/|fd| over the probes={err_fd:.2e}  "
         f"rel_l2 vs fp64={err_64:.2e} vs fp32 evaluate_cpu={err_32:.2e}")
+    say(f"  max|g-fd|/rms(fd)={np.max(np.abs(g[ia] - fd)) / scale:.2e}")
     # the oracle gradient itself matches the FD probes
     assert np.max(np.abs(g64[ia] - fd)) / scale < 1e-6
-    assert abs(float(f) - f64) / abs(f64) < 1e-5
+    assert abs(float(f.detach()) - f64) / abs(f64) < 1e-5
     assert err_64 <= tol and err_32 <= tol
-    assert err_fd <= 10 * tol
+    assert err_fd <= 2 * tol
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
