@@ -35,7 +35,8 @@ class UserData(C.Structure):
 
 class HipConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("bf16_passes", C.c_int),
-                ("verbose", C.c_int), ("chunk_rows", C.c_int)]
+                ("bf16_fwd_passes", C.c_int), ("verbose", C.c_int), ("chunk_rows", C.c_int),
+                ("gemm_variant", C.c_int)]
 
 
 class CGParam(C.Structure):
@@ -62,7 +63,7 @@ EXPORTED = [
     "ccm_launch_neg_weighted_onehot", "ccm_softmax_rows_blocks", "ccm_softmax_rows_nblk",
     "ccm_softmax_cols_blocks", "ccm_softmax_cols_nblk", "ccm_launch_softmax_rows",
     "ccm_launch_softmax_cols", "ccm_launch_convert_wt", "ccm_launch_gemm_fwd",
-    "ccm_launch_gemm_grad", "ccm_pair_epilogue_blocks", "ccm_singles_blocks",
+    "ccm_launch_gemm_grad", "ccm_gemm_variants", "ccm_gemm_default_variant", "ccm_pair_epilogue_blocks", "ccm_singles_blocks",
     "ccm_launch_pair_epilogue", "ccm_launch_hsym", "ccm_launch_singles_finalize",
     "ccm_launch_sum3", "ccm_dots_partial_len", "ccm_launch_dots", "ccm_launch_axpy",
     "ccm_launch_direction", "ccm_launch_negate",

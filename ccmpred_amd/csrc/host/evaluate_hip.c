@@ -116,8 +116,10 @@ void ccm_hip_default_config(ccm_hip_config *cfg)
 	cfg->device = 0;
 	cfg->precision = CCM_PRECISION_FP32;
 	cfg->bf16_passes = 1;
+	cfg->bf16_fwd_passes = 0;
 	cfg->verbose = 1;
 	cfg->chunk_rows = 0;
+	cfg->gemm_variant = -1;
 }
 
 static hipError_t dalloc(ccm_hip_state *st, void **p, size_t bytes)
@@ -165,8 +167,10 @@ int init_hip_ex(void *instance, const ccm_hip_config *cfg_in)
 		cfg = *cfg_in;
 	else
 		ccm_hip_default_config(&cfg);
-	if (cfg.bf16_passes < 1 || cfg.bf16_passes > 2 || cfg.chunk_rows < 0 || ud->ncol <= 0 ||
-	    ud->nrow <= 0) {
+	if (cfg.bf16_fwd_passes == 0)
+		cfg.bf16_fwd_passes = cfg.bf16_passes;
+	if (cfg.bf16_passes < 1 || cfg.bf16_passes > 2 || cfg.bf16_fwd_passes < 1 ||
+	    cfg.bf16_fwd_passes > 2 || cfg.chunk_rows < 0 || ud->ncol <= 0 || ud->nrow <= 0) {
 		fprintf(stderr, "ccmpred_amd: invalid configuration\n");
 		return 1;
 	}
@@ -181,7 +185,7 @@ int init_hip_ex(void *instance, const ccm_hip_config *cfg_in)
 	const int L = ud->ncol, N = ud->nrow;
 	st->L = L;
 	st->N = N;
-	st->Lp = (int)round_up(L, 32);
+	st->Lp = (int)round_up(L, 64);
 	st->Np = (int)round_up(N, 128);
 	st->P = CCM_N_ALPHA * st->Lp;
 	st->nsp = ccm_nsingle_padded(L);
@@ -276,7 +280,7 @@ int init_hip_ex(void *instance, const ccm_hip_config *cfg_in)
 		st->n_g1blk = ccm_softmax_rows_nblk(N);
 	} else {
 		const int passes = cfg.bf16_passes;
-		st->ldwt = (long)passes * P;
+		st->ldwt = (long)st->cfg.bf16_fwd_passes * P;
 		st->lddt = (long)passes * st->ccols;
 		HIPCHK(dalloc(st, (void **)&st->d_wt, sizeof(uint16_t) * (size_t)P * st->ldwt));
 		HIPCHK(dalloc(st, (void **)&st->d_zt, sizeof(float) * (size_t)P * st->ccols));
@@ -415,8 +419,8 @@ static int run_chunks(ccm_hip_state *st, const float *d_x, int what, int *step)
 			const int ncols = (int)round_up(nc, 128);
 			if (first)
 				MARK((*step)++);
-			KCHK(ccm_launch_gemm_fwd(st->d_wt, st->ldwt, Lp, passes, msap, ncols, st->d_zt,
-			                         (long)ncols, s));
+			KCHK(ccm_launch_gemm_fwd(st->d_wt, st->ldwt, Lp, st->cfg.bf16_fwd_passes, msap, ncols,
+			                         st->d_zt, (long)ncols, st->cfg.gemm_variant, s));
 			if (first)
 				MARK((*step)++);
 			KCHK(ccm_launch_softmax_cols(st->d_zt, (long)ncols, d_x, st->d_msapt + n0, (long)Np,
@@ -428,7 +432,7 @@ static int run_chunks(ccm_hip_state *st, const float *d_x, int what, int *step)
 			if (what == RUN_FULL)
 				KCHK(ccm_launch_gemm_grad(st->d_dt, (long)passes * ncols, Lp, passes,
 				                          st->d_msapt + n0, (long)Np, ncols, st->d_gt, (long)P,
-				                          !first, s));
+				                          !first, st->cfg.gemm_variant, s));
 		}
 	}
 	return 0;
@@ -445,7 +449,8 @@ static int pair_gradient_from_forward(ccm_hip_state *st)
 		                                st->d_graw, st->ldg, 0, s));
 	else
 		KCHK(ccm_launch_gemm_grad(st->d_dt, st->lddt, st->Lp, st->cfg.bf16_passes, st->d_msapt,
-		                          (long)st->Np, st->ccols, st->d_gt, (long)st->P, 0, s));
+		                          (long)st->Np, st->ccols, st->d_gt, (long)st->P, 0,
+		                          st->cfg.gemm_variant, s));
 	return 0;
 fail:
 	return -1;
@@ -491,7 +496,7 @@ int ccm_hip_evaluate(void *instance, const float *d_x, float *d_g, int nvar, dou
 	st->forward_valid = 0;
 	if (st->cfg.precision != CCM_PRECISION_FP32) {
 		MARK(step++);
-		KCHK(ccm_launch_convert_wt(d_x, st->L, st->Lp, st->d_wt, st->ldwt, st->cfg.bf16_passes,
+		KCHK(ccm_launch_convert_wt(d_x, st->L, st->Lp, st->d_wt, st->ldwt, st->cfg.bf16_fwd_passes,
 		                           s));
 	}
 	if (run_chunks(st, d_x, RUN_FULL, &step))
@@ -536,7 +541,7 @@ int ccm_hip_objective(void *instance, const float *d_x, int nvar, double *f)
 	st->timing = 0;
 	st->forward_valid = 0;
 	if (st->cfg.precision != CCM_PRECISION_FP32)
-		KCHK(ccm_launch_convert_wt(d_x, st->L, st->Lp, st->d_wt, st->ldwt, st->cfg.bf16_passes,
+		KCHK(ccm_launch_convert_wt(d_x, st->L, st->Lp, st->d_wt, st->ldwt, st->cfg.bf16_fwd_passes,
 		                           s));
 	if (run_chunks(st, d_x, RUN_OBJECTIVE, &step))
 		goto fail;
@@ -564,7 +569,7 @@ int ccm_hip_forward(void *instance, const float *d_x, int nvar, double *f)
 	st->timing = 0;
 	st->forward_valid = 0;
 	if (st->cfg.precision != CCM_PRECISION_FP32)
-		KCHK(ccm_launch_convert_wt(d_x, st->L, st->Lp, st->d_wt, st->ldwt, st->cfg.bf16_passes,
+		KCHK(ccm_launch_convert_wt(d_x, st->L, st->Lp, st->d_wt, st->ldwt, st->cfg.bf16_fwd_passes,
 		                           s));
 	if (run_chunks(st, d_x, RUN_FORWARD, &step))
 		goto fail;

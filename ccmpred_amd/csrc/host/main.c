@@ -50,7 +50,8 @@ static void usage(const char *exe, int full)
 		       "[default: 0.2]\n");
 		printf("\t-A        \tDisable average product correction (APC)\n");
 		printf("\t-R        \tRe-normalize output matrix to [0,1]\n");
-		printf("\t-P PREC   \tPrecision form: fp32, bf16 or bf16x2 [default: fp32]\n");
+		printf("\t-P PREC   \tPrecision form: fp32, bf16, bf16f (two-pass forward) or bf16x2 "
+		       "[default: fp32]\n");
 		printf("\t-h        \tDisplay help\n\n");
 	}
 	exit(1);
@@ -98,6 +99,11 @@ int main(int argc, char **argv)
 				} else if (!strcmp(val, "bf16x2")) {
 					cfg.precision = CCM_PRECISION_BF16;
 					cfg.bf16_passes = 2;
+					cfg.bf16_fwd_passes = 2;
+				} else if (!strcmp(val, "bf16f")) {
+					cfg.precision = CCM_PRECISION_BF16;
+					cfg.bf16_passes = 1;
+					cfg.bf16_fwd_passes = 2;
 				} else {
 					usage(argv[0], 0);
 				}

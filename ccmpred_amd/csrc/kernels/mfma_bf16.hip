@@ -18,11 +18,6 @@
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-#define GM_TM 256
-#define GM_TN 128
-#define GM_TK 32
-#define GM_LDA (GM_TK + 8)  // LDS row stride in bf16 (80 bytes)
-
 // coupling matrix -> Wt (bf16, P x lda), zero outside the real 21L x 21L block
 __global__ __launch_bounds__(256) void k_convert_wt(const float* __restrict__ x2, int L, int Lp,
                                                     uint16_t* __restrict__ wt, long lda, int passes,
@@ -73,102 +68,117 @@ __device__ __forceinline__ bf16x8_t onehot8(uint2 v, uint32_t b) {
   return __builtin_bit_cast(bf16x8_t, o);
 }
 
-// MODE 0: forward.  cols = sequences n;      B[kk][n] = [msap[n][kk%Lp] == kk/Lp], Kb = P
-// MODE 1: gradient. cols = coupling rows r;  B[kk][r] = [msapt[r%Lp][kk] == r/Lp], Kb = Np
-template <int MODE>
-__global__ __launch_bounds__(256, 2) void k_gemm_onehot(const uint16_t* __restrict__ A, long lda,
-                                                        int M, int Kb, int passes,
-                                                        const uint8_t* __restrict__ bytes,
-                                                        long ldbytes, int Lp, int Ncol,
-                                                        float* __restrict__ C, long ldc,
-                                                        int accumulate, uint32_t tiles_n,
-                                                        uint32_t tiles, uint32_t total) {
-  __shared__ __attribute__((aligned(16))) uint16_t As[2][GM_TM * GM_LDA];
+// One workgroup computes a TM x TN tile of C with WM x WN waves; each wave
+// owns RB x CB blocks of 32 x 32 and issues v_mfma_f32_32x32x16_bf16.  The
+// dense operand moves through LDS in K steps of TK (two stages, the next one
+// prefetched through registers while the current one is consumed); the
+// one-hot operand is synthesised per lane from 8 MSA bytes per fragment.
+//
+// MODE 0: forward.  cols = sequences n;     B[kk][n] = [msap[n][kk%Lp] == kk/Lp], Kb = P
+// MODE 1: gradient. cols = coupling rows r; B[kk][r] = [msapt[r%Lp][kk] == r/Lp], Kb = chunk
+// Requires Lp % TK == 0 (forward) and Kb % TK == 0.
+template <int MODE, int WM, int WN, int RB, int CB, int TK>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_onehot(
+    const uint16_t* __restrict__ A, long lda, int M, int Kb, int passes,
+    const uint8_t* __restrict__ bytes, long ldbytes, int Lp, int Ncol, float* __restrict__ C,
+    long ldc, int accumulate, uint32_t tiles_n, uint32_t tiles, uint32_t total) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int TM = WM * RB * 32, TN = WN * CB * 32;
+  constexpr int LDA = TK + 8;  // LDS row stride in bf16
+  constexpr int KCH = TK / 8;  // 16-byte chunks per row and step
+  constexpr int CPT = TM * KCH / NT;
+  static_assert((TM * KCH) % NT == 0, "loader mapping");
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][TM * LDA];
+
   const uint32_t qid = ccm_xcd_remap(blockIdx.x, total);
   if (qid >= tiles) return;
   const int mt = qid / tiles_n, nt = qid % tiles_n;
-  const int m0 = mt * GM_TM, n0 = nt * GM_TN;
+  const int m0 = mt * TM, n0 = nt * TN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WN, wn = wave % WN;
   const int l32 = lane & 31, q = lane >> 5;
 
-  // per-lane one-hot column information for the 2 column blocks
-  int colv[2];
-  uint32_t bval[2];
-  const uint8_t* bptr[2];
+  int colv[CB];
+  uint32_t bval[CB];
+  const uint8_t* bptr[CB];
 #pragma unroll
-  for (int cb = 0; cb < 2; cb++) {
-    const int col = n0 + wn * 64 + cb * 32 + l32;
+  for (int cb = 0; cb < CB; cb++) {
+    const int col = n0 + (wn * CB + cb) * 32 + l32;
     colv[cb] = col;
     if (MODE == 0) {
       bval[cb] = 0;
-      bptr[cb] = bytes + (long)col * ldbytes;  // msap row n (Np rows, always allocated)
+      // msap row n; columns past Ncol read row Ncol-1 and are never stored
+      bptr[cb] = bytes + (long)(col < Ncol ? col : Ncol - 1) * ldbytes;
     } else {
-      const int bb = col / Lp, kk = col % Lp;
-      bval[cb] = (uint32_t)bb;                 // >= 21 past the real rows: never matches
-      bptr[cb] = bytes + (long)kk * ldbytes;   // msapt row k
+      const int bb = col / Lp, kk = col - bb * Lp;
+      bval[cb] = (uint32_t)bb;                // >= 21 past the real rows: never matches
+      bptr[cb] = bytes + (long)kk * ldbytes;  // msapt row k
     }
   }
 
-  floatx16 acc[4][2];
+  floatx16 acc[RB][CB];
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < RB; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = (floatx16)0.0f;
+    for (int j = 0; j < CB; j++) acc[i][j] = (floatx16)0.0f;
 
-  const int K = passes * Kb;
-  const int nsteps = K / GM_TK;
-  // A tile loader: thread t loads row m0+t, 32 bf16 = 4 x 16 bytes
-  const int arow = m0 + tid;
-  const bool arow_ok = arow < M;
-  const uint4* asrc = reinterpret_cast<const uint4*>(A + (long)(arow_ok ? arow : 0) * lda);
-  uint4 pre[4];
+  const int nsteps = passes * Kb / TK;
+  // loader: chunk c = tid + i*NT -> row c / KCH, 16-byte column c % KCH
+  const uint4* asrc[CPT];
+  int adst[CPT];
+  bool aok[CPT];
 #pragma unroll
-  for (int i = 0; i < 4; i++) pre[i] = arow_ok ? asrc[i] : make_uint4(0, 0, 0, 0);
-  {
-    uint4* dst = reinterpret_cast<uint4*>(&As[0][tid * GM_LDA]);
-#pragma unroll
-    for (int i = 0; i < 4; i++) dst[i] = pre[i];
+  for (int i = 0; i < CPT; i++) {
+    const int c = tid + i * NT;
+    const int row = c / KCH, col8 = c % KCH;
+    aok[i] = m0 + row < M;
+    asrc[i] = reinterpret_cast<const uint4*>(A + (long)(aok[i] ? m0 + row : 0) * lda) + col8;
+    adst[i] = row * LDA + col8 * 8;
   }
+  uint4 pre[CPT];
+#pragma unroll
+  for (int i = 0; i < CPT; i++) pre[i] = aok[i] ? asrc[i][0] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < CPT; i++) *reinterpret_cast<uint4*>(&As[0][adst[i]]) = pre[i];
   __syncthreads();
 
   // uniform position of the current K step inside one pass: for the forward
-  // the coupling row block bb and the column offset kin (kin + 32 <= Lp since
-  // Lp is a multiple of 32); for the gradient the sequence offset kin
+  // the coupling row block bb and the column offset kin; for the gradient
+  // the sequence offset kin
   int bb = 0, kin = 0;
   for (int s = 0; s < nsteps; s++) {
     const int cur = s & 1;
     if (s + 1 < nsteps) {
-      const uint4* src = asrc + (long)(s + 1) * (GM_TK / 8);
 #pragma unroll
-      for (int i = 0; i < 4; i++) pre[i] = arow_ok ? src[i] : make_uint4(0, 0, 0, 0);
+      for (int i = 0; i < CPT; i++)
+        pre[i] = aok[i] ? asrc[i][(long)(s + 1) * KCH] : make_uint4(0, 0, 0, 0);
     }
+    const uint16_t* as = As[cur];
 #pragma unroll
-    for (int sub = 0; sub < 2; sub++) {
+    for (int sub = 0; sub < TK / 16; sub++) {
       const int koff = kin + sub * 16 + 8 * q;  // first of this lane's 8 K indices
-      bf16x8_t bfrag[2];
+      bf16x8_t bfrag[CB];
 #pragma unroll
-      for (int cb = 0; cb < 2; cb++) {
+      for (int cb = 0; cb < CB; cb++) {
         const uint2 v = *reinterpret_cast<const uint2*>(bptr[cb] + koff);
         bfrag[cb] = onehot8(v, MODE == 0 ? (uint32_t)bb : bval[cb]);
       }
 #pragma unroll
-      for (int rb = 0; rb < 4; rb++) {
-        const uint4 av =
-            *reinterpret_cast<const uint4*>(&As[cur][(wm * 128 + rb * 32 + l32) * GM_LDA + sub * 16 + 8 * q]);
+      for (int rb = 0; rb < RB; rb++) {
+        const uint4 av = *reinterpret_cast<const uint4*>(
+            &as[((wm * RB + rb) * 32 + l32) * LDA + sub * 16 + 8 * q]);
         const bf16x8_t afrag = __builtin_bit_cast(bf16x8_t, av);
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++)
+        for (int cb = 0; cb < CB; cb++)
           acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afrag, bfrag[cb], acc[rb][cb], 0, 0, 0);
       }
     }
     if (s + 1 < nsteps) {
-      uint4* dst = reinterpret_cast<uint4*>(&As[cur ^ 1][tid * GM_LDA]);
 #pragma unroll
-      for (int i = 0; i < 4; i++) dst[i] = pre[i];
+      for (int i = 0; i < CPT; i++) *reinterpret_cast<uint4*>(&As[cur ^ 1][adst[i]]) = pre[i];
     }
     __syncthreads();
-    kin += GM_TK;
+    kin += TK;
     if (MODE == 0) {
       if (kin == Lp) {
         kin = 0;
@@ -182,14 +192,14 @@ __global__ __launch_bounds__(256, 2) void k_gemm_onehot(const uint16_t* __restri
   // 32x32 accumulator layout: lane -> column l32; register i -> row
   // (i/4)*8 + 4*q + (i%4)
 #pragma unroll
-  for (int rb = 0; rb < 4; rb++) {
+  for (int rb = 0; rb < RB; rb++) {
 #pragma unroll
-    for (int cb = 0; cb < 2; cb++) {
+    for (int cb = 0; cb < CB; cb++) {
       const int col = colv[cb];
       if (col >= Ncol) continue;
 #pragma unroll
       for (int i = 0; i < 16; i++) {
-        const int row = m0 + wm * 128 + rb * 32 + (i >> 2) * 8 + 4 * q + (i & 3);
+        const int row = m0 + (wm * RB + rb) * 32 + (i >> 2) * 8 + 4 * q + (i & 3);
         if (row < M) {
           float* dst = C + (long)row * ldc + col;
           *dst = accumulate ? *dst + acc[rb][cb][i] : acc[rb][cb][i];
@@ -214,35 +224,84 @@ int ccm_launch_convert_wt(const float* d_x, int L, int Lp, uint16_t* d_wt, long 
   return CCM_OK;
 }
 
-// Zt[c][n] (P x ldc floats, ldc >= ncols) = sum_r Wt[c][r] [x_nk = b] for the
-// ncols (multiple of 128) sequences whose padded MSA rows start at d_msap
-int ccm_launch_gemm_fwd(const uint16_t* d_wt, long lda, int Lp, int passes, const uint8_t* d_msap,
-                        int ncols, float* d_zt, long ldc, hipStream_t s) {
-  const int P = CCM_NA * Lp;
-  if (Lp % 32 || ncols % GM_TN || ncols <= 0 || lda < (long)passes * P || ldc < ncols || (lda % 8))
-    return CCM_ERR_ARG;
-  const uint32_t tm = (P + GM_TM - 1) / GM_TM, tn = ncols / GM_TN;
-  const uint32_t tiles = tm * tn, total = round8(tiles);
-  hipLaunchKernelGGL(k_gemm_onehot<0>, dim3(total), dim3(256), 0, s, d_wt, lda, P, P, passes, d_msap,
-                     (long)Lp, Lp, ncols, d_zt, ldc, 0, tn, tiles, total);
+}  // extern "C"
+
+// Tile variants: {waves M, waves N, row blocks, col blocks} with TK = 64
+//   0: 2x4 waves, 256 x 256      1: 2x2 waves, 256 x 128      2: 2x2 waves, 128 x 128
+template <int MODE>
+static int launch_gemm(int variant, const uint16_t* A, long lda, int M, int Kb, int passes,
+                       const uint8_t* bytes, long ldbytes, int Lp, int Ncol, float* C, long ldc,
+                       int accumulate, hipStream_t s) {
+  int tm, tn;
+  switch (variant) {
+    case 0: tm = 256; tn = 256; break;
+    case 1: tm = 256; tn = 128; break;
+    case 2: tm = 128; tn = 128; break;
+    default: return CCM_ERR_ARG;
+  }
+  const uint32_t tiles_m = (M + tm - 1) / tm, tiles_n = (Ncol + tn - 1) / tn;
+  const uint32_t tiles = tiles_m * tiles_n, total = round8(tiles);
+  switch (variant) {
+    case 0:
+      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 4, 4, 2, 64>), dim3(total), dim3(512), 0, s, A, lda, M,
+                         Kb, passes, bytes, ldbytes, Lp, Ncol, C, ldc, accumulate, tiles_n, tiles, total);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 2, 4, 2, 64>), dim3(total), dim3(256), 0, s, A, lda, M,
+                         Kb, passes, bytes, ldbytes, Lp, Ncol, C, ldc, accumulate, tiles_n, tiles, total);
+      break;
+    default:
+      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 2, 2, 2, 64>), dim3(total), dim3(256), 0, s, A, lda, M,
+                         Kb, passes, bytes, ldbytes, Lp, Ncol, C, ldc, accumulate, tiles_n, tiles, total);
+      break;
+  }
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }
 
+// workgroups of a launch for a tile variant
+static uint32_t gemm_tiles(int variant, int M, int Ncol) {
+  const int tm = variant == 2 ? 128 : 256, tn = variant == 0 ? 256 : 128;
+  return (uint32_t)(((M + tm - 1) / tm) * ((Ncol + tn - 1) / tn));
+}
+
+// default variant: the largest tile that still gives every CU a workgroup
+static int gemm_default_variant(int M, int Ncol) {
+  if (gemm_tiles(0, M, Ncol) >= 256) return 0;
+  if (gemm_tiles(1, M, Ncol) >= 256) return 1;
+  return 2;
+}
+
+extern "C" {
+
+int ccm_gemm_variants(void) { return 3; }
+
+int ccm_gemm_default_variant(int M, int Ncol) { return gemm_default_variant(M, Ncol); }
+
+// Zt[c][n] (P x ldc floats, ldc >= ncols) = sum_r Wt[c][r] [x_nk = b] for the
+// ncols (multiple of 128) sequences whose padded MSA rows start at d_msap;
+// variant < 0 picks the default tile
+int ccm_launch_gemm_fwd(const uint16_t* d_wt, long lda, int Lp, int passes, const uint8_t* d_msap,
+                        int ncols, float* d_zt, long ldc, int variant, hipStream_t s) {
+  const int P = CCM_NA * Lp;
+  if (Lp % 64 || ncols % 128 || ncols <= 0 || lda < (long)passes * P || ldc < ncols || (lda % 8))
+    return CCM_ERR_ARG;
+  if (variant < 0) variant = gemm_default_variant(P, ncols);
+  return launch_gemm<0>(variant, d_wt, lda, P, P, passes, d_msap, (long)Lp, Lp, ncols, d_zt, ldc, 0, s);
+}
+
 // Gt[c][r] (P x ldc floats, ldc >= P) (+)= sum_n Dt[c][n] [x_nk = b] over the
-// kcols (multiple of 32) sequences of Dt; d_msapt points at the chunk's first
+// kcols (multiple of 64) sequences of Dt; d_msapt points at the chunk's first
 // column of the transposed MSA, whose rows are ldmt bytes apart
 int ccm_launch_gemm_grad(const uint16_t* d_dt, long lda, int Lp, int passes, const uint8_t* d_msapt,
-                         long ldmt, int kcols, float* d_gt, long ldc, int accumulate, hipStream_t s) {
+                         long ldmt, int kcols, float* d_gt, long ldc, int accumulate, int variant,
+                         hipStream_t s) {
   const int P = CCM_NA * Lp;
-  if (Lp % 32 || kcols % GM_TK || kcols <= 0 || lda < (long)passes * kcols || ldc < P || (lda % 8))
+  if (Lp % 64 || kcols % 64 || kcols <= 0 || lda < (long)passes * kcols || ldc < P || (lda % 8))
     return CCM_ERR_ARG;
-  const uint32_t tm = (P + GM_TM - 1) / GM_TM, tn = (P + GM_TN - 1) / GM_TN;
-  const uint32_t tiles = tm * tn, total = round8(tiles);
-  hipLaunchKernelGGL(k_gemm_onehot<1>, dim3(total), dim3(256), 0, s, d_dt, lda, P, kcols, passes,
-                     d_msapt, ldmt, Lp, P, d_gt, ldc, accumulate, tn, tiles, total);
-  CCM_LAUNCH_CHECK();
-  return CCM_OK;
+  if (variant < 0) variant = gemm_default_variant(P, P);
+  return launch_gemm<1>(variant, d_dt, lda, P, kcols, passes, d_msapt, ldmt, Lp, P, d_gt, ldc,
+                        accumulate, s);
 }
 
 }  // extern "C"

@@ -14,7 +14,8 @@ import numpy as np
 
 from . import _lib
 
-PRECISIONS = {"fp32": (0, 1), "bf16": (1, 1), "bf16x2": (1, 2)}
+# precision form -> (precision enum, gradient-operand passes, forward-operand passes)
+PRECISIONS = {"fp32": (0, 1, 1), "bf16": (1, 1, 1), "bf16x2": (1, 2, 2), "bf16f": (1, 1, 2)}
 FP32_STEPS = ["gather_fwd", "softmax_rows", "gather_grad", "pair_epilogue", "singles_sum"]
 BF16_STEPS = ["convert_wt", "gemm_fwd", "softmax_cols", "gemm_grad", "pair_epilogue", "singles_sum"]
 
@@ -63,7 +64,8 @@ class PLL:
     """One MSA on one GPU: weights, f+grad evaluation and the device CG."""
 
     def __init__(self, msa, theta=0.8, lambda_single=0.01, lambda_pair=None,
-                 lambda_pair_factor=0.2, precision="fp32", device=0, verbose=False, chunk_rows=0):
+                 lambda_pair_factor=0.2, precision="fp32", device=0, verbose=False, chunk_rows=0,
+                 gemm_variant=-1):
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {list(PRECISIONS)}")
         self.msa = np.ascontiguousarray(msa, dtype=np.uint8)
@@ -91,9 +93,10 @@ class PLL:
         cfg = _lib.HipConfig()
         _lib.lib().ccm_hip_default_config(C.byref(cfg))
         cfg.device = int(device)
-        cfg.precision, cfg.bf16_passes = PRECISIONS[precision]
+        cfg.precision, cfg.bf16_passes, cfg.bf16_fwd_passes = PRECISIONS[precision]
         cfg.verbose = 1 if verbose else 0
         cfg.chunk_rows = int(chunk_rows)
+        cfg.gemm_variant = int(gemm_variant)
         rc = _lib.lib().init_hip_ex(self._uptr, C.byref(cfg))
         if rc != 0:
             raise RuntimeError("init_hip_ex failed")

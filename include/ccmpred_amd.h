@@ -65,9 +65,11 @@ enum { CCM_PRECISION_FP32 = 0, CCM_PRECISION_BF16 = 1 };
 typedef struct ccm_hip_config {
 	int device;      /* HIP device ordinal */
 	int precision;   /* CCM_PRECISION_FP32 (plain) or CCM_PRECISION_BF16 (MFMA) */
-	int bf16_passes; /* 1: BF16 operands; 2: hi+lo BF16 split of the dense operand */
+	int bf16_passes; /* gradient contraction: 1 = BF16 operand, 2 = hi+lo BF16 split */
+	int bf16_fwd_passes; /* forward contraction, same meaning; 0 = as bf16_passes */
 	int verbose;     /* 1: print the reweighting summary like the reference */
 	int chunk_rows;  /* sequences per chunk (rounded up to 256); 0 = all at once */
+	int gemm_variant; /* MFMA tile variant (0..ccm_gemm_variants()-1); -1 = default */
 } ccm_hip_config;
 
 void ccm_hip_default_config(ccm_hip_config *cfg);

@@ -50,11 +50,13 @@ int ccm_launch_softmax_cols(const float *d_zt, long ldzt, const float *d_x1, con
 /* BF16 MFMA contractions */
 int ccm_launch_convert_wt(const float *d_x, int L, int Lp, uint16_t *d_wt, long lda, int passes,
                           ccm_stream_t s);
+int ccm_gemm_variants(void);
+int ccm_gemm_default_variant(int M, int Ncol);
 int ccm_launch_gemm_fwd(const uint16_t *d_wt, long lda, int Lp, int passes, const uint8_t *d_msap,
-                        int ncols, float *d_zt, long ldc, ccm_stream_t s);
+                        int ncols, float *d_zt, long ldc, int variant, ccm_stream_t s);
 int ccm_launch_gemm_grad(const uint16_t *d_dt, long lda, int Lp, int passes,
                          const uint8_t *d_msapt, long ldmt, int kcols, float *d_gt, long ldc,
-                         int accumulate, ccm_stream_t s);
+                         int accumulate, int variant, ccm_stream_t s);
 
 /* gradient assembly (cu:150-219, cu:390-461) and objective reduction */
 int ccm_pair_epilogue_blocks(int L);

@@ -57,12 +57,12 @@ def test_cg_reaches_minimum():
                    options=dict(maxiter=5000, gtol=1e-10, ftol=1e-15))
     fstar = res.fun
     r = ora.cg_pll_f32(msa, w, np.zeros(ora.nvar_padded(L), np.float32), lam1, lam2,
-                       max_iterations=400)
-    # at the FP32 resolution of f the line search may stop with -1
-    assert r["ret"] in (0, 2, -1) and r["iterations"] > 100
+                       max_iterations=400, evaluator="f64")
+    # once f stops decreasing at FP32 vector resolution the line search stops with -1
+    assert r["ret"] in (0, 2, -1) and r["iterations"] > 40
     f0, _ = ora.evaluate_f64(msa, w, np.zeros(ora.nvar_padded(L)), lam1, lam2)
-    assert r["fx"] == pytest.approx(fstar, rel=1e-4)
-    assert (f0 - r["fx"]) / (f0 - fstar) > 0.999
+    assert r["fx"] == pytest.approx(fstar, rel=2e-5)
+    assert (f0 - r["fx"]) / (f0 - fstar) > 0.9999
 
 
 def test_cg_epsilon_stop():
