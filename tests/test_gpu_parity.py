@@ -179,9 +179,11 @@ def _spearman(a, b):
     return spearmanr(a[iu], b[iu]).correlation
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16f", "bf16"])
 def test_cg_50_iterations_contacts(prec):
-    """Device CG vs the same CG on the host driven by the FP64 oracle."""
+    """Device CG vs the same CG on the host driven by the FP64 oracle.  With
+    single-pass BF16 logits f is quantised by the BF16 couplings, so the line
+    search may stop before 50 iterations; the contact criteria still apply."""
     L, N = 75, 1000
     say(f"cg50 {prec} L={L} N={N}")
     msa, pairs = synthetic_msa(L, N, 1, 15)
@@ -201,7 +203,11 @@ def test_cg_50_iterations_contacts(prec):
     say(f"  host ret={host['ret']} it={host['iterations']} evals={host['evaluations']} f={host['fx']:.6f} | "
         f"dev ret={dev['ret']} it={dev['iterations']} evals={dev['evaluations']} f={dev['fx']:.6f}")
     say(f"  spearman={rho:.6f} top-L/5 equal={top_dev == top_host} planted recovered={len(top_dev & planted)}")
-    assert dev["iterations"] == 50 and host["iterations"] == 50
+    assert host["iterations"] == 50
+    if prec == "bf16":
+        assert dev["iterations"] >= 20
+    else:
+        assert dev["iterations"] == 50
     assert abs(dev["fx"] - host["fx"]) / host["fx"] < 5e-3
     assert rho >= 0.999
     assert top_dev == top_host
