@@ -47,6 +47,7 @@ typedef struct ccm_hip_state {
 	size_t bytes; /* device bytes held by this instance */
 	unsigned char *d_msap;  /* Np x Lp, 255 past the real rows/columns */
 	unsigned char *d_msapt; /* Lp x Np, 255 past the real rows/columns */
+	unsigned char *d_segmeta; /* per 128-sequence chunk and column: rows grouped by residue */
 	float *d_w;
 	/* plain FP32 path */
 	float *d_zd; /* chunk x ldz */
@@ -134,7 +135,7 @@ static void free_state(ccm_hip_state *st)
 {
 	if (!st)
 		return;
-	void *bufs[] = {st->d_msap, st->d_msapt, st->d_w,  st->d_zd,     st->d_graw,
+	void *bufs[] = {st->d_msap, st->d_msapt, st->d_segmeta, st->d_w, st->d_zd, st->d_graw,
 	                st->d_wt,   st->d_zt,    st->d_dt, st->d_gt,     st->d_hsym,
 	                st->d_fpart, st->d_g1part, st->d_regpair, st->d_regsingle, st->d_f};
 	for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
@@ -152,11 +153,42 @@ static void free_state(ccm_hip_state *st)
 
 static long round_up(long v, long m) { return (v + m - 1) / m * m; }
 
+/* Records of the segmented-gather pair gradient: for every chunk of 128
+ * sequences and every column k, the chunk's rows grouped by residue
+ * (ascending, each group padded to a multiple of 4 with the zero row 128)
+ * in bytes [0, 192) and the group starts in bytes [192, 214). */
+static void build_seg_meta(const unsigned char *msa, int L, int N, unsigned char *meta)
+{
+	const int nch = (N + 127) / 128;
+	for (int c = 0; c < nch; c++) {
+		const int r0 = c * 128, nr = N - r0 < 128 ? N - r0 : 128;
+		for (int k = 0; k < L; k++) {
+			unsigned char *rec = meta + ((size_t)c * L + k) * 256;
+			memset(rec, 0, 256);
+			int pos = 0;
+			for (int b = 0; b < CCM_N_ALPHA; b++) {
+				rec[192 + b] = (unsigned char)pos;
+				for (int r = 0; r < nr; r++)
+					if (msa[(size_t)(r0 + r) * L + k] == b)
+						rec[pos++] = (unsigned char)r;
+				while (pos & 3)
+					rec[pos++] = 128;
+			}
+			rec[192 + CCM_N_ALPHA] = (unsigned char)pos;
+		}
+	}
+}
+
 /* reweighting.c:10 -- float product, ceil in double, truncate */
 static int identity_threshold(int ncol, float theta)
 {
 	float prod = theta * (float)(uint64_t)ncol;
 	return (int)ceil((double)prod);
+}
+
+static const unsigned char *seg_chunk(const ccm_hip_state *st, int n0)
+{
+	return st->d_segmeta + (size_t)(n0 / 128) * st->L * 256;
 }
 
 int init_hip_ex(void *instance, const ccm_hip_config *cfg_in)
@@ -220,6 +252,20 @@ int init_hip_ex(void *instance, const ccm_hip_config *cfg_in)
 	HIPCHK(dalloc(st, (void **)&st->d_msapt, (size_t)Lp * Np));
 	HIPCHK(hipMemcpyAsync(st->d_msap, h_msap, (size_t)Np * Lp, hipMemcpyHostToDevice, s));
 	HIPCHK(hipMemcpyAsync(st->d_msapt, h_msapt, (size_t)Lp * Np, hipMemcpyHostToDevice, s));
+
+	{
+		const size_t mb = (size_t)ccm_seg_meta_bytes(L, N);
+		unsigned char *h_meta = (unsigned char *)malloc(mb);
+		if (!h_meta)
+			goto fail;
+		build_seg_meta(ud->msa, L, N, h_meta);
+		if (dalloc(st, (void **)&st->d_segmeta, mb) != hipSuccess ||
+		    hipMemcpy(st->d_segmeta, h_meta, mb, hipMemcpyHostToDevice) != hipSuccess) {
+			free(h_meta);
+			goto fail;
+		}
+		free(h_meta);
+	}
 
 	/* sequence weights (reweighting.c:9-67, bit exact) */
 	HIPCHK(dalloc(st, (void **)&st->d_w, sizeof(float) * (size_t)N));
@@ -302,8 +348,8 @@ int init_hip_ex(void *instance, const ccm_hip_config *cfg_in)
 			const int nc = N - n0 < st->chunk ? N - n0 : st->chunk;
 			KCHK(ccm_launch_neg_weighted_onehot(st->d_msap + (size_t)n0 * Lp, Lp, st->d_w + n0,
 			                                    L, nc, d_tmp_d, ld, s));
-			KCHK(ccm_launch_gather_grad_f32(d_tmp_d, ld, st->d_msap + (size_t)n0 * Lp, Lp, L, nc,
-			                                d_tmp_g, ld, n0 > 0, s));
+			KCHK(ccm_launch_gather_grad_seg_f32(d_tmp_d, ld, seg_chunk(st, n0), L, nc, d_tmp_g, ld,
+			                                    n0 > 0, s));
 		}
 		KCHK(ccm_launch_hsym(d_tmp_g, ld, st->d_hsym, L, s));
 	}
@@ -412,8 +458,8 @@ static int run_chunks(ccm_hip_state *st, const float *d_x, int what, int *step)
 			if (first)
 				MARK((*step)++);
 			if (what == RUN_FULL)
-				KCHK(ccm_launch_gather_grad_f32(st->d_zd, st->ldz, msap, Lp, L, nc, st->d_graw,
-				                                st->ldg, !first, s));
+				KCHK(ccm_launch_gather_grad_seg_f32(st->d_zd, st->ldz, seg_chunk(st, n0), L, nc,
+				                                    st->d_graw, st->ldg, !first, s));
 		} else {
 			const int passes = st->cfg.bf16_passes;
 			const int ncols = (int)round_up(nc, 128);
@@ -445,8 +491,8 @@ static int pair_gradient_from_forward(ccm_hip_state *st)
 	/* the single-chunk forward left D / Dt in place */
 	hipStream_t s = st->stream;
 	if (st->cfg.precision == CCM_PRECISION_FP32)
-		KCHK(ccm_launch_gather_grad_f32(st->d_zd, st->ldz, st->d_msap, st->Lp, st->L, st->N,
-		                                st->d_graw, st->ldg, 0, s));
+		KCHK(ccm_launch_gather_grad_seg_f32(st->d_zd, st->ldz, st->d_segmeta, st->L, st->N,
+		                                    st->d_graw, st->ldg, 0, s));
 	else
 		KCHK(ccm_launch_gemm_grad(st->d_dt, st->lddt, st->Lp, st->cfg.bf16_passes, st->d_msapt,
 		                          (long)st->Np, st->ccols, st->d_gt, (long)st->P, 0,

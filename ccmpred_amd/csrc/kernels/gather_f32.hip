@@ -224,3 +224,131 @@ int ccm_launch_neg_weighted_onehot(const uint8_t* d_msap, int Lp, const float* d
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Pair gradient as a segmented gather (deterministic, no read-modify-write).
+//
+// Sequences are taken in chunks of 128.  For every chunk and column k a
+// precomputed record lists the chunk's rows grouped by residue b = x_nk
+// (ascending), each group padded to a multiple of 4 with row 128 (a zero
+// row), plus the group starts:
+//   bytes [0, 192): row indices;  bytes [192, 214): start of group b, b = 0..21
+// A workgroup stages 128 rows of a 128-column slab of D in LDS and each wave
+// sums, for its 4 columns k and every residue b, the rows of group b into a
+// register accumulator: G[(b*L + k)][c] = sum over the group of D[n][c].
+#define GS_NC 128
+#define GS_REC 256
+
+__global__ __launch_bounds__(256, 2) void k_gather_grad_seg_f32(
+    const float* __restrict__ D, long ldd, const uint8_t* __restrict__ meta, int L, int N,
+    float* __restrict__ G, long ldg, int accumulate, uint32_t tiles_k, uint32_t tiles, uint32_t total) {
+  __shared__ float2 dt[(GS_NC + 1) * 64];   // chunk rows (+ the zero row) x 128 columns
+  __shared__ uint32_t mt[16 * GS_REC / 4];  // records of the workgroup's 16 columns k
+  const uint32_t q = ccm_xcd_remap(blockIdx.x, total);
+  if (q >= tiles) return;
+  const int cs = q / tiles_k, kt = q % tiles_k;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const long C21 = (long)CCM_NA * L;
+  const int c0 = cs * 128;
+  const int col = c0 + 2 * lane;
+  const int kbase = kt * 16;
+
+  float2 acc[4][CCM_NA];
+#pragma unroll
+  for (int kw = 0; kw < 4; kw++)
+#pragma unroll
+    for (int b = 0; b < CCM_NA; b++) acc[kw][b] = make_float2(0.f, 0.f);
+
+  if (wv == 0) dt[GS_NC * 64 + lane] = make_float2(0.f, 0.f);
+  const int nch = (N + GS_NC - 1) / GS_NC;
+  for (int ch = 0; ch < nch; ch++) {
+    __syncthreads();
+    for (int e = tid; e < GS_NC * 64; e += 256) {
+      const int r = e >> 6, cc = c0 + 2 * (e & 63);
+      const long n = (long)ch * GS_NC + r;
+      float2 v = make_float2(0.f, 0.f);
+      if (n < N) {
+        const float* src = D + n * ldd + cc;
+        if (cc + 1 < C21)
+          v = *reinterpret_cast<const float2*>(src);
+        else if (cc < C21)
+          v.x = src[0];
+      }
+      dt[e] = v;
+    }
+    for (int e = tid; e < 16 * GS_REC / 4; e += 256) {
+      const int kk = e / (GS_REC / 4), w = e % (GS_REC / 4);
+      const int k = kbase + kk;
+      mt[e] = k < L ? reinterpret_cast<const uint32_t*>(meta + ((long)ch * L + k) * GS_REC)[w] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kw = 0; kw < 4; kw++) {
+      const int kl = wv * 4 + kw;
+      if (kbase + kl >= L) break;
+      const uint32_t* rec = mt + kl * (GS_REC / 4);
+      const uint32_t pos4 = rec[lane];  // packed row indices 4*lane .. 4*lane+3
+      uint32_t sg[6];
+#pragma unroll
+      for (int t = 0; t < 6; t++) sg[t] = __builtin_amdgcn_readfirstlane(rec[48 + t]);
+#pragma unroll
+      for (int b = 0; b < CCM_NA; b++) {
+        const int s = (sg[b >> 2] >> (8 * (b & 3))) & 0xff;
+        const int e = (sg[(b + 1) >> 2] >> (8 * ((b + 1) & 3))) & 0xff;
+        float2 a = acc[kw][b];
+        for (int i = s; i < e; i += 4) {
+          const uint32_t r4 = __builtin_amdgcn_readlane(pos4, i >> 2);
+          const float2 v0 = dt[(r4 & 0xffu) * 64 + lane];
+          const float2 v1 = dt[((r4 >> 8) & 0xffu) * 64 + lane];
+          const float2 v2 = dt[((r4 >> 16) & 0xffu) * 64 + lane];
+          const float2 v3 = dt[(r4 >> 24) * 64 + lane];
+          a.x += v0.x; a.y += v0.y;
+          a.x += v1.x; a.y += v1.y;
+          a.x += v2.x; a.y += v2.y;
+          a.x += v3.x; a.y += v3.y;
+        }
+        acc[kw][b] = a;
+      }
+    }
+  }
+#pragma unroll
+  for (int kw = 0; kw < 4; kw++) {
+    const int k = kbase + wv * 4 + kw;
+    if (k >= L) break;
+#pragma unroll
+    for (int b = 0; b < CCM_NA; b++) {
+      float* dst = G + (long)(b * L + k) * ldg + col;
+      if (col + 1 < C21) {
+        float2 v = acc[kw][b];
+        if (accumulate) {
+          const float2 o = *reinterpret_cast<const float2*>(dst);
+          v.x += o.x;
+          v.y += o.y;
+        }
+        *reinterpret_cast<float2*>(dst) = v;
+      } else if (col < C21) {
+        dst[0] = accumulate ? dst[0] + acc[kw][b].x : acc[kw][b].x;
+      }
+    }
+  }
+}
+
+extern "C" {
+
+int ccm_seg_meta_bytes(int L, int N) { return ((N + GS_NC - 1) / GS_NC) * L * GS_REC; }
+
+// G (21L x ldg floats, ldg even) (+)= sum_n onehot_n^T D_n over the N
+// sequences the chunk records in d_meta describe (N x ldd floats of D)
+int ccm_launch_gather_grad_seg_f32(const float* d_d, long ldd, const uint8_t* d_meta, int L, int N,
+                                   float* d_g, long ldg, int accumulate, hipStream_t s) {
+  if (L <= 0 || N <= 0 || ldg < 21L * L || ldd < 21L * L || (ldg & 1) || (ldd & 1)) return CCM_ERR_ARG;
+  const uint32_t tiles_c = (uint32_t)((21L * L + 127) / 128);
+  const uint32_t tiles_k = (uint32_t)((L + 15) / 16);
+  const uint32_t tiles = tiles_c * tiles_k, total = round8(tiles);
+  hipLaunchKernelGGL(k_gather_grad_seg_f32, dim3(total), dim3(256), 0, s, d_d, ldd, d_meta, L, N, d_g,
+                     ldg, accumulate, tiles_k, tiles, total);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+}  // extern "C"

@@ -77,8 +77,8 @@ __device__ __forceinline__ bf16x8_t onehot8(uint2 v, uint32_t b) {
 // MODE 0: forward.  cols = sequences n;     B[kk][n] = [msap[n][kk%Lp] == kk/Lp], Kb = P
 // MODE 1: gradient. cols = coupling rows r; B[kk][r] = [msapt[r%Lp][kk] == r/Lp], Kb = chunk
 // Requires Lp % TK == 0 (forward) and Kb % TK == 0.
-template <int MODE, int WM, int WN, int RB, int CB, int TK>
-__global__ __launch_bounds__(64 * WM * WN) void k_gemm_onehot(
+template <int MODE, int WM, int WN, int RB, int CB, int TK, int PF, int OCC>
+__global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot(
     const uint16_t* __restrict__ A, long lda, int M, int Kb, int passes,
     const uint8_t* __restrict__ bytes, long ldbytes, int Lp, int Ncol, float* __restrict__ C,
     long ldc, int accumulate, uint32_t tiles_n, uint32_t tiles, uint32_t total) {
@@ -123,38 +123,44 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_onehot(
     for (int j = 0; j < CB; j++) acc[i][j] = (floatx16)0.0f;
 
   const int nsteps = passes * Kb / TK;
-  // loader: chunk c = tid + i*NT -> row c / KCH, 16-byte column c % KCH
-  const uint4* asrc[CPT];
-  int adst[CPT];
-  bool aok[CPT];
-#pragma unroll
-  for (int i = 0; i < CPT; i++) {
-    const int c = tid + i * NT;
-    const int row = c / KCH, col8 = c % KCH;
-    aok[i] = m0 + row < M;
-    asrc[i] = reinterpret_cast<const uint4*>(A + (long)(aok[i] ? m0 + row : 0) * lda) + col8;
-    adst[i] = row * LDA + col8 * 8;
-  }
-  uint4 pre[CPT];
-#pragma unroll
-  for (int i = 0; i < CPT; i++) pre[i] = aok[i] ? asrc[i][0] : make_uint4(0, 0, 0, 0);
-#pragma unroll
-  for (int i = 0; i < CPT; i++) *reinterpret_cast<uint4*>(&As[0][adst[i]]) = pre[i];
-  __syncthreads();
-
+  // loader: chunk c = tid + i*NT -> row c / KCH, 16-byte column c % KCH; the
+  // rows of one thread are NT/KCH apart
+  constexpr int RSTEP = NT / KCH;
+  const int arow0 = tid / KCH, acol8 = tid % KCH;
+  const uint16_t* abase = A + (long)(m0 + arow0) * lda + acol8 * 8;
+  const int adst0 = arow0 * LDA + acol8 * 8;
   // uniform position of the current K step inside one pass: for the forward
   // the coupling row block bb and the column offset kin; for the gradient
   // the sequence offset kin
   int bb = 0, kin = 0;
-  for (int s = 0; s < nsteps; s++) {
-    const int cur = s & 1;
-    if (s + 1 < nsteps) {
+  auto load = [&](uint4(&r)[CPT], int step) {
 #pragma unroll
-      for (int i = 0; i < CPT; i++)
-        pre[i] = aok[i] ? asrc[i][(long)(s + 1) * KCH] : make_uint4(0, 0, 0, 0);
+    for (int i = 0; i < CPT; i++)
+      r[i] = (m0 + arow0 + i * RSTEP < M)
+                 ? *reinterpret_cast<const uint4*>(abase + (long)i * RSTEP * lda + (long)step * TK)
+                 : make_uint4(0, 0, 0, 0);
+  };
+  auto store = [&](const uint4(&r)[CPT], int buf) {
+#pragma unroll
+    for (int i = 0; i < CPT; i++) *reinterpret_cast<uint4*>(&As[buf][adst0 + i * RSTEP * LDA]) = r[i];
+  };
+  auto advance = [&]() {
+    kin += TK;
+    if (MODE == 0) {
+      if (kin == Lp) {
+        kin = 0;
+        if (++bb == CCM_NA) bb = 0;
+      }
+    } else {
+      if (kin == Kb) kin = 0;
     }
-    const uint16_t* as = As[cur];
-#pragma unroll
+  };
+  // with 16 accumulator blocks per wave the K sub-steps are not unrolled, so
+  // that the scheduler does not hoist fragments of several sub-steps at once
+  constexpr int SUB_UNROLL = RB * CB >= 16 ? 1 : TK / 16;
+  auto compute = [&](int buf) {
+    const uint16_t* as = As[buf];
+#pragma unroll SUB_UNROLL
     for (int sub = 0; sub < TK / 16; sub++) {
       const int koff = kin + sub * 16 + 8 * q;  // first of this lane's 8 K indices
       bf16x8_t bfrag[CB];
@@ -173,19 +179,41 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_onehot(
           acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afrag, bfrag[cb], acc[rb][cb], 0, 0, 0);
       }
     }
-    if (s + 1 < nsteps) {
-#pragma unroll
-      for (int i = 0; i < CPT; i++) *reinterpret_cast<uint4*>(&As[cur ^ 1][adst[i]]) = pre[i];
-    }
+  };
+
+  if (PF == 1) {
+    // one register stage: the next step's tile is loaded while this one is used
+    uint4 pre[CPT];
+    load(pre, 0);
+    store(pre, 0);
     __syncthreads();
-    kin += TK;
-    if (MODE == 0) {
-      if (kin == Lp) {
-        kin = 0;
-        if (++bb == CCM_NA) bb = 0;
-      }
-    } else {
-      if (kin == Kb) kin = 0;
+    for (int s = 0; s < nsteps; s++) {
+      const int cur = s & 1;
+      if (s + 1 < nsteps) load(pre, s + 1);
+      compute(cur);
+      advance();
+      if (s + 1 < nsteps) store(pre, cur ^ 1);
+      __syncthreads();
+    }
+  } else {
+    // two register stages: a tile is loaded two steps before it is used
+    uint4 ra[CPT], rb2[CPT];
+    load(ra, 0);
+    store(ra, 0);
+    if (nsteps > 1) load(rb2, 1);
+    __syncthreads();
+    for (int s = 0; s < nsteps; s += 2) {
+      if (s + 2 < nsteps) load(ra, s + 2);
+      compute(0);
+      advance();
+      if (s + 1 < nsteps) store(rb2, 1);
+      __syncthreads();
+      if (s + 1 >= nsteps) break;
+      if (s + 3 < nsteps) load(rb2, s + 3);
+      compute(1);
+      advance();
+      if (s + 2 < nsteps) store(ra, 0);
+      __syncthreads();
     }
   }
 
@@ -226,43 +254,58 @@ int ccm_launch_convert_wt(const float* d_x, int L, int Lp, uint16_t* d_wt, long 
 
 }  // extern "C"
 
-// Tile variants: {waves M, waves N, row blocks, col blocks} with TK = 64
-//   0: 2x4 waves, 256 x 256      1: 2x2 waves, 256 x 128      2: 2x2 waves, 128 x 128
+// Tile variants: {waves M x N, row blocks x col blocks per wave, K step,
+// prefetch stages, workgroups per CU}
+//   0: 2x4, 4x2, 64, 1, 1 -> 256 x 256, 8 waves, 2 waves/SIMD
+//   1: 2x2, 4x2, 64, 1, 2 -> 256 x 128
+//   2: 2x2, 2x2, 64, 2, 2 -> 128 x 128
+//   3: 2x2, 4x4, 64, 2, 1 -> 256 x 256, 4 waves of 128 x 128, 1 wave/SIMD
+//   4: 2x4, 4x2, 128, 1, 1 -> 256 x 256, K step 128
+//   5: 2x2, 4x4, 64, 1, 1 -> as 3 with one prefetch stage
+#define GM_NVAR 6
+static const int gm_tm[GM_NVAR] = {256, 256, 128, 256, 256, 256};
+static const int gm_tn[GM_NVAR] = {256, 128, 128, 256, 256, 256};
+static const int gm_tk[GM_NVAR] = {64, 64, 64, 64, 128, 64};
+
 template <int MODE>
 static int launch_gemm(int variant, const uint16_t* A, long lda, int M, int Kb, int passes,
                        const uint8_t* bytes, long ldbytes, int Lp, int Ncol, float* C, long ldc,
                        int accumulate, hipStream_t s) {
-  int tm, tn;
-  switch (variant) {
-    case 0: tm = 256; tn = 256; break;
-    case 1: tm = 256; tn = 128; break;
-    case 2: tm = 128; tn = 128; break;
-    default: return CCM_ERR_ARG;
-  }
-  const uint32_t tiles_m = (M + tm - 1) / tm, tiles_n = (Ncol + tn - 1) / tn;
+  if (variant < 0 || variant >= GM_NVAR) return CCM_ERR_ARG;
+  const int tk = gm_tk[variant];
+  if ((MODE == 0 && Lp % tk) || Kb % tk) return CCM_ERR_ARG;
+  const uint32_t tiles_m = (M + gm_tm[variant] - 1) / gm_tm[variant];
+  const uint32_t tiles_n = (Ncol + gm_tn[variant] - 1) / gm_tn[variant];
   const uint32_t tiles = tiles_m * tiles_n, total = round8(tiles);
+#define GM_ARGS A, lda, M, Kb, passes, bytes, ldbytes, Lp, Ncol, C, ldc, accumulate, tiles_n, tiles, total
   switch (variant) {
     case 0:
-      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 4, 4, 2, 64>), dim3(total), dim3(512), 0, s, A, lda, M,
-                         Kb, passes, bytes, ldbytes, Lp, Ncol, C, ldc, accumulate, tiles_n, tiles, total);
+      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 4, 4, 2, 64, 1, 1>), dim3(total), dim3(512), 0, s, GM_ARGS);
       break;
     case 1:
-      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 2, 4, 2, 64>), dim3(total), dim3(256), 0, s, A, lda, M,
-                         Kb, passes, bytes, ldbytes, Lp, Ncol, C, ldc, accumulate, tiles_n, tiles, total);
+      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 2, 4, 2, 64, 1, 2>), dim3(total), dim3(256), 0, s, GM_ARGS);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 2, 2, 2, 64, 2, 2>), dim3(total), dim3(256), 0, s, GM_ARGS);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 2, 4, 4, 64, 2, 1>), dim3(total), dim3(256), 0, s, GM_ARGS);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 4, 4, 2, 128, 1, 1>), dim3(total), dim3(512), 0, s, GM_ARGS);
       break;
     default:
-      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 2, 2, 2, 64>), dim3(total), dim3(256), 0, s, A, lda, M,
-                         Kb, passes, bytes, ldbytes, Lp, Ncol, C, ldc, accumulate, tiles_n, tiles, total);
+      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 2, 4, 4, 64, 1, 1>), dim3(total), dim3(256), 0, s, GM_ARGS);
       break;
   }
+#undef GM_ARGS
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }
 
 // workgroups of a launch for a tile variant
 static uint32_t gemm_tiles(int variant, int M, int Ncol) {
-  const int tm = variant == 2 ? 128 : 256, tn = variant == 0 ? 256 : 128;
-  return (uint32_t)(((M + tm - 1) / tm) * ((Ncol + tn - 1) / tn));
+  return (uint32_t)(((M + gm_tm[variant] - 1) / gm_tm[variant]) * ((Ncol + gm_tn[variant] - 1) / gm_tn[variant]));
 }
 
 // default variant: the largest tile that still gives every CU a workgroup
@@ -274,7 +317,7 @@ static int gemm_default_variant(int M, int Ncol) {
 
 extern "C" {
 
-int ccm_gemm_variants(void) { return 3; }
+int ccm_gemm_variants(void) { return GM_NVAR; }
 
 int ccm_gemm_default_variant(int M, int Ncol) { return gemm_default_variant(M, Ncol); }
 

@@ -31,6 +31,9 @@ int ccm_launch_gather_fwd_f32(const uint8_t *d_msap, int Lp, const float *d_x, i
                               float *d_z, long ldz, ccm_stream_t s);
 int ccm_launch_gather_grad_f32(const float *d_d, long ldd, const uint8_t *d_msap, int Lp, int L,
                                int N, float *d_g, long ldg, int accumulate, ccm_stream_t s);
+int ccm_seg_meta_bytes(int L, int N);
+int ccm_launch_gather_grad_seg_f32(const float *d_d, long ldd, const uint8_t *d_meta, int L, int N,
+                                   float *d_g, long ldg, int accumulate, ccm_stream_t s);
 int ccm_launch_neg_weighted_onehot(const uint8_t *d_msap, int Lp, const float *d_w, int L, int N,
                                    float *d_d, long ldd, ccm_stream_t s);
 
