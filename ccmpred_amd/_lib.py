@@ -61,7 +61,7 @@ EXPORTED = [
     "ccm_reweight_words", "ccm_launch_identity_counts", "ccm_launch_weights_from_counts",
     "ccm_launch_fill_f32", "ccm_launch_gather_fwd_f32", "ccm_launch_gather_grad_f32",
     "ccm_launch_neg_weighted_onehot", "ccm_seg_meta_bytes", "ccm_launch_gather_grad_seg_f32", "ccm_softmax_rows_blocks", "ccm_softmax_rows_nblk",
-    "ccm_softmax_cols_blocks", "ccm_softmax_cols_nblk", "ccm_launch_softmax_rows",
+    "ccm_softmax_cols_blocks", "ccm_softmax_cols_nblk", "ccm_softmax_cols_granule", "ccm_launch_softmax_rows",
     "ccm_launch_softmax_cols", "ccm_launch_convert_wt", "ccm_launch_gemm_fwd",
     "ccm_launch_gemm_grad", "ccm_gemm_variants", "ccm_gemm_default_variant", "ccm_pair_epilogue_blocks", "ccm_singles_blocks",
     "ccm_launch_pair_epilogue", "ccm_launch_hsym", "ccm_launch_singles_finalize",

@@ -224,9 +224,11 @@ int init_hip_ex(void *instance, const ccm_hip_config *cfg_in)
 	st->nvp = ccm_nvar_padded(L);
 	st->lam1 = ud->lambda_single;
 	st->lam2 = ud->lambda_pair;
-	st->chunk = cfg.chunk_rows > 0 ? (int)round_up(cfg.chunk_rows, 256) : (int)round_up(N, 256);
+	/* chunks are whole blocks of the softmax kernel of the precision form */
+	const int gran = cfg.precision == CCM_PRECISION_BF16 ? ccm_softmax_cols_granule() : 256;
+	st->chunk = cfg.chunk_rows > 0 ? (int)round_up(cfg.chunk_rows, gran) : (int)round_up(N, gran);
 	if (st->chunk >= N)
-		st->chunk = (int)round_up(N, 256);
+		st->chunk = (int)round_up(N, gran);
 	st->ccols = (int)round_up(st->chunk < N ? st->chunk : N, 128);
 	const int Lp = st->Lp, Np = st->Np, P = st->P;
 	const int chunk_rows = st->chunk < N ? st->chunk : N;

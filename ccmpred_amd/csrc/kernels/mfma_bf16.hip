@@ -308,10 +308,12 @@ static uint32_t gemm_tiles(int variant, int M, int Ncol) {
   return (uint32_t)(((M + gm_tm[variant] - 1) / gm_tm[variant]) * ((Ncol + gm_tn[variant] - 1) / gm_tn[variant]));
 }
 
-// default variant: the largest tile that still gives every CU a workgroup
-static int gemm_default_variant(int M, int Ncol) {
-  if (gemm_tiles(0, M, Ncol) >= 256) return 0;
-  if (gemm_tiles(1, M, Ncol) >= 256) return 1;
+// default variant: the 256 x 256 tile with the longest K step the shape
+// allows while it still gives most CUs a workgroup, then smaller tiles
+static int gemm_default_variant(int M, int Ncol, int kstep_ok128) {
+  if (kstep_ok128 && gemm_tiles(4, M, Ncol) >= 192) return 4;
+  if (gemm_tiles(0, M, Ncol) >= 192) return 0;
+  if (gemm_tiles(1, M, Ncol) >= 192) return 1;
   return 2;
 }
 
@@ -319,7 +321,7 @@ extern "C" {
 
 int ccm_gemm_variants(void) { return GM_NVAR; }
 
-int ccm_gemm_default_variant(int M, int Ncol) { return gemm_default_variant(M, Ncol); }
+int ccm_gemm_default_variant(int M, int Ncol) { return gemm_default_variant(M, Ncol, 1); }
 
 // Zt[c][n] (P x ldc floats, ldc >= ncols) = sum_r Wt[c][r] [x_nk = b] for the
 // ncols (multiple of 128) sequences whose padded MSA rows start at d_msap;
@@ -329,7 +331,7 @@ int ccm_launch_gemm_fwd(const uint16_t* d_wt, long lda, int Lp, int passes, cons
   const int P = CCM_NA * Lp;
   if (Lp % 64 || ncols % 128 || ncols <= 0 || lda < (long)passes * P || ldc < ncols || (lda % 8))
     return CCM_ERR_ARG;
-  if (variant < 0) variant = gemm_default_variant(P, ncols);
+  if (variant < 0) variant = gemm_default_variant(P, ncols, Lp % 128 == 0);
   return launch_gemm<0>(variant, d_wt, lda, P, P, passes, d_msap, (long)Lp, Lp, ncols, d_zt, ldc, 0, s);
 }
 
@@ -342,7 +344,7 @@ int ccm_launch_gemm_grad(const uint16_t* d_dt, long lda, int Lp, int passes, con
   const int P = CCM_NA * Lp;
   if (Lp % 64 || kcols % 64 || kcols <= 0 || lda < (long)passes * kcols || ldc < P || (lda % 8))
     return CCM_ERR_ARG;
-  if (variant < 0) variant = gemm_default_variant(P, P);
+  if (variant < 0) variant = gemm_default_variant(P, P, kcols % 128 == 0);
   return launch_gemm<1>(variant, d_dt, lda, P, kcols, passes, d_msapt, ldmt, Lp, P, d_gt, ldc,
                         accumulate, s);
 }

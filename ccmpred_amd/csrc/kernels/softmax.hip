@@ -87,8 +87,11 @@ __global__ __launch_bounds__(256) void k_softmax_rows(float* __restrict__ ZD, lo
   }
 }
 
-#define SC_JT 8  // columns j per block in the transposed variant
+#define SC_R 8  // sequences per thread in the transposed variant (2048 per block)
 
+// One block: column j (blockIdx.y) for 2048 consecutive sequences; each
+// thread takes 8 sequences 256 apart and accumulates its g1 terms before a
+// single wave reduction per residue.
 __global__ __launch_bounds__(256) void k_softmax_cols(const float* __restrict__ Zt, long ldzt,
                                                       const float* __restrict__ x1,
                                                       const uint8_t* __restrict__ msapt, long ldmt,
@@ -97,23 +100,38 @@ __global__ __launch_bounds__(256) void k_softmax_cols(const float* __restrict__ 
                                                       long lddt, long lo_off, int passes, int mode,
                                                       int blk0, double* __restrict__ fpart,
                                                       double* __restrict__ g1part) {
-  __shared__ double g1red[4][SC_JT][20];
+  __shared__ float g1red[4][20];
   __shared__ double fred[4];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int n = blockIdx.x * 256 + threadIdx.x;
-  const bool live = n < Nc;
-  const bool pad = !live && n < ncols;  // padded columns of Dt are kept at zero
-  const float wn = live ? w[n] : 0.f;
+  const int j = blockIdx.y;
+  float v1[20];
+#pragma unroll
+  for (int a = 0; a < 20; a++) v1[a] = x1[(long)a * L + j];
+  float g1[20];
+#pragma unroll
+  for (int a = 0; a < 20; a++) g1[a] = 0.f;
   double fl = 0.0;
-  for (int jj = 0; jj < SC_JT; jj++) {
-    const int j = blockIdx.y * SC_JT + jj;
-    if (j >= L) break;
+  for (int r = 0; r < SC_R; r++) {
+    const int n = blockIdx.x * (256 * SC_R) + r * 256 + threadIdx.x;
+    if (n >= ncols) break;
+    uint16_t* dcol = Dt + (long)j * lddt + n;
+    if (n >= Nc) {
+      // padded columns of Dt stay zero
+      if (mode) {
+#pragma unroll
+        for (int a = 0; a < CCM_NA; a++) {
+          dcol[(long)a * Lp * lddt] = 0;
+          if (passes > 1) dcol[(long)a * Lp * lddt + lo_off] = 0;
+        }
+      }
+      continue;
+    }
     float z[CCM_NA];
     float m = -INFINITY;
 #pragma unroll
     for (int a = 0; a < CCM_NA; a++) {
-      float v = live ? Zt[(long)(a * Lp + j) * ldzt + n] : 0.f;
-      if (a < 20) v += x1[(long)a * L + j];
+      float v = Zt[(long)(a * Lp + j) * ldzt + n];
+      if (a < 20) v += v1[a];
       z[a] = v;
       m = fmaxf(m, v);
     }
@@ -121,47 +139,41 @@ __global__ __launch_bounds__(256) void k_softmax_cols(const float* __restrict__ 
 #pragma unroll
     for (int a = 0; a < CCM_NA; a++) s += expf(z[a] - m);
     const float lse = m + logf(s);
-    const int x = live ? msapt[(long)j * ldmt + n] : 255;
+    const int x = msapt[(long)j * ldmt + n];
+    const float wn = w[n];
     float zx = 0.f;
 #pragma unroll
     for (int a = 0; a < CCM_NA; a++) zx = (a == x) ? z[a] : zx;
-    if (live) fl += (double)(wn * (lse - zx));
+    fl += (double)(wn * (lse - zx));
     if (mode) {
 #pragma unroll
       for (int a = 0; a < CCM_NA; a++) {
-        const float p = expf(z[a] - lse);
-        const float qv = wn * p;
-        uint16_t* dst = Dt + (long)(a * Lp + j) * lddt + n;
-        if (live) {
-          const uint16_t hi = ccm_f2bf(qv);
-          dst[0] = hi;
-          if (passes > 1) dst[lo_off] = ccm_f2bf(qv - ccm_bf2f(hi));
-        } else if (pad) {
-          dst[0] = 0;
-          if (passes > 1) dst[lo_off] = 0;
-        }
-        if (a < 20) {
-          const float d = live ? (qv - ((a == x) ? wn : 0.f)) : 0.f;
-          const double t = ccm_wave_sum_f64((double)d);
-          if (lane == 0) g1red[wv][jj][a] = t;
-        }
+        const float qv = wn * expf(z[a] - lse);
+        const uint16_t hi = ccm_f2bf(qv);
+        uint16_t* dst = dcol + (long)a * Lp * lddt;
+        dst[0] = hi;
+        if (passes > 1) dst[lo_off] = ccm_f2bf(qv - ccm_bf2f(hi));
+        if (a < 20) g1[a] += qv - ((a == x) ? wn : 0.f);
       }
+    }
+  }
+  if (mode) {
+#pragma unroll
+    for (int a = 0; a < 20; a++) {
+      const float t = ccm_wave_sum_f32(g1[a]);
+      if (lane == 0) g1red[wv][a] = t;
     }
   }
   fl = ccm_wave_sum_f64(fl);
   if (lane == 0) fred[wv] = fl;
   __syncthreads();
-  const int t = threadIdx.x;
   const long gb = (long)blk0 + blockIdx.x;
-  if (mode && t < SC_JT * 20) {
-    const int jj = t / 20, a = t % 20;
-    const int j = blockIdx.y * SC_JT + jj;
-    if (j < L) {
-      const double v = g1red[0][jj][a] + g1red[1][jj][a] + g1red[2][jj][a] + g1red[3][jj][a];
-      g1part[(gb * 20 + a) * L + j] = v;
-    }
+  if (mode && threadIdx.x < 20) {
+    const int a = threadIdx.x;
+    g1part[(gb * 20 + a) * L + j] = (double)g1red[0][a] + (double)g1red[1][a] + (double)g1red[2][a] +
+                                    (double)g1red[3][a];
   }
-  if (t == 0) fpart[gb * gridDim.y + blockIdx.y] = fred[0] + fred[1] + fred[2] + fred[3];
+  if (threadIdx.x == 0) fpart[gb * L + j] = fred[0] + fred[1] + fred[2] + fred[3];
 }
 
 extern "C" {
@@ -169,8 +181,9 @@ extern "C" {
 // partial-array geometry over the whole alignment (chunks are multiples of 256)
 int ccm_softmax_rows_blocks(int L, int N) { return ((L + 63) / 64) * ((N + 63) / 64); }
 int ccm_softmax_rows_nblk(int N) { return (N + 63) / 64; }
-int ccm_softmax_cols_blocks(int L, int N) { return ((N + 255) / 256) * ((L + SC_JT - 1) / SC_JT); }
-int ccm_softmax_cols_nblk(int N) { return (N + 255) / 256; }
+int ccm_softmax_cols_blocks(int L, int N) { return ((N + 256 * SC_R - 1) / (256 * SC_R)) * L; }
+int ccm_softmax_cols_nblk(int N) { return (N + 256 * SC_R - 1) / (256 * SC_R); }
+int ccm_softmax_cols_granule(void) { return 256 * SC_R; }
 
 // rows layout on sequences [n0, n0+Nc) of the alignment; pointers are
 // already offset to the chunk, n0 must be a multiple of 64
@@ -185,17 +198,18 @@ int ccm_launch_softmax_rows(float* d_zd, long ldz, const uint8_t* d_msap, int Lp
   return CCM_OK;
 }
 
-// cols layout on a chunk of Nc sequences starting at n0 (multiple of 256);
-// ncols: padded chunk width of Zt/Dt; ldmt: row stride of the transposed MSA
+// cols layout on a chunk of Nc sequences starting at n0 (a multiple of
+// ccm_softmax_cols_granule()); ncols: padded chunk width of Zt/Dt; ldmt: row
+// stride of the transposed MSA
 int ccm_launch_softmax_cols(const float* d_zt, long ldzt, const float* d_x1, const uint8_t* d_msapt,
                             long ldmt, const float* d_w, int L, int Lp, int Nc, int ncols, int n0,
                             uint16_t* d_dt, long lddt, long lo_off, int passes, int mode,
                             double* d_fpart, double* d_g1part, hipStream_t s) {
-  if (L <= 0 || Nc <= 0 || ncols < Nc || (n0 % 256)) return CCM_ERR_ARG;
-  const int nb = (ncols + 255) / 256;
-  hipLaunchKernelGGL(k_softmax_cols, dim3(nb, (L + SC_JT - 1) / SC_JT), dim3(256), 0, s, d_zt, ldzt,
-                     d_x1, d_msapt, ldmt, d_w, L, Lp, Nc, ncols, d_dt, lddt, lo_off, passes, mode,
-                     n0 / 256, d_fpart, d_g1part);
+  if (L <= 0 || Nc <= 0 || ncols < Nc || (n0 % (256 * SC_R))) return CCM_ERR_ARG;
+  const int nb = (ncols + 256 * SC_R - 1) / (256 * SC_R);
+  hipLaunchKernelGGL(k_softmax_cols, dim3(nb, L), dim3(256), 0, s, d_zt, ldzt, d_x1, d_msapt, ldmt, d_w, L,
+                     Lp, Nc, ncols, d_dt, lddt, lo_off, passes, mode, n0 / (256 * SC_R), d_fpart,
+                     d_g1part);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }

@@ -42,6 +42,8 @@ int ccm_softmax_rows_blocks(int L, int N);
 int ccm_softmax_rows_nblk(int N);
 int ccm_softmax_cols_blocks(int L, int N);
 int ccm_softmax_cols_nblk(int N);
+int ccm_softmax_cols_granule(void);
+/* sequence chunks must start at multiples of ccm_softmax_cols_granule() */
 int ccm_launch_softmax_rows(float *d_zd, long ldz, const uint8_t *d_msap, int Lp, const float *d_w,
                             int L, int Nc, int n0, int mode, double *d_fpart, double *d_g1part,
                             ccm_stream_t s);

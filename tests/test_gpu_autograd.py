@@ -77,11 +77,12 @@ def test_autograd_against_fd_and_reference(prec, tol):
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
 def test_recompute_matches_stored(prec):
-    L, N = 120, 2000
+    L, N = 120, 5000
     say(f"recompute vs stored {prec} L={L} N={N}")
     msa, _ = synthetic_msa(L, N, 5, 24)
     x = torch.from_numpy(make_x(msa, 56)).cuda()
-    with P.PLL(msa, precision=prec) as stored, P.PLL(msa, precision=prec, chunk_rows=512) as chunked:
+    # chunks of 2048 sequences: the BF16 softmax works on whole 2048-sequence blocks
+    with P.PLL(msa, precision=prec) as stored, P.PLL(msa, precision=prec, chunk_rows=2048) as chunked:
         xs = x.clone().requires_grad_(True)
         fs = pll_objective(xs, stored, recompute=False)
         fs.backward()
@@ -108,7 +109,7 @@ def test_recompute_large_n_memory():
     msa, _ = synthetic_msa(L, N, 6, 60)
     x = torch.from_numpy(make_x(msa, 57)).cuda()
     torch.cuda.synchronize()
-    with P.PLL(msa, precision="bf16", chunk_rows=4096) as chunked:
+    with P.PLL(msa, precision="bf16", chunk_rows=8192) as chunked:
         xr = x.clone().requires_grad_(True)
         fr = pll_objective(xr, chunked, recompute=True)
         fr.backward()
