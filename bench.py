@@ -82,6 +82,86 @@ def time_cpu(msa, x_host, evals):
     return dict(evals_per_s=1.0 / med, seconds=ts, threads=threads)
 
 
+def batched_workload(nmsa, seed=999):
+    """nmsa synthetic MSAs, L uniform in [50, 600], N log-uniform in
+    [200, 20000], MSA i generated with seed 1000 + i and L // 4 planted pairs."""
+    from ccmpred_amd.msa import synthetic_msa
+    rng = np.random.Generator(np.random.PCG64(seed))
+    Ls = rng.integers(50, 601, nmsa)
+    Ns = np.round(np.exp(rng.uniform(np.log(200), np.log(20000), nmsa))).astype(int)
+    return [synthetic_msa(int(L), int(N), 1000 + i, max(1, int(L) // 4))[0]
+            for i, (L, N) in enumerate(zip(Ls, Ns))]
+
+
+def run_batched(args):
+    """Batched form: every MSA solved for `iterations` CG iterations (epsilon
+    0, no early stop) from init_bias, all MSAs together; value = MSAs/s over
+    the solve (set-up and autotuning reported beside it)."""
+    import torch
+    from ccmpred_amd.batch import Batch
+    from ccmpred_amd.pll import PLL, init_bias
+    msas = batched_workload(args.batch_size)
+    t0 = time.perf_counter()
+    b = Batch(msas, precision=args.batch_precision)
+    t1 = time.perf_counter()
+    print(f"batched: {len(msas)} MSAs set up in {t1 - t0:.1f} s", file=sys.stderr, flush=True)
+    tuned = b.autotune(reps=3)
+    t2 = time.perf_counter()
+    print(f"batched: autotuned in {t2 - t1:.1f} s", file=sys.stderr, flush=True)
+    for i, m in enumerate(msas):
+        b.set_x(i, init_bias(m))
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    rounds = b.solve(max_iterations=args.iterations, epsilon=0.0, rounds_per_graph=args.rounds_per_graph)
+    t4 = time.perf_counter()
+    print(f"batched: solved in {t4 - t3:.2f} s ({rounds} rounds)", file=sys.stderr, flush=True)
+    res = [b.result(i) for i in range(b.n)]
+    fwd, grad = b.variants()
+    dev_gb = b.device_bytes / 1e9
+    b.close()
+    flops = sum(r["evaluations"] * 4.0 * m.shape[0] * (21 * m.shape[1]) ** 2 for r, m in zip(res, msas))
+    line = {
+        "metric": f"batched CG: {len(msas)} MSAs x {args.iterations} iterations, MSAs/s",
+        "value": round(len(msas) / (t4 - t3), 4),
+        "unit": "MSAs/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": f"{len(msas)} synthetic MSAs, L uniform [50,600], N log-uniform "
+                               "[200,20000], seeds 1000+i, x0 = init_bias",
+                   "L": [int(m.shape[1]) for m in msas], "N": [int(m.shape[0]) for m in msas],
+                   "iterations": args.iterations, "rounds_per_graph": args.rounds_per_graph},
+        "solve_s": round(t4 - t3, 3), "setup_s": round(t1 - t0, 3), "autotune_s": round(t2 - t1, 3),
+        "tuned_gemm_ms": [round(v, 3) for v in tuned],
+        "precision": args.batch_precision,
+        "rounds": rounds, "evaluations": int(sum(r["evaluations"] for r in res)),
+        "iterations_done": [r["iterations"] for r in res],
+        "status": {str(k): int(sum(r["ret"] == k for r in res)) for k in sorted({r["ret"] for r in res})},
+        "effective_tflops": round(flops / (t4 - t3) / 1e12, 1),
+        "device_gb": round(dev_gb, 2),
+        "variants_fwd": fwd.tolist(), "variants_grad": grad.tolist(),
+    }
+    if args.sequential:
+        # the same solves one MSA after another through the single-alignment
+        # CG (set-up of each instance outside the timed calls)
+        total = 0.0
+        same = 0
+        for i, m in enumerate(msas):
+            with PLL(m, precision=args.batch_precision) as p:
+                xd = torch.from_numpy(init_bias(m)).cuda()
+                torch.cuda.synchronize()
+                ts = time.perf_counter()
+                r = p.conjugrad(xd, max_iterations=args.iterations, epsilon=0.0)
+                total += time.perf_counter() - ts
+                same += int(r["fx"] == res[i]["fx"] and r["iterations"] == res[i]["iterations"])
+        print(f"batched: sequential solves in {total:.2f} s", file=sys.stderr, flush=True)
+        line["sequential_s"] = round(total, 3)
+        line["sequential_msas_per_s"] = round(len(msas) / total, 4)
+        line["sequential_same_result"] = same
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,11 +169,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-evals", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched", action="store_true",
+                    help="time the batched form (many MSAs) instead of single evaluations")
+    ap.add_argument("--batch-size", type=int, default=64)
+    ap.add_argument("--iterations", type=int, default=50)
+    ap.add_argument("--rounds-per-graph", type=int, default=8)
+    ap.add_argument("--batch-precision", default="bf16f", choices=["bf16", "bf16f", "bf16x2"],
+                    help="BF16 form of the batched solves (bf16f: two-pass forward, whose "
+                         "objective keeps the line search reliable near the optimum)")
+    ap.add_argument("--sequential", action="store_true",
+                    help="with --batched: also time the same solves one MSA at a time")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the evaluate callback works on one device); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.batched:
+        run_batched(args)
+        return
 
     from ccmpred_amd.msa import synthetic_msa
     msa, _ = synthetic_msa(L_COLS, N_SEQS, SEED, N_PAIRS)

@@ -67,6 +67,16 @@ EXPORTED = [
     "ccm_launch_pair_epilogue", "ccm_launch_hsym", "ccm_launch_singles_finalize",
     "ccm_launch_sum3", "ccm_dots_partial_len", "ccm_launch_dots", "ccm_launch_axpy",
     "ccm_launch_direction", "ccm_launch_negate",
+    "ccm_batch_create", "ccm_batch_destroy", "ccm_batch_size", "ccm_batch_x", "ccm_batch_g",
+    "ccm_batch_write_x", "ccm_batch_read", "ccm_batch_stream", "ccm_batch_device_bytes",
+    "ccm_batch_set_variants", "ccm_batch_get_variants", "ccm_batch_autotune", "ccm_batch_evaluate",
+    "ccm_batch_solve", "ccm_batch_result",
+    "ccm_convert_wt_blocks", "ccm_gemm_tile_count", "ccm_gemm_variant_ok",
+    "ccm_gemm_default_variant_for", "ccm_dots_blocks", "ccm_cg_vec_blocks",
+    "ccm_launch_convert_wt_grouped", "ccm_launch_gemm_fwd_grouped", "ccm_launch_gemm_grad_grouped",
+    "ccm_launch_softmax_cols_grouped", "ccm_launch_pair_epilogue_grouped",
+    "ccm_launch_singles_finalize_grouped", "ccm_launch_sum3_grouped", "ccm_launch_dots_grouped",
+    "ccm_launch_cg_vec_grouped", "ccm_launch_cg_ctrl",
 ]
 
 
@@ -133,6 +143,44 @@ def _declare(L):
     L.ccm_normalize.argtypes = [vp, ip]
     L.ccm_read_raw.argtypes = [C.c_char_p, ip, vp]
     L.ccm_read_raw.restype = ip
+    # batched form
+    L.ccm_batch_create.argtypes = [vp, ip, C.POINTER(HipConfig), C.POINTER(C.c_void_p)]
+    L.ccm_batch_create.restype = ip
+    L.ccm_batch_destroy.argtypes = [vp]
+    L.ccm_batch_destroy.restype = None
+    L.ccm_batch_size.argtypes = [vp]
+    L.ccm_batch_size.restype = ip
+    L.ccm_batch_x.argtypes = [vp, ip]
+    L.ccm_batch_x.restype = vp
+    L.ccm_batch_g.argtypes = [vp, ip]
+    L.ccm_batch_g.restype = vp
+    L.ccm_batch_write_x.argtypes = [vp, ip, vp]
+    L.ccm_batch_write_x.restype = ip
+    L.ccm_batch_read.argtypes = [vp, ip, vp, vp]
+    L.ccm_batch_read.restype = ip
+    L.ccm_batch_stream.argtypes = [vp]
+    L.ccm_batch_stream.restype = vp
+    L.ccm_batch_device_bytes.argtypes = [vp]
+    L.ccm_batch_device_bytes.restype = C.c_size_t
+    L.ccm_batch_set_variants.argtypes = [vp, vp, vp]
+    L.ccm_batch_set_variants.restype = ip
+    L.ccm_batch_get_variants.argtypes = [vp, vp, vp]
+    L.ccm_batch_get_variants.restype = ip
+    L.ccm_batch_autotune.argtypes = [vp, ip, vp]
+    L.ccm_batch_autotune.restype = ip
+    L.ccm_batch_evaluate.argtypes = [vp, vp]
+    L.ccm_batch_evaluate.restype = ip
+    L.ccm_batch_solve.argtypes = [vp, C.POINTER(CGParam), ip, C.POINTER(C.c_int)]
+    L.ccm_batch_solve.restype = ip
+    L.ccm_batch_result.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int), C.POINTER(C.c_int), vp]
+    L.ccm_batch_result.restype = ip
+    L.ccm_gemm_tile_count.argtypes = [ip, ip, ip]
+    L.ccm_gemm_tile_count.restype = C.c_uint32
+    L.ccm_gemm_variant_ok.argtypes = [ip, ip, ip, ip]
+    L.ccm_gemm_variant_ok.restype = ip
+    L.ccm_gemm_default_variant_for.argtypes = [ip, ip, ip]
+    L.ccm_gemm_default_variant_for.restype = ip
 
 
 def evaluate_fnptr():

@@ -19,7 +19,7 @@
  *   failed steepest-descent search ends the run with
  *   CONJUGRADERR_MAXIMUMLINESEARCH, otherwise the direction is reset to -g
  *   beta = max(0, (<g,g> - <g,g_prev>) / <g_prev,g_prev>)   (Polak-Ribiere+)
- *   s = -g + beta s; if <g, s> >= 0 then s = -g
+ *   s = -g + beta s unless its slope -<g,g> + beta <g,s_old> >= 0, then s = -g
  *   next alpha = alpha * <g_prev, s_prev> / <g, s>
  *   stop with CONJUGRAD_SUCCESS once (f_{it-k} - f_it) / |f_it| < epsilon.
  * All scalars, dot products and tests are FP64; vectors are FP32.
@@ -199,11 +199,11 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 		it++;
 
 		double d3[3];
-		if (dots(&c, g, g, gprev, NULL, 2, d3)) {
+		if (dots(&c, g, sdir, g, gprev, 3, d3)) {
 			ret = CONJUGRADERR_DEVICE;
 			goto done;
 		}
-		const double ggn = d3[0], ggp = d3[1];
+		const double dgs = d3[0], ggn = d3[1], ggp = d3[2];
 		if (progress) {
 			double xx;
 			if (dots(&c, d_x, d_x, NULL, NULL, 1, &xx)) {
@@ -223,15 +223,8 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 		double beta = (ggn - ggp) / gg;
 		if (beta < 0)
 			beta = 0;
-		if (ccm_launch_direction(sdir, g, (float)beta, n, c.s)) {
-			ret = CONJUGRADERR_DEVICE;
-			goto done;
-		}
-		double dgn;
-		if (dots(&c, g, sdir, NULL, NULL, 1, &dgn)) {
-			ret = CONJUGRADERR_DEVICE;
-			goto done;
-		}
+		/* slope of the new direction -g + beta s, from the dots above */
+		double dgn = -ggn + beta * dgs;
 		if (dgn >= 0) {
 			if (ccm_launch_negate(sdir, g, n, c.s)) {
 				ret = CONJUGRADERR_DEVICE;
@@ -240,6 +233,10 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 			dgn = -ggn;
 			steepest = 1;
 		} else {
+			if (ccm_launch_direction(sdir, g, (float)beta, n, c.s)) {
+				ret = CONJUGRADERR_DEVICE;
+				goto done;
+			}
 			steepest = beta == 0.0;
 		}
 		alpha = alpha * dg0 / dgn;

@@ -34,52 +34,7 @@
 #include "ccmpred_amd.h"
 #include "ccmpred_amd_kernels.h"
 
-#define CCM_TIMING_STEPS 6
-
-typedef struct ccm_hip_state {
-	ccm_hip_config cfg;
-	hipStream_t stream;
-	int L, N, Lp, P, Np;
-	int chunk;  /* sequences per chunk, multiple of 256 (>= N when not chunked) */
-	int ccols;  /* padded width of one chunk (multiple of 128) */
-	long nsp, nvp;
-	float lam1, lam2;
-	size_t bytes; /* device bytes held by this instance */
-	unsigned char *d_msap;  /* Np x Lp, 255 past the real rows/columns */
-	unsigned char *d_msapt; /* Lp x Np, 255 past the real rows/columns */
-	unsigned char *d_segmeta; /* per 128-sequence chunk and column: rows grouped by residue */
-	float *d_w;
-	/* plain FP32 path */
-	float *d_zd; /* chunk x ldz */
-	long ldz;
-	float *d_graw;
-	long ldg;
-	/* BF16 path */
-	uint16_t *d_wt;
-	long ldwt;
-	float *d_zt;    /* P x ccols */
-	uint16_t *d_dt; /* P x passes*ccols */
-	long lddt;
-	float *d_gt;
-	float *d_hsym;
-	/* reductions */
-	double *d_fpart;
-	int n_fpart;
-	double *d_g1part;
-	int n_g1blk;
-	double *d_regpair;
-	int n_regpair;
-	double *d_regsingle;
-	int n_regsingle;
-	double *d_f;
-	double *h_f;
-	int forward_valid; /* ccm_hip_forward left probabilities for ccm_hip_backward */
-	/* optional per-step timing with HIP events on the instance stream */
-	int timing;
-	hipEvent_t ev[CCM_TIMING_STEPS + 1];
-	double step_ms[CCM_TIMING_STEPS];
-	int timed_calls;
-} ccm_hip_state;
+#include "state.h"
 
 #define HIPCHK(call)                                                                       \
 	do {                                                                               \

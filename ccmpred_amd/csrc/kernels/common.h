@@ -66,3 +66,20 @@ static inline long ccm_nsp(int L) {
   long ns = (long)L * (CCM_NA - 1);
   return ns + CCM_NAP - (ns % CCM_NAP);
 }
+
+// ---- grouped launches over a batch of alignments ----
+#include "ccmpred_amd_batch.h"
+
+// index i of the group entry whose block range [offs[i], offs[i+1]) holds
+// blk (offs ascending, offs[0] = 0, blk < offs[np]); uniform per block
+__device__ __forceinline__ int ccm_group_find(const uint32_t* offs, int np, uint32_t blk) {
+  int lo = 0, hi = np;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (offs[mid] <= blk)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}

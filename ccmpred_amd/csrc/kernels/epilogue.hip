@@ -12,14 +12,15 @@
 
 #define EP_T 64
 
-__global__ __launch_bounds__(256) void k_pair_epilogue(const float* __restrict__ M, long ldm, int Lm,
-                                                       const float* __restrict__ hsym,
-                                                       const float* __restrict__ x2,
-                                                       float* __restrict__ g2, int L, float lam2,
-                                                       uint32_t tiles_c, double* __restrict__ regpart) {
+__device__ __forceinline__ void pair_epilogue_block(const float* __restrict__ M, long ldm, int Lm,
+                                                    const float* __restrict__ hsym,
+                                                    const float* __restrict__ x2,
+                                                    float* __restrict__ g2, int L, float lam2,
+                                                    uint32_t tiles_c, double* __restrict__ regpart,
+                                                    uint32_t blk) {
   __shared__ float T[EP_T][EP_T + 1];
   __shared__ double red[4];
-  const int tr = blockIdx.x / tiles_c, tc = blockIdx.x % tiles_c;
+  const int tr = blk / tiles_c, tc = blk % tiles_c;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int C21 = CCM_NA * L, ld2 = CCM_NAP * L;
   const float invL = 1.0f / (float)L;
@@ -68,7 +69,27 @@ __global__ __launch_bounds__(256) void k_pair_epilogue(const float* __restrict__
   reg = ccm_wave_sum_f64(reg);
   if (lane == 0) red[wv] = reg;
   __syncthreads();
-  if (threadIdx.x == 0) regpart[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) regpart[blk] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void k_pair_epilogue(const float* __restrict__ M, long ldm, int Lm,
+                                                       const float* __restrict__ hsym,
+                                                       const float* __restrict__ x2,
+                                                       float* __restrict__ g2, int L, float lam2,
+                                                       uint32_t tiles_c, double* __restrict__ regpart) {
+  pair_epilogue_block(M, ldm, Lm, hsym, x2, g2, L, lam2, tiles_c, regpart, blockIdx.x);
+}
+
+// BF16 layout of every alignment of a group: pair_epilogue_blocks(L) blocks each
+__global__ __launch_bounds__(256) void k_pair_epilogue_grouped(const ccm_bprob* __restrict__ probs,
+                                                               const uint32_t* __restrict__ offs,
+                                                               const int32_t* __restrict__ pidx, int np) {
+  const int gi = ccm_group_find(offs, np, blockIdx.x);
+  const ccm_bprob& B = probs[pidx[gi]];
+  if (!B.ctl->eval_on) return;
+  const uint32_t tc = (uint32_t)((CCM_NAP * B.L + EP_T - 1) / EP_T);
+  pair_epilogue_block(B.gt, B.P, B.Lp, B.hsym, B.x + B.nsp, B.g + B.nsp, B.L, B.lam2, tc, B.regpair,
+                      blockIdx.x - offs[gi]);
 }
 
 // symmetrised -2 H2 in the x2 layout with zero self-edge blocks and padding:
@@ -102,12 +123,12 @@ __global__ __launch_bounds__(256) void k_hsym(const float* __restrict__ Mh, long
   }
 }
 
-__global__ __launch_bounds__(256) void k_singles_finalize(const double* __restrict__ g1part, int nblk,
-                                                          const float* __restrict__ x1,
-                                                          float* __restrict__ g1, int L, long nsp,
-                                                          float lam1, double* __restrict__ regpart) {
+__device__ __forceinline__ void singles_block(const double* __restrict__ g1part, int nblk,
+                                              const float* __restrict__ x1, float* __restrict__ g1,
+                                              int L, long nsp, float lam1, double* __restrict__ regpart,
+                                              uint32_t blk) {
   __shared__ double red[4];
-  const long v = (long)blockIdx.x * 256 + threadIdx.x;
+  const long v = (long)blk * 256 + threadIdx.x;
   const long ns = 20L * L;
   double reg = 0.0;
   if (v < ns) {
@@ -123,7 +144,25 @@ __global__ __launch_bounds__(256) void k_singles_finalize(const double* __restri
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) red[wv] = reg;
   __syncthreads();
-  if (threadIdx.x == 0) regpart[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) regpart[blk] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void k_singles_finalize(const double* __restrict__ g1part, int nblk,
+                                                          const float* __restrict__ x1,
+                                                          float* __restrict__ g1, int L, long nsp,
+                                                          float lam1, double* __restrict__ regpart) {
+  singles_block(g1part, nblk, x1, g1, L, nsp, lam1, regpart, blockIdx.x);
+}
+
+// every alignment of a group: singles_blocks(L) blocks each
+__global__ __launch_bounds__(256) void k_singles_finalize_grouped(const ccm_bprob* __restrict__ probs,
+                                                                  const uint32_t* __restrict__ offs,
+                                                                  const int32_t* __restrict__ pidx,
+                                                                  int np) {
+  const int gi = ccm_group_find(offs, np, blockIdx.x);
+  const ccm_bprob& B = probs[pidx[gi]];
+  if (!B.ctl->eval_on) return;
+  singles_block(B.g1part, B.n_g1blk, B.x, B.g, B.L, B.nsp, B.lam1, B.regsingle, blockIdx.x - offs[gi]);
 }
 
 // regularisation terms alone: lam1 sum V^2 + 0.5 lam2 sum x2^2 (padded block)
@@ -149,10 +188,10 @@ __global__ __launch_bounds__(256) void k_reg_only(const float* __restrict__ x, l
 
 // Neumaier-compensated FP64 sum of up to three partial arrays, fixed order.
 // out[0] = total (double)
-__global__ __launch_bounds__(256) void k_sum3(const double* __restrict__ a, int na,
-                                              const double* __restrict__ b, int nb,
-                                              const double* __restrict__ c, int nc,
-                                              double* __restrict__ out) {
+__device__ __forceinline__ void sum3_block(const double* __restrict__ a, int na,
+                                           const double* __restrict__ b, int nb,
+                                           const double* __restrict__ c, int nc,
+                                           double* __restrict__ out) {
   __shared__ double sh[256], shc[256];
   double s = 0.0, comp = 0.0;
   auto add = [&](double v) {
@@ -178,6 +217,22 @@ __global__ __launch_bounds__(256) void k_sum3(const double* __restrict__ a, int 
     }
     out[0] = s + comp;
   }
+}
+
+__global__ __launch_bounds__(256) void k_sum3(const double* __restrict__ a, int na,
+                                              const double* __restrict__ b, int nb,
+                                              const double* __restrict__ c, int nc,
+                                              double* __restrict__ out) {
+  sum3_block(a, na, b, nb, c, nc, out);
+}
+
+// one block per alignment of a group: f = softmax partials + pair and
+// single regularisation partials, into the alignment's control block
+__global__ __launch_bounds__(256) void k_sum3_grouped(const ccm_bprob* __restrict__ probs,
+                                                      const int32_t* __restrict__ pidx) {
+  const ccm_bprob& B = probs[pidx[blockIdx.x]];
+  if (!B.ctl->eval_on) return;
+  sum3_block(B.fpart, B.n_fpart, B.regpair, B.n_regpair, B.regsingle, B.n_regsingle, &B.ctl->fx);
 }
 
 extern "C" {
@@ -229,6 +284,29 @@ int ccm_launch_reg_only(const float* d_x, int L, float lam1, float lam2, double*
 int ccm_launch_sum3(const double* a, int na, const double* b, int nb, const double* c, int nc,
                     double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_sum3, dim3(1), dim3(256), 0, s, a, na, b, nb, c, nc, out);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_pair_epilogue_grouped(const ccm_bprob* d_probs, ccm_group g, hipStream_t s) {
+  if (g.np <= 0 || g.total == 0) return CCM_ERR_ARG;
+  hipLaunchKernelGGL(k_pair_epilogue_grouped, dim3(g.total), dim3(256), 0, s, d_probs, g.offs, g.pidx, g.np);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_singles_finalize_grouped(const ccm_bprob* d_probs, ccm_group g, hipStream_t s) {
+  if (g.np <= 0 || g.total == 0) return CCM_ERR_ARG;
+  hipLaunchKernelGGL(k_singles_finalize_grouped, dim3(g.total), dim3(256), 0, s, d_probs, g.offs, g.pidx,
+                     g.np);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+// g.total is ignored: one block per group entry
+int ccm_launch_sum3_grouped(const ccm_bprob* d_probs, ccm_group g, hipStream_t s) {
+  if (g.np <= 0) return CCM_ERR_ARG;
+  hipLaunchKernelGGL(k_sum3_grouped, dim3(g.np), dim3(256), 0, s, d_probs, g.pidx);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }

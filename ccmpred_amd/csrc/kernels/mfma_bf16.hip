@@ -19,12 +19,12 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 // coupling matrix -> Wt (bf16, P x lda), zero outside the real 21L x 21L block
-__global__ __launch_bounds__(256) void k_convert_wt(const float* __restrict__ x2, int L, int Lp,
-                                                    uint16_t* __restrict__ wt, long lda, int passes,
-                                                    uint32_t tiles_r) {
+__device__ __forceinline__ void convert_wt_block(const float* __restrict__ x2, int L, int Lp,
+                                                 uint16_t* __restrict__ wt, long lda, int passes,
+                                                 uint32_t tiles_r, uint32_t blk) {
   __shared__ float T[64][65];
   const int P = CCM_NA * Lp;
-  const int tc = blockIdx.x / tiles_r, tr = blockIdx.x % tiles_r;
+  const int tc = blk / tiles_r, tr = blk % tiles_r;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long ld2 = (long)CCM_NAP * L;
   {
@@ -53,6 +53,23 @@ __global__ __launch_bounds__(256) void k_convert_wt(const float* __restrict__ x2
   }
 }
 
+__global__ __launch_bounds__(256) void k_convert_wt(const float* __restrict__ x2, int L, int Lp,
+                                                    uint16_t* __restrict__ wt, long lda, int passes,
+                                                    uint32_t tiles_r) {
+  convert_wt_block(x2, L, Lp, wt, lda, passes, tiles_r, blockIdx.x);
+}
+
+// every alignment of a group: (P/64)^2 blocks each
+__global__ __launch_bounds__(256) void k_convert_wt_grouped(const ccm_bprob* __restrict__ probs,
+                                                            const uint32_t* __restrict__ offs,
+                                                            const int32_t* __restrict__ pidx, int np) {
+  const int gi = ccm_group_find(offs, np, blockIdx.x);
+  const ccm_bprob& B = probs[pidx[gi]];
+  if (!B.ctl->eval_on) return;
+  convert_wt_block(B.x + B.nsp, B.L, B.Lp, B.wt, B.ldwt, B.passes_fwd, (uint32_t)((B.P + 63) / 64),
+                   blockIdx.x - offs[gi]);
+}
+
 __device__ __forceinline__ uint32_t oh_pair(uint32_t lo_byte, uint32_t hi_byte, uint32_t b) {
   return (lo_byte == b ? 0x3F80u : 0u) | (hi_byte == b ? 0x3F800000u : 0u);
 }
@@ -77,11 +94,11 @@ __device__ __forceinline__ bf16x8_t onehot8(uint2 v, uint32_t b) {
 // MODE 0: forward.  cols = sequences n;     B[kk][n] = [msap[n][kk%Lp] == kk/Lp], Kb = P
 // MODE 1: gradient. cols = coupling rows r; B[kk][r] = [msapt[r%Lp][kk] == r/Lp], Kb = chunk
 // Requires Lp % TK == 0 (forward) and Kb % TK == 0.
-template <int MODE, int WM, int WN, int RB, int CB, int TK, int PF, int OCC>
-__global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot(
-    const uint16_t* __restrict__ A, long lda, int M, int Kb, int passes,
-    const uint8_t* __restrict__ bytes, long ldbytes, int Lp, int Ncol, float* __restrict__ C,
-    long ldc, int accumulate, uint32_t tiles_n, uint32_t tiles, uint32_t total) {
+template <int MODE, int WM, int WN, int RB, int CB, int TK, int PF>
+__device__ __forceinline__ void gemm_onehot_tile(const uint16_t* __restrict__ A, long lda, int M, int Kb,
+                                                 int passes, const uint8_t* __restrict__ bytes,
+                                                 long ldbytes, int Lp, int Ncol, float* __restrict__ C,
+                                                 long ldc, int accumulate, int mt, int nt) {
   constexpr int NT = 64 * WM * WN;
   constexpr int TM = WM * RB * 32, TN = WN * CB * 32;
   constexpr int LDA = TK + 8;  // LDS row stride in bf16
@@ -90,9 +107,6 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot(
   static_assert((TM * KCH) % NT == 0, "loader mapping");
   __shared__ __attribute__((aligned(16))) uint16_t As[2][TM * LDA];
 
-  const uint32_t qid = ccm_xcd_remap(blockIdx.x, total);
-  if (qid >= tiles) return;
-  const int mt = qid / tiles_n, nt = qid % tiles_n;
   const int m0 = mt * TM, n0 = nt * TN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WN, wn = wave % WN;
@@ -237,6 +251,44 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot(
   }
 }
 
+template <int MODE, int WM, int WN, int RB, int CB, int TK, int PF, int OCC>
+__global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot(
+    const uint16_t* __restrict__ A, long lda, int M, int Kb, int passes,
+    const uint8_t* __restrict__ bytes, long ldbytes, int Lp, int Ncol, float* __restrict__ C,
+    long ldc, int accumulate, uint32_t tiles_n, uint32_t tiles, uint32_t total) {
+  const uint32_t qid = ccm_xcd_remap(blockIdx.x, total);
+  if (qid >= tiles) return;
+  gemm_onehot_tile<MODE, WM, WN, RB, CB, TK, PF>(A, lda, M, Kb, passes, bytes, ldbytes, Lp, Ncol, C, ldc,
+                                                 accumulate, qid / tiles_n, qid % tiles_n);
+}
+
+// The same tiles over a group of alignments, all with this tile variant:
+// the XCD remap runs over the whole grid, so that each XCD takes a run of
+// consecutive tiles (mostly of one alignment).
+//   MODE 0: Zt = Wt . onehot(msap)      (M = P, K = P, cols = ncols)
+//   MODE 1: Gt = Dt . onehot(msapt)^T   (M = P, K = ncols, cols = P)
+template <int MODE, int WM, int WN, int RB, int CB, int TK, int PF, int OCC>
+__global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot_grouped(
+    const ccm_bprob* __restrict__ probs, const uint32_t* __restrict__ offs,
+    const int32_t* __restrict__ pidx, int np, uint32_t total) {
+  constexpr int TN = WN * CB * 32;
+  const uint32_t qid = ccm_xcd_remap(blockIdx.x, total);
+  if (qid >= offs[np]) return;
+  const int gi = ccm_group_find(offs, np, qid);
+  const ccm_bprob& B = probs[pidx[gi]];
+  if (!B.ctl->eval_on) return;
+  const uint32_t t = qid - offs[gi];
+  if (MODE == 0) {
+    const uint32_t tiles_n = (uint32_t)((B.ncols + TN - 1) / TN);
+    gemm_onehot_tile<0, WM, WN, RB, CB, TK, PF>(B.wt, B.ldwt, B.P, B.P, B.passes_fwd, B.msap, B.Lp, B.Lp,
+                                                B.ncols, B.zt, B.ncols, 0, t / tiles_n, t % tiles_n);
+  } else {
+    const uint32_t tiles_n = (uint32_t)((B.P + TN - 1) / TN);
+    gemm_onehot_tile<1, WM, WN, RB, CB, TK, PF>(B.dt, B.lddt, B.P, B.ncols, B.passes, B.msapt, B.ldmt,
+                                                B.Lp, B.P, B.gt, B.P, 0, t / tiles_n, t % tiles_n);
+  }
+}
+
 static inline uint32_t round8(uint32_t v) { return (v + 7u) & ~7u; }
 
 extern "C" {
@@ -303,6 +355,36 @@ static int launch_gemm(int variant, const uint16_t* A, long lda, int M, int Kb, 
   return CCM_OK;
 }
 
+template <int MODE>
+static int launch_gemm_grouped(int variant, const ccm_bprob* probs, const ccm_group& g, hipStream_t s) {
+  if (variant < 0 || variant >= GM_NVAR || g.np <= 0) return CCM_ERR_ARG;
+  const uint32_t total = round8(g.total);
+#define GG_ARGS probs, g.offs, g.pidx, g.np, total
+  switch (variant) {
+    case 0:
+      hipLaunchKernelGGL((k_gemm_onehot_grouped<MODE, 2, 4, 4, 2, 64, 1, 1>), dim3(total), dim3(512), 0, s, GG_ARGS);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_gemm_onehot_grouped<MODE, 2, 2, 4, 2, 64, 1, 2>), dim3(total), dim3(256), 0, s, GG_ARGS);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_gemm_onehot_grouped<MODE, 2, 2, 2, 2, 64, 2, 2>), dim3(total), dim3(256), 0, s, GG_ARGS);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_gemm_onehot_grouped<MODE, 2, 2, 4, 4, 64, 2, 1>), dim3(total), dim3(256), 0, s, GG_ARGS);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_gemm_onehot_grouped<MODE, 2, 4, 4, 2, 128, 1, 1>), dim3(total), dim3(512), 0, s, GG_ARGS);
+      break;
+    default:
+      hipLaunchKernelGGL((k_gemm_onehot_grouped<MODE, 2, 2, 4, 4, 64, 1, 1>), dim3(total), dim3(256), 0, s, GG_ARGS);
+      break;
+  }
+#undef GG_ARGS
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
 // workgroups of a launch for a tile variant
 static uint32_t gemm_tiles(int variant, int M, int Ncol) {
   return (uint32_t)(((M + gm_tm[variant] - 1) / gm_tm[variant]) * ((Ncol + gm_tn[variant] - 1) / gm_tn[variant]));
@@ -320,6 +402,48 @@ static int gemm_default_variant(int M, int Ncol, int kstep_ok128) {
 extern "C" {
 
 int ccm_gemm_variants(void) { return GM_NVAR; }
+
+// tiles of a variant over an M x Ncol output (0 for an unknown variant)
+uint32_t ccm_gemm_tile_count(int variant, int M, int Ncol) {
+  if (variant < 0 || variant >= GM_NVAR) return 0;
+  return gemm_tiles(variant, M, Ncol);
+}
+
+// whether a variant's K step divides the contraction: the forward needs
+// Lp % TK == 0 (mode 0), the gradient kcols % TK == 0 (mode 1)
+int ccm_gemm_variant_ok(int variant, int mode, int Lp, int kcols) {
+  if (variant < 0 || variant >= GM_NVAR) return 0;
+  return mode == 0 ? Lp % gm_tk[variant] == 0 : kcols % gm_tk[variant] == 0;
+}
+
+// default variant of the forward (mode 0) or gradient (mode 1) contraction
+int ccm_gemm_default_variant_for(int mode, int Lp, int ncols) {
+  const int P = CCM_NA * Lp;
+  return mode == 0 ? gemm_default_variant(P, ncols, Lp % 128 == 0)
+                   : gemm_default_variant(P, P, ncols % 128 == 0);
+}
+
+int ccm_convert_wt_blocks(int Lp) {
+  const int t = (CCM_NA * Lp + 63) / 64;
+  return t * t;
+}
+
+// grouped forms: g.offs holds each alignment's block range; every alignment
+// of a GEMM group must accept the variant (ccm_gemm_variant_ok)
+int ccm_launch_convert_wt_grouped(const ccm_bprob* d_probs, ccm_group g, hipStream_t s) {
+  if (g.np <= 0 || g.total == 0) return CCM_ERR_ARG;
+  hipLaunchKernelGGL(k_convert_wt_grouped, dim3(g.total), dim3(256), 0, s, d_probs, g.offs, g.pidx, g.np);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_gemm_fwd_grouped(const ccm_bprob* d_probs, ccm_group g, int variant, hipStream_t s) {
+  return launch_gemm_grouped<0>(variant, d_probs, g, s);
+}
+
+int ccm_launch_gemm_grad_grouped(const ccm_bprob* d_probs, ccm_group g, int variant, hipStream_t s) {
+  return launch_gemm_grouped<1>(variant, d_probs, g, s);
+}
 
 int ccm_gemm_default_variant(int M, int Ncol) { return gemm_default_variant(M, Ncol, 1); }
 

@@ -92,18 +92,17 @@ __global__ __launch_bounds__(256) void k_softmax_rows(float* __restrict__ ZD, lo
 // One block: column j (blockIdx.y) for 2048 consecutive sequences; each
 // thread takes 8 sequences 256 apart and accumulates its g1 terms before a
 // single wave reduction per residue.
-__global__ __launch_bounds__(256) void k_softmax_cols(const float* __restrict__ Zt, long ldzt,
-                                                      const float* __restrict__ x1,
-                                                      const uint8_t* __restrict__ msapt, long ldmt,
-                                                      const float* __restrict__ w, int L, int Lp,
-                                                      int Nc, int ncols, uint16_t* __restrict__ Dt,
-                                                      long lddt, long lo_off, int passes, int mode,
-                                                      int blk0, double* __restrict__ fpart,
-                                                      double* __restrict__ g1part) {
+__device__ __forceinline__ void softmax_cols_block(const float* __restrict__ Zt, long ldzt,
+                                                   const float* __restrict__ x1,
+                                                   const uint8_t* __restrict__ msapt, long ldmt,
+                                                   const float* __restrict__ w, int L, int Lp, int Nc,
+                                                   int ncols, uint16_t* __restrict__ Dt, long lddt,
+                                                   long lo_off, int passes, int mode, int blk0,
+                                                   double* __restrict__ fpart,
+                                                   double* __restrict__ g1part, int bx, int j) {
   __shared__ float g1red[4][20];
   __shared__ double fred[4];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int j = blockIdx.y;
   float v1[20];
 #pragma unroll
   for (int a = 0; a < 20; a++) v1[a] = x1[(long)a * L + j];
@@ -112,7 +111,7 @@ __global__ __launch_bounds__(256) void k_softmax_cols(const float* __restrict__ 
   for (int a = 0; a < 20; a++) g1[a] = 0.f;
   double fl = 0.0;
   for (int r = 0; r < SC_R; r++) {
-    const int n = blockIdx.x * (256 * SC_R) + r * 256 + threadIdx.x;
+    const int n = bx * (256 * SC_R) + r * 256 + threadIdx.x;
     if (n >= ncols) break;
     uint16_t* dcol = Dt + (long)j * lddt + n;
     if (n >= Nc) {
@@ -167,13 +166,38 @@ __global__ __launch_bounds__(256) void k_softmax_cols(const float* __restrict__ 
   fl = ccm_wave_sum_f64(fl);
   if (lane == 0) fred[wv] = fl;
   __syncthreads();
-  const long gb = (long)blk0 + blockIdx.x;
+  const long gb = (long)blk0 + bx;
   if (mode && threadIdx.x < 20) {
     const int a = threadIdx.x;
     g1part[(gb * 20 + a) * L + j] = (double)g1red[0][a] + (double)g1red[1][a] + (double)g1red[2][a] +
                                     (double)g1red[3][a];
   }
   if (threadIdx.x == 0) fpart[gb * L + j] = fred[0] + fred[1] + fred[2] + fred[3];
+}
+
+__global__ __launch_bounds__(256) void k_softmax_cols(const float* __restrict__ Zt, long ldzt,
+                                                      const float* __restrict__ x1,
+                                                      const uint8_t* __restrict__ msapt, long ldmt,
+                                                      const float* __restrict__ w, int L, int Lp,
+                                                      int Nc, int ncols, uint16_t* __restrict__ Dt,
+                                                      long lddt, long lo_off, int passes, int mode,
+                                                      int blk0, double* __restrict__ fpart,
+                                                      double* __restrict__ g1part) {
+  softmax_cols_block(Zt, ldzt, x1, msapt, ldmt, w, L, Lp, Nc, ncols, Dt, lddt, lo_off, passes, mode, blk0,
+                     fpart, g1part, blockIdx.x, blockIdx.y);
+}
+
+// every alignment of a group, whole alignment as one chunk: nblk(N) * L blocks
+__global__ __launch_bounds__(256) void k_softmax_cols_grouped(const ccm_bprob* __restrict__ probs,
+                                                              const uint32_t* __restrict__ offs,
+                                                              const int32_t* __restrict__ pidx, int np) {
+  const int gi = ccm_group_find(offs, np, blockIdx.x);
+  const ccm_bprob& B = probs[pidx[gi]];
+  if (!B.ctl->eval_on) return;
+  const int nb = (B.ncols + 256 * SC_R - 1) / (256 * SC_R);
+  const int t = (int)(blockIdx.x - offs[gi]);
+  softmax_cols_block(B.zt, B.ncols, B.x, B.msapt, B.ldmt, B.w, B.L, B.Lp, B.N, B.ncols, B.dt, B.lddt,
+                     B.ncols, B.passes, 1, 0, B.fpart, B.g1part, t % nb, t / nb);
 }
 
 extern "C" {
@@ -210,6 +234,13 @@ int ccm_launch_softmax_cols(const float* d_zt, long ldzt, const float* d_x1, con
   hipLaunchKernelGGL(k_softmax_cols, dim3(nb, L), dim3(256), 0, s, d_zt, ldzt, d_x1, d_msapt, ldmt, d_w, L,
                      Lp, Nc, ncols, d_dt, lddt, lo_off, passes, mode, n0 / (256 * SC_R), d_fpart,
                      d_g1part);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_softmax_cols_grouped(const ccm_bprob* d_probs, ccm_group g, hipStream_t s) {
+  if (g.np <= 0 || g.total == 0) return CCM_ERR_ARG;
+  hipLaunchKernelGGL(k_softmax_cols_grouped, dim3(g.total), dim3(256), 0, s, d_probs, g.offs, g.pidx, g.np);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }

@@ -68,7 +68,7 @@ typedef struct ccm_hip_config {
 	int bf16_passes; /* gradient contraction: 1 = BF16 operand, 2 = hi+lo BF16 split */
 	int bf16_fwd_passes; /* forward contraction, same meaning; 0 = as bf16_passes */
 	int verbose;     /* 1: print the reweighting summary like the reference */
-	int chunk_rows;  /* sequences per chunk (rounded up to 256); 0 = all at once */
+	int chunk_rows;  /* sequences per chunk (rounded up to 256 in FP32, 2048 in BF16); 0 = all */
 	int gemm_variant; /* MFMA tile variant (0..ccm_gemm_variants()-1); -1 = default */
 } ccm_hip_config;
 
@@ -156,6 +156,59 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
                          conjugrad_progress_t progress, void *instance,
                          const ccm_conjugrad_parameter *param, int *iterations, int *evaluations,
                          double *fhist);
+
+/* ---- batched form: many alignments optimised together ----------------- */
+/* No counterpart in the reference (one alignment per ccmpred process).
+ * Every alignment gets the BF16 evaluator of init_hip_ex (unchunked) and its
+ * own device vectors x, g, gprev, s; each step of an evaluation or of the
+ * conjugate gradient is one grouped launch over all alignments, and the line
+ * search runs as a device-side state machine (the decisions of
+ * ccm_conjugrad_gpu_ex), so that a whole CG round is host-free and rounds
+ * are replayed from a captured hipGraph. */
+typedef struct ccm_batch ccm_batch;
+
+/* uds[0..nprob): msa, ncol, nrow, lambdas and reweighting threshold set by
+ * the caller; weights (host, nrow floats) are filled as init_hip does.
+ * cfg: precision must be CCM_PRECISION_BF16 (chunk_rows is ignored).
+ * Returns 0, or non-zero with *out = NULL. */
+int ccm_batch_create(userdata *uds, int nprob, const ccm_hip_config *cfg, ccm_batch **out);
+void ccm_batch_destroy(ccm_batch *b);
+int ccm_batch_size(const ccm_batch *b);
+/* device vectors of alignment p (nvar_padded floats): x is the caller's to
+ * fill before a solve and holds the result after it; g the last gradient */
+float *ccm_batch_x(ccm_batch *b, int p);
+float *ccm_batch_g(ccm_batch *b, int p);
+/* host copies of alignment p's vectors (nvar_padded floats each; read:
+ * either pointer may be NULL) */
+int ccm_batch_write_x(ccm_batch *b, int p, const float *host_x);
+int ccm_batch_read(ccm_batch *b, int p, float *host_x, float *host_g);
+void *ccm_batch_stream(ccm_batch *b);
+size_t ccm_batch_device_bytes(const ccm_batch *b);
+
+/* GEMM tile variant per alignment for the forward (fwd) and gradient (grad)
+ * contractions; set: NULL or -1 entries take the default of the shape. */
+int ccm_batch_set_variants(ccm_batch *b, const int *fwd, const int *grad);
+int ccm_batch_get_variants(const ccm_batch *b, int *fwd, int *grad);
+/* Autotuner: times the grouped contractions on the device (median of reps)
+ * over tile assignments and keeps the fastest; ms[0..1] receive the tuned
+ * forward and gradient times (may be NULL). */
+int ccm_batch_autotune(ccm_batch *b, int reps, double *ms);
+
+/* One grouped evaluation of every alignment at its x: gradients into
+ * ccm_batch_g, f[0..nprob) FP64 objectives. */
+int ccm_batch_evaluate(ccm_batch *b, double *f);
+
+/* Conjugate gradient on every alignment from its x, rounds_per_graph CG
+ * rounds (vector step, evaluation, dots, control) per captured graph.
+ * *rounds (may be NULL) receives the rounds run. */
+int ccm_batch_solve(ccm_batch *b, const ccm_conjugrad_parameter *param, int rounds_per_graph,
+                    int *rounds);
+/* Outcome of alignment p after ccm_batch_solve: status (CONJUGRAD_* codes),
+ * final f (as the single-alignment CG reports it), counters and the
+ * per-iteration log (4 doubles per iteration: f, step, line-search trials,
+ * <g,g>; room for max_iterations entries, may be NULL). */
+int ccm_batch_result(ccm_batch *b, int p, int *status, double *fx, int *iterations,
+                     int *evaluations, double *log);
 
 /* ---- host model set-up and contact-map output ------------------------- */
 

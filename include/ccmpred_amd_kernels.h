@@ -12,6 +12,8 @@
 
 #include <stdint.h>
 
+#include "ccmpred_amd_batch.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -85,6 +87,31 @@ int ccm_launch_dots(const float *a, const float *b0, const float *b1, const floa
 int ccm_launch_axpy(float *x, const float *sdir, float alpha, long n, ccm_stream_t s);
 int ccm_launch_direction(float *sdir, const float *g, float beta, long n, ccm_stream_t s);
 int ccm_launch_negate(float *sdir, const float *g, long n, ccm_stream_t s);
+
+/* Grouped forms for a batch of alignments (descriptors in
+ * ccmpred_amd_batch.h).  No counterpart in the reference, which runs one
+ * alignment per process.  Each step is one launch over every alignment of
+ * the group, with the device code of the single-alignment launchers above;
+ * d_probs is a device array of ccm_bprob and a group's offs/pidx live in
+ * device memory.  Alignments whose control block has eval_on == 0
+ * (vec_on == 0 for the vector step) return at once. */
+int ccm_convert_wt_blocks(int Lp);
+uint32_t ccm_gemm_tile_count(int variant, int M, int Ncol);
+int ccm_gemm_variant_ok(int variant, int mode, int Lp, int kcols);
+int ccm_gemm_default_variant_for(int mode, int Lp, int ncols);
+int ccm_dots_blocks(long n);
+int ccm_cg_vec_blocks(long n);
+int ccm_launch_convert_wt_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
+int ccm_launch_gemm_fwd_grouped(const ccm_bprob *d_probs, ccm_group g, int variant, ccm_stream_t s);
+int ccm_launch_gemm_grad_grouped(const ccm_bprob *d_probs, ccm_group g, int variant, ccm_stream_t s);
+int ccm_launch_softmax_cols_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
+int ccm_launch_pair_epilogue_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
+int ccm_launch_singles_finalize_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
+int ccm_launch_sum3_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
+int ccm_launch_dots_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_group gfinal, ccm_stream_t s);
+int ccm_launch_cg_vec_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
+int ccm_launch_cg_ctrl(ccm_bprob *d_probs, int nprob, const ccm_cg_prm *prm, int32_t *d_nactive,
+                       ccm_stream_t s);
 
 #ifdef __cplusplus
 }

@@ -467,21 +467,23 @@ int ora_cg(int n, float *x, double *fx, ora_eval_fn eval, void *ctx, const ora_c
 				break;
 			}
 		}
+		const double dgs = ora_dot(g, s, n);
 		double ggn = ora_dot(g, g, n);
 		double ggp = ora_dot(g, gprev, n);
 		double beta = (ggn - ggp) / gg;
 		if (beta < 0)
 			beta = 0;
-		const float fb = (float)beta;
-		for (long i = 0; i < n; i++)
-			s[i] = -g[i] + fb * s[i];
-		double dgn = ora_dot(g, s, n);
+		/* slope of the new direction -g + beta s, from the dots above */
+		double dgn = -ggn + beta * dgs;
 		if (dgn >= 0) {
 			for (long i = 0; i < n; i++)
 				s[i] = -g[i];
 			dgn = -ggn;
 			steepest = 1;
 		} else {
+			const float fb = (float)beta;
+			for (long i = 0; i < n; i++)
+				s[i] = -g[i] + fb * s[i];
 			steepest = beta == 0.0;
 		}
 		alpha = alpha * dg0 / dgn;
