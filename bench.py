@@ -169,6 +169,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-evals", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run a device copy of this many MiB (counter calibration)")
     ap.add_argument("--batched", action="store_true",
                     help="time the batched form (many MSAs) instead of single evaluations")
     ap.add_argument("--batch-size", type=int, default=64)
@@ -191,6 +193,16 @@ def main():
     from ccmpred_amd.msa import synthetic_msa
     msa, _ = synthetic_msa(L_COLS, N_SEQS, SEED, N_PAIRS)
     x = make_x(msa)
+    if args.copy_mib:
+        # a plain device copy of known size: the reference point that scales
+        # the FETCH_SIZE / WRITE_SIZE counters of a profiled run to bytes
+        import torch
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     bf16 = time_gpu(msa, x, "bf16", args.steps, args.warmup)
     fp32 = time_gpu(msa, x, "fp32", max(10, args.steps // 4), max(2, args.warmup // 2))

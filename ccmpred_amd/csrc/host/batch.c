@@ -37,11 +37,11 @@
 
 #include "state.h"
 
-#define NVAR_MAX 16 /* GEMM tile variants a batch can hold */
+#define NVAR_MAX 16 /* GEMM launches (tile variant x K step) per contraction */
 
 typedef struct {
 	ccm_group g;
-	int variant; /* GEMM groups only */
+	int variant, tk; /* GEMM groups only */
 } bgroup;
 
 struct ccm_batch {
@@ -188,19 +188,24 @@ static int build_tables(ccm_batch *b)
 	    add_group(&t, &b->g_vec, b->h_probs, all, b->n, K_VEC, 0, 0))
 		goto out;
 	b->n_fwd = b->n_grad = 0;
+	/* one GEMM launch per (tile variant, K step) in use */
 	for (int mode = 0; mode < 2; mode++) {
 		const int *var = mode == 0 ? b->var_fwd : b->var_grad;
-		for (int v = 0; v < ccm_gemm_variants() && v < NVAR_MAX; v++) {
-			int np = 0;
-			for (int i = 0; i < b->n; i++)
-				if (var[i] == v)
-					list[np++] = i;
-			if (!np)
-				continue;
-			bgroup *bg = mode == 0 ? &b->g_fwd[b->n_fwd++] : &b->g_grad[b->n_grad++];
-			if (add_group(&t, bg, b->h_probs, list, np, -1, mode, v))
-				goto out;
-		}
+		for (int v = 0; v < ccm_gemm_variants(); v++)
+			for (int tk = 64; tk <= 128; tk *= 2) {
+				int np = 0;
+				for (int i = 0; i < b->n; i++)
+					if (var[i] == v && ccm_gemm_tk(mode, b->h_probs[i].Lp) == tk)
+						list[np++] = i;
+				if (!np)
+					continue;
+				if ((mode == 0 ? b->n_fwd : b->n_grad) >= NVAR_MAX)
+					goto out;
+				bgroup *bg = mode == 0 ? &b->g_fwd[b->n_fwd++] : &b->g_grad[b->n_grad++];
+				if (add_group(&t, bg, b->h_probs, list, np, -1, mode, v))
+					goto out;
+				bg->tk = tk;
+			}
 	}
 	if (b->d_tab && b->tab_len < t.len) {
 		(void)hipFree(b->d_tab);
@@ -424,7 +429,7 @@ int ccm_batch_set_variants(ccm_batch *b, const int *fwd, const int *grad)
 			vf = ccm_gemm_default_variant_for(0, p->Lp, p->ncols);
 		if (vg < 0)
 			vg = ccm_gemm_default_variant_for(1, p->Lp, p->ncols);
-		if (vf >= NVAR_MAX || vg >= NVAR_MAX || !ccm_gemm_variant_ok(vf, 0, p->Lp, p->ncols) ||
+		if (!ccm_gemm_variant_ok(vf, 0, p->Lp, p->ncols) ||
 		    !ccm_gemm_variant_ok(vg, 1, p->Lp, p->ncols)) {
 			fprintf(stderr, "ccmpred_amd batch: tile variant %d/%d does not fit alignment %d\n",
 			        vf, vg, i);
@@ -454,8 +459,9 @@ static int enqueue_gemms(ccm_batch *b, int mode)
 	const int ng = mode == 0 ? b->n_fwd : b->n_grad;
 	const bgroup *gs = mode == 0 ? b->g_fwd : b->g_grad;
 	for (int i = 0; i < ng; i++) {
-		int r = mode == 0 ? ccm_launch_gemm_fwd_grouped(b->d_probs, gs[i].g, gs[i].variant, b->s)
-		                  : ccm_launch_gemm_grad_grouped(b->d_probs, gs[i].g, gs[i].variant, b->s);
+		int r = mode == 0
+		            ? ccm_launch_gemm_fwd_grouped(b->d_probs, gs[i].g, gs[i].variant, gs[i].tk, b->s)
+		            : ccm_launch_gemm_grad_grouped(b->d_probs, gs[i].g, gs[i].variant, gs[i].tk, b->s);
 		if (r)
 			return r;
 	}
@@ -566,7 +572,7 @@ fail:
  * the grouped launches get faster by more than 1 %. */
 int ccm_batch_autotune(ccm_batch *b, int reps, double *ms_out)
 {
-	const int n = b->n, nv = ccm_gemm_variants() < NVAR_MAX ? ccm_gemm_variants() : NVAR_MAX;
+	const int n = b->n, nv = ccm_gemm_variants();
 	int *cur = (int *)malloc(sizeof(int) * (size_t)n), *trial = (int *)malloc(sizeof(int) * (size_t)n);
 	int *order = (int *)malloc(sizeof(int) * (size_t)n);
 	double *area = (double *)malloc(sizeof(double) * (size_t)n);

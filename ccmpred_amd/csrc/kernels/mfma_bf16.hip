@@ -87,14 +87,21 @@ __device__ __forceinline__ bf16x8_t onehot8(uint2 v, uint32_t b) {
 
 // One workgroup computes a TM x TN tile of C with WM x WN waves; each wave
 // owns RB x CB blocks of 32 x 32 and issues v_mfma_f32_32x32x16_bf16.  The
-// dense operand moves through LDS in K steps of TK (two stages, the next one
-// prefetched through registers while the current one is consumed); the
-// one-hot operand is synthesised per lane from 8 MSA bytes per fragment.
+// dense operand moves through LDS in K steps of TK (two LDS stages, the next
+// step prefetched through registers while the current one is used).  Within
+// a step, lane half q of a wave takes the K range [q TK/2, (q+1) TK/2) and
+// sub-step s the 8 indices s*8.. of it (the order of K inside an MFMA is
+// free as long as both operands agree), so that the one-hot bytes a lane
+// needs for a step are TK/2 contiguous bytes, fetched with 16-byte loads.
 //
-// MODE 0: forward.  cols = sequences n;     B[kk][n] = [msap[n][kk%Lp] == kk/Lp], Kb = P
-// MODE 1: gradient. cols = coupling rows r; B[kk][r] = [msapt[r%Lp][kk] == r/Lp], Kb = chunk
+// MODE 0: forward.  cols = sequences n; K index (pass, bb, k) -> A column
+//   pass*P + bb*Lp + k, B = [msap[n][k] == bb], Kb = P.  The K loop takes
+//   blocks of TK columns k outermost and (pass, bb) inside, so that a lane's
+//   MSA bytes are loaded once per block and serve all 21*passes steps.
+// MODE 1: gradient. cols = coupling rows r = bb*Lp + k; K = sequences n of
+//   the chunk (Kb of them): B = [msapt[k][n] == bb], bytes loaded per step.
 // Requires Lp % TK == 0 (forward) and Kb % TK == 0.
-template <int MODE, int WM, int WN, int RB, int CB, int TK, int PF>
+template <int MODE, int WM, int WN, int RB, int CB, int TK>
 __device__ __forceinline__ void gemm_onehot_tile(const uint16_t* __restrict__ A, long lda, int M, int Kb,
                                                  int passes, const uint8_t* __restrict__ bytes,
                                                  long ldbytes, int Lp, int Ncol, float* __restrict__ C,
@@ -104,6 +111,7 @@ __device__ __forceinline__ void gemm_onehot_tile(const uint16_t* __restrict__ A,
   constexpr int LDA = TK + 8;  // LDS row stride in bf16
   constexpr int KCH = TK / 8;  // 16-byte chunks per row and step
   constexpr int CPT = TM * KCH / NT;
+  constexpr int HALF = TK / 2, NB16 = HALF / 16, SUBS = TK / 16;
   static_assert((TM * KCH) % NT == 0, "loader mapping");
   __shared__ __attribute__((aligned(16))) uint16_t As[2][TM * LDA];
 
@@ -122,11 +130,11 @@ __device__ __forceinline__ void gemm_onehot_tile(const uint16_t* __restrict__ A,
     if (MODE == 0) {
       bval[cb] = 0;
       // msap row n; columns past Ncol read row Ncol-1 and are never stored
-      bptr[cb] = bytes + (long)(col < Ncol ? col : Ncol - 1) * ldbytes;
+      bptr[cb] = bytes + (long)(col < Ncol ? col : Ncol - 1) * ldbytes + q * HALF;
     } else {
       const int bb = col / Lp, kk = col - bb * Lp;
-      bval[cb] = (uint32_t)bb;                // >= 21 past the real rows: never matches
-      bptr[cb] = bytes + (long)kk * ldbytes;  // msapt row k
+      bval[cb] = (uint32_t)bb;                           // >= 21 past the real rows: never matches
+      bptr[cb] = bytes + (long)kk * ldbytes + q * HALF;  // msapt row k
     }
   }
 
@@ -136,57 +144,64 @@ __device__ __forceinline__ void gemm_onehot_tile(const uint16_t* __restrict__ A,
 #pragma unroll
     for (int j = 0; j < CB; j++) acc[i][j] = (floatx16)0.0f;
 
-  const int nsteps = passes * Kb / TK;
+  // steps: MODE 0 (kc, pass, bb) with kc outermost; MODE 1 (pass, kc)
+  const int kblocks = (MODE == 0 ? Lp : Kb) / TK;
+  const int inner = MODE == 0 ? passes * CCM_NA : 1;
+  const int nsteps = MODE == 0 ? kblocks * inner : passes * kblocks;
+  auto acol = [&](int s) -> long {
+    if (MODE == 0) {
+      const int kc = s / inner, r = s - kc * inner, pass = r / CCM_NA, bb = r - pass * CCM_NA;
+      return (long)pass * Kb + (long)bb * Lp + (long)kc * TK;
+    } else {
+      const int pass = s / kblocks, kc = s - pass * kblocks;
+      return (long)pass * Kb + (long)kc * TK;
+    }
+  };
+
   // loader: chunk c = tid + i*NT -> row c / KCH, 16-byte column c % KCH; the
   // rows of one thread are NT/KCH apart
   constexpr int RSTEP = NT / KCH;
   const int arow0 = tid / KCH, acol8 = tid % KCH;
   const uint16_t* abase = A + (long)(m0 + arow0) * lda + acol8 * 8;
   const int adst0 = arow0 * LDA + acol8 * 8;
-  // uniform position of the current K step inside one pass: for the forward
-  // the coupling row block bb and the column offset kin; for the gradient
-  // the sequence offset kin
-  int bb = 0, kin = 0;
   auto load = [&](uint4(&r)[CPT], int step) {
+    const long c = acol(step);
 #pragma unroll
     for (int i = 0; i < CPT; i++)
       r[i] = (m0 + arow0 + i * RSTEP < M)
-                 ? *reinterpret_cast<const uint4*>(abase + (long)i * RSTEP * lda + (long)step * TK)
+                 ? *reinterpret_cast<const uint4*>(abase + (long)i * RSTEP * lda + c)
                  : make_uint4(0, 0, 0, 0);
   };
   auto store = [&](const uint4(&r)[CPT], int buf) {
 #pragma unroll
     for (int i = 0; i < CPT; i++) *reinterpret_cast<uint4*>(&As[buf][adst0 + i * RSTEP * LDA]) = r[i];
   };
-  auto advance = [&]() {
-    kin += TK;
-    if (MODE == 0) {
-      if (kin == Lp) {
-        kin = 0;
-        if (++bb == CCM_NA) bb = 0;
-      }
-    } else {
-      if (kin == Kb) kin = 0;
-    }
+  // a lane's one-hot bytes of K block kc
+  auto loadb = [&](uint4(&bv)[CB][NB16], int kc) {
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+      for (int i = 0; i < NB16; i++)
+        bv[cb][i] = *reinterpret_cast<const uint4*>(bptr[cb] + (long)kc * TK + 16 * i);
   };
   // with 16 accumulator blocks per wave the K sub-steps are not unrolled, so
   // that the scheduler does not hoist fragments of several sub-steps at once
-  constexpr int SUB_UNROLL = RB * CB >= 16 ? 1 : TK / 16;
-  auto compute = [&](int buf) {
+  constexpr int SUB_UNROLL = RB * CB >= 16 ? 1 : SUBS;
+  auto compute = [&](int buf, const uint4(&bv)[CB][NB16], uint32_t bsel) {
     const uint16_t* as = As[buf];
 #pragma unroll SUB_UNROLL
-    for (int sub = 0; sub < TK / 16; sub++) {
-      const int koff = kin + sub * 16 + 8 * q;  // first of this lane's 8 K indices
+    for (int sub = 0; sub < SUBS; sub++) {
       bf16x8_t bfrag[CB];
 #pragma unroll
       for (int cb = 0; cb < CB; cb++) {
-        const uint2 v = *reinterpret_cast<const uint2*>(bptr[cb] + koff);
-        bfrag[cb] = onehot8(v, MODE == 0 ? (uint32_t)bb : bval[cb]);
+        const uint4 w = bv[cb][sub >> 1];
+        const uint2 v = (sub & 1) ? make_uint2(w.z, w.w) : make_uint2(w.x, w.y);
+        bfrag[cb] = onehot8(v, MODE == 0 ? bsel : bval[cb]);
       }
 #pragma unroll
       for (int rb = 0; rb < RB; rb++) {
         const uint4 av = *reinterpret_cast<const uint4*>(
-            &as[((wm * RB + rb) * 32 + l32) * LDA + sub * 16 + 8 * q]);
+            &as[((wm * RB + rb) * 32 + l32) * LDA + q * HALF + sub * 8]);
         const bf16x8_t afrag = __builtin_bit_cast(bf16x8_t, av);
 #pragma unroll
         for (int cb = 0; cb < CB; cb++)
@@ -195,40 +210,39 @@ __device__ __forceinline__ void gemm_onehot_tile(const uint16_t* __restrict__ A,
     }
   };
 
-  if (PF == 1) {
-    // one register stage: the next step's tile is loaded while this one is used
-    uint4 pre[CPT];
-    load(pre, 0);
-    store(pre, 0);
-    __syncthreads();
-    for (int s = 0; s < nsteps; s++) {
-      const int cur = s & 1;
-      if (s + 1 < nsteps) load(pre, s + 1);
-      compute(cur);
-      advance();
-      if (s + 1 < nsteps) store(pre, cur ^ 1);
-      __syncthreads();
+  // MODE 1 prefetches the next step's bytes; MODE 0 reloads them once per
+  // K block (every 21*passes steps), after the block's last step
+  constexpr int NBX = MODE == 1 ? NB16 : 1;
+  uint4 pre[CPT];
+  uint4 bcur[CB][NB16], bnxt[CB][NBX];
+  load(pre, 0);
+  store(pre, 0);
+  loadb(bcur, 0);
+  __syncthreads();
+  for (int s = 0; s < nsteps; s++) {
+    const int cur = s & 1;
+    if (s + 1 < nsteps) {
+      load(pre, s + 1);
+      if constexpr (MODE == 1) loadb(bnxt, (s + 1) % kblocks);
     }
-  } else {
-    // two register stages: a tile is loaded two steps before it is used
-    uint4 ra[CPT], rb2[CPT];
-    load(ra, 0);
-    store(ra, 0);
-    if (nsteps > 1) load(rb2, 1);
-    __syncthreads();
-    for (int s = 0; s < nsteps; s += 2) {
-      if (s + 2 < nsteps) load(ra, s + 2);
-      compute(0);
-      advance();
-      if (s + 1 < nsteps) store(rb2, 1);
-      __syncthreads();
-      if (s + 1 >= nsteps) break;
-      if (s + 3 < nsteps) load(rb2, s + 3);
-      compute(1);
-      advance();
-      if (s + 2 < nsteps) store(ra, 0);
-      __syncthreads();
+    uint32_t bsel = 0;
+    if (MODE == 0) {
+      const int r = s % inner;
+      bsel = (uint32_t)(r - (r / CCM_NA) * CCM_NA);
     }
+    compute(cur, bcur, bsel);
+    if (s + 1 < nsteps) {
+      store(pre, cur ^ 1);
+      if constexpr (MODE == 1) {
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+          for (int i = 0; i < NB16; i++) bcur[cb][i] = bnxt[cb][i];
+      } else {
+        if ((s + 1) % inner == 0) loadb(bcur, (s + 1) / inner);
+      }
+    }
+    __syncthreads();
   }
 
   // 32x32 accumulator layout: lane -> column l32; register i -> row
@@ -251,14 +265,14 @@ __device__ __forceinline__ void gemm_onehot_tile(const uint16_t* __restrict__ A,
   }
 }
 
-template <int MODE, int WM, int WN, int RB, int CB, int TK, int PF, int OCC>
+template <int MODE, int WM, int WN, int RB, int CB, int TK, int OCC>
 __global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot(
     const uint16_t* __restrict__ A, long lda, int M, int Kb, int passes,
     const uint8_t* __restrict__ bytes, long ldbytes, int Lp, int Ncol, float* __restrict__ C,
     long ldc, int accumulate, uint32_t tiles_n, uint32_t tiles, uint32_t total) {
   const uint32_t qid = ccm_xcd_remap(blockIdx.x, total);
   if (qid >= tiles) return;
-  gemm_onehot_tile<MODE, WM, WN, RB, CB, TK, PF>(A, lda, M, Kb, passes, bytes, ldbytes, Lp, Ncol, C, ldc,
+  gemm_onehot_tile<MODE, WM, WN, RB, CB, TK>(A, lda, M, Kb, passes, bytes, ldbytes, Lp, Ncol, C, ldc,
                                                  accumulate, qid / tiles_n, qid % tiles_n);
 }
 
@@ -267,7 +281,7 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot(
 // consecutive tiles (mostly of one alignment).
 //   MODE 0: Zt = Wt . onehot(msap)      (M = P, K = P, cols = ncols)
 //   MODE 1: Gt = Dt . onehot(msapt)^T   (M = P, K = ncols, cols = P)
-template <int MODE, int WM, int WN, int RB, int CB, int TK, int PF, int OCC>
+template <int MODE, int WM, int WN, int RB, int CB, int TK, int OCC>
 __global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot_grouped(
     const ccm_bprob* __restrict__ probs, const uint32_t* __restrict__ offs,
     const int32_t* __restrict__ pidx, int np, uint32_t total) {
@@ -280,11 +294,11 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot_grouped(
   const uint32_t t = qid - offs[gi];
   if (MODE == 0) {
     const uint32_t tiles_n = (uint32_t)((B.ncols + TN - 1) / TN);
-    gemm_onehot_tile<0, WM, WN, RB, CB, TK, PF>(B.wt, B.ldwt, B.P, B.P, B.passes_fwd, B.msap, B.Lp, B.Lp,
+    gemm_onehot_tile<0, WM, WN, RB, CB, TK>(B.wt, B.ldwt, B.P, B.P, B.passes_fwd, B.msap, B.Lp, B.Lp,
                                                 B.ncols, B.zt, B.ncols, 0, t / tiles_n, t % tiles_n);
   } else {
     const uint32_t tiles_n = (uint32_t)((B.P + TN - 1) / TN);
-    gemm_onehot_tile<1, WM, WN, RB, CB, TK, PF>(B.dt, B.lddt, B.P, B.ncols, B.passes, B.msapt, B.ldmt,
+    gemm_onehot_tile<1, WM, WN, RB, CB, TK>(B.dt, B.lddt, B.P, B.ncols, B.passes, B.msapt, B.ldmt,
                                                 B.Lp, B.P, B.gt, B.P, 0, t / tiles_n, t % tiles_n);
   }
 }
@@ -306,81 +320,73 @@ int ccm_launch_convert_wt(const float* d_x, int L, int Lp, uint16_t* d_wt, long 
 
 }  // extern "C"
 
-// Tile variants: {waves M x N, row blocks x col blocks per wave, K step,
-// prefetch stages, workgroups per CU}
-//   0: 2x4, 4x2, 64, 1, 1 -> 256 x 256, 8 waves, 2 waves/SIMD
-//   1: 2x2, 4x2, 64, 1, 2 -> 256 x 128
-//   2: 2x2, 2x2, 64, 2, 2 -> 128 x 128
-//   3: 2x2, 4x4, 64, 2, 1 -> 256 x 256, 4 waves of 128 x 128, 1 wave/SIMD
-//   4: 2x4, 4x2, 128, 1, 1 -> 256 x 256, K step 128
-//   5: 2x2, 4x4, 64, 1, 1 -> as 3 with one prefetch stage
-#define GM_NVAR 6
-static const int gm_tm[GM_NVAR] = {256, 256, 128, 256, 256, 256};
-static const int gm_tn[GM_NVAR] = {256, 128, 128, 256, 256, 256};
-static const int gm_tk[GM_NVAR] = {64, 64, 64, 64, 128, 64};
+// Tile variants (the K step is not part of a variant: it follows from the
+// contraction and the shape, see gemm_tk, so that every variant sums each
+// output in the same order and gives bit-identical results):
+//   {waves M x N, 32 x 32 blocks per wave M x N, workgroups per CU (one
+//   with the K step 128, whose LDS stages take 139 KB for 256 rows)}
+//   0: 2x4, 4x2, 1 -> 256 x 256, 8 waves
+//   1: 2x2, 4x2, 2 -> 256 x 128, 4 waves
+//   2: 2x2, 2x2, 2 -> 128 x 128, 4 waves
+#define GM_VARIANTS(X) \
+  X(0, 2, 4, 4, 2, 1)  \
+  X(1, 2, 2, 4, 2, 2)  \
+  X(2, 2, 2, 2, 2, 2)
+#define GM_NVAR 3
+static const int gm_tm[GM_NVAR] = {256, 256, 128};
+static const int gm_tn[GM_NVAR] = {256, 128, 128};
+
+// K step: the forward takes 128 where the padded column count allows (its
+// one-hot bytes are reused over 21 steps, so longer steps only cut barrier
+// and loop costs); the gradient loads its bytes every step and runs best at 64
+static int gemm_tk(int mode, int Lp) { return mode == 0 && Lp % 128 == 0 ? 128 : 64; }
 
 template <int MODE>
 static int launch_gemm(int variant, const uint16_t* A, long lda, int M, int Kb, int passes,
                        const uint8_t* bytes, long ldbytes, int Lp, int Ncol, float* C, long ldc,
                        int accumulate, hipStream_t s) {
   if (variant < 0 || variant >= GM_NVAR) return CCM_ERR_ARG;
-  const int tk = gm_tk[variant];
+  const int tk = gemm_tk(MODE, Lp);
   if ((MODE == 0 && Lp % tk) || Kb % tk) return CCM_ERR_ARG;
   const uint32_t tiles_m = (M + gm_tm[variant] - 1) / gm_tm[variant];
   const uint32_t tiles_n = (Ncol + gm_tn[variant] - 1) / gm_tn[variant];
   const uint32_t tiles = tiles_m * tiles_n, total = round8(tiles);
-#define GM_ARGS A, lda, M, Kb, passes, bytes, ldbytes, Lp, Ncol, C, ldc, accumulate, tiles_n, tiles, total
-  switch (variant) {
-    case 0:
-      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 4, 4, 2, 64, 1, 1>), dim3(total), dim3(512), 0, s, GM_ARGS);
-      break;
-    case 1:
-      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 2, 4, 2, 64, 1, 2>), dim3(total), dim3(256), 0, s, GM_ARGS);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 2, 2, 2, 64, 2, 2>), dim3(total), dim3(256), 0, s, GM_ARGS);
-      break;
-    case 3:
-      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 2, 4, 4, 64, 2, 1>), dim3(total), dim3(256), 0, s, GM_ARGS);
-      break;
-    case 4:
-      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 4, 4, 2, 128, 1, 1>), dim3(total), dim3(512), 0, s, GM_ARGS);
-      break;
-    default:
-      hipLaunchKernelGGL((k_gemm_onehot<MODE, 2, 2, 4, 4, 64, 1, 1>), dim3(total), dim3(256), 0, s, GM_ARGS);
-      break;
-  }
-#undef GM_ARGS
+#define GM_LAUNCH(TK, WM, WN, RB, CB, OCC)                                                             \
+  hipLaunchKernelGGL((k_gemm_onehot<MODE, WM, WN, RB, CB, TK, OCC>), dim3(total), dim3(64 * WM * WN), 0, s, \
+                     A, lda, M, Kb, passes, bytes, ldbytes, Lp, Ncol, C, ldc, accumulate, tiles_n, tiles, total)
+#define GM_CASE(V, WM, WN, RB, CB, OCC)                 \
+  case V:                                               \
+    if (MODE == 0 && tk == 128)                         \
+      GM_LAUNCH((MODE == 0 ? 128 : 64), WM, WN, RB, CB, 1); \
+    else                                                \
+      GM_LAUNCH(64, WM, WN, RB, CB, OCC);               \
+    break;
+  switch (variant) { GM_VARIANTS(GM_CASE) }
+#undef GM_CASE
+#undef GM_LAUNCH
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }
 
+// grouped: every alignment of the group shares the K step tk
 template <int MODE>
-static int launch_gemm_grouped(int variant, const ccm_bprob* probs, const ccm_group& g, hipStream_t s) {
-  if (variant < 0 || variant >= GM_NVAR || g.np <= 0) return CCM_ERR_ARG;
+static int launch_gemm_grouped(int variant, int tk, const ccm_bprob* probs, const ccm_group& g,
+                               hipStream_t s) {
+  if (variant < 0 || variant >= GM_NVAR || g.np <= 0 || (tk != 64 && tk != 128)) return CCM_ERR_ARG;
   const uint32_t total = round8(g.total);
-#define GG_ARGS probs, g.offs, g.pidx, g.np, total
-  switch (variant) {
-    case 0:
-      hipLaunchKernelGGL((k_gemm_onehot_grouped<MODE, 2, 4, 4, 2, 64, 1, 1>), dim3(total), dim3(512), 0, s, GG_ARGS);
-      break;
-    case 1:
-      hipLaunchKernelGGL((k_gemm_onehot_grouped<MODE, 2, 2, 4, 2, 64, 1, 2>), dim3(total), dim3(256), 0, s, GG_ARGS);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_gemm_onehot_grouped<MODE, 2, 2, 2, 2, 64, 2, 2>), dim3(total), dim3(256), 0, s, GG_ARGS);
-      break;
-    case 3:
-      hipLaunchKernelGGL((k_gemm_onehot_grouped<MODE, 2, 2, 4, 4, 64, 2, 1>), dim3(total), dim3(256), 0, s, GG_ARGS);
-      break;
-    case 4:
-      hipLaunchKernelGGL((k_gemm_onehot_grouped<MODE, 2, 4, 4, 2, 128, 1, 1>), dim3(total), dim3(512), 0, s, GG_ARGS);
-      break;
-    default:
-      hipLaunchKernelGGL((k_gemm_onehot_grouped<MODE, 2, 2, 4, 4, 64, 1, 1>), dim3(total), dim3(256), 0, s, GG_ARGS);
-      break;
-  }
-#undef GG_ARGS
+#define GG_LAUNCH(TK, WM, WN, RB, CB, OCC)                                                              \
+  hipLaunchKernelGGL((k_gemm_onehot_grouped<MODE, WM, WN, RB, CB, TK, OCC>), dim3(total), dim3(64 * WM * WN), \
+                     0, s, probs, g.offs, g.pidx, g.np, total)
+#define GG_CASE(V, WM, WN, RB, CB, OCC)                 \
+  case V:                                               \
+    if (MODE == 0 && tk == 128)                         \
+      GG_LAUNCH((MODE == 0 ? 128 : 64), WM, WN, RB, CB, 1); \
+    else                                                \
+      GG_LAUNCH(64, WM, WN, RB, CB, OCC);               \
+    break;
+  switch (variant) { GM_VARIANTS(GG_CASE) }
+#undef GG_CASE
+#undef GG_LAUNCH
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }
@@ -390,11 +396,11 @@ static uint32_t gemm_tiles(int variant, int M, int Ncol) {
   return (uint32_t)(((M + gm_tm[variant] - 1) / gm_tm[variant]) * ((Ncol + gm_tn[variant] - 1) / gm_tn[variant]));
 }
 
-// default variant: the 256 x 256 tile with the longest K step the shape
-// allows while it still gives most CUs a workgroup, then smaller tiles
-static int gemm_default_variant(int M, int Ncol, int kstep_ok128) {
-  if (kstep_ok128 && gemm_tiles(4, M, Ncol) >= 192) return 4;
-  if (gemm_tiles(0, M, Ncol) >= 192) return 0;
+// default variant: the largest tile that still gives most CUs a workgroup
+// (forward: 256 x 256; gradient, measured faster with 4-wave workgroups:
+// 256 x 128), then smaller tiles
+static int gemm_default_variant(int mode, int M, int Ncol) {
+  if (mode == 0 && gemm_tiles(0, M, Ncol) >= 192) return 0;
   if (gemm_tiles(1, M, Ncol) >= 192) return 1;
   return 2;
 }
@@ -409,18 +415,17 @@ uint32_t ccm_gemm_tile_count(int variant, int M, int Ncol) {
   return gemm_tiles(variant, M, Ncol);
 }
 
-// whether a variant's K step divides the contraction: the forward needs
-// Lp % TK == 0 (mode 0), the gradient kcols % TK == 0 (mode 1)
+// whether a variant fits a contraction: the forward (mode 0) needs Lp a
+// multiple of 64, the gradient (mode 1) kcols a multiple of 64
 int ccm_gemm_variant_ok(int variant, int mode, int Lp, int kcols) {
   if (variant < 0 || variant >= GM_NVAR) return 0;
-  return mode == 0 ? Lp % gm_tk[variant] == 0 : kcols % gm_tk[variant] == 0;
+  return mode == 0 ? Lp % gemm_tk(0, Lp) == 0 : kcols % gemm_tk(1, Lp) == 0;
 }
 
 // default variant of the forward (mode 0) or gradient (mode 1) contraction
 int ccm_gemm_default_variant_for(int mode, int Lp, int ncols) {
   const int P = CCM_NA * Lp;
-  return mode == 0 ? gemm_default_variant(P, ncols, Lp % 128 == 0)
-                   : gemm_default_variant(P, P, ncols % 128 == 0);
+  return gemm_default_variant(mode, P, mode == 0 ? ncols : P);
 }
 
 int ccm_convert_wt_blocks(int Lp) {
@@ -437,15 +442,18 @@ int ccm_launch_convert_wt_grouped(const ccm_bprob* d_probs, ccm_group g, hipStre
   return CCM_OK;
 }
 
-int ccm_launch_gemm_fwd_grouped(const ccm_bprob* d_probs, ccm_group g, int variant, hipStream_t s) {
-  return launch_gemm_grouped<0>(variant, d_probs, g, s);
+// tk: the K step of every alignment of the group (ccm_gemm_tk)
+int ccm_launch_gemm_fwd_grouped(const ccm_bprob* d_probs, ccm_group g, int variant, int tk, hipStream_t s) {
+  return launch_gemm_grouped<0>(variant, tk, d_probs, g, s);
 }
 
-int ccm_launch_gemm_grad_grouped(const ccm_bprob* d_probs, ccm_group g, int variant, hipStream_t s) {
-  return launch_gemm_grouped<1>(variant, d_probs, g, s);
+int ccm_launch_gemm_grad_grouped(const ccm_bprob* d_probs, ccm_group g, int variant, int tk, hipStream_t s) {
+  return launch_gemm_grouped<1>(variant, tk, d_probs, g, s);
 }
 
-int ccm_gemm_default_variant(int M, int Ncol) { return gemm_default_variant(M, Ncol, 1); }
+int ccm_gemm_tk(int mode, int Lp) { return gemm_tk(mode, Lp); }
+
+int ccm_gemm_default_variant(int M, int Ncol) { return gemm_default_variant(0, M, Ncol); }
 
 // Zt[c][n] (P x ldc floats, ldc >= ncols) = sum_r Wt[c][r] [x_nk = b] for the
 // ncols (multiple of 128) sequences whose padded MSA rows start at d_msap;
@@ -455,7 +463,7 @@ int ccm_launch_gemm_fwd(const uint16_t* d_wt, long lda, int Lp, int passes, cons
   const int P = CCM_NA * Lp;
   if (Lp % 64 || ncols % 128 || ncols <= 0 || lda < (long)passes * P || ldc < ncols || (lda % 8))
     return CCM_ERR_ARG;
-  if (variant < 0) variant = gemm_default_variant(P, ncols, Lp % 128 == 0);
+  if (variant < 0) variant = gemm_default_variant(0, P, ncols);
   return launch_gemm<0>(variant, d_wt, lda, P, P, passes, d_msap, (long)Lp, Lp, ncols, d_zt, ldc, 0, s);
 }
 
@@ -468,7 +476,7 @@ int ccm_launch_gemm_grad(const uint16_t* d_dt, long lda, int Lp, int passes, con
   const int P = CCM_NA * Lp;
   if (Lp % 64 || kcols % 64 || kcols <= 0 || lda < (long)passes * kcols || ldc < P || (lda % 8))
     return CCM_ERR_ARG;
-  if (variant < 0) variant = gemm_default_variant(P, P, kcols % 128 == 0);
+  if (variant < 0) variant = gemm_default_variant(1, P, P);
   return launch_gemm<1>(variant, d_dt, lda, P, kcols, passes, d_msapt, ldmt, Lp, P, d_gt, ldc,
                         accumulate, s);
 }
