@@ -28,6 +28,25 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
 
 
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "pll_evaluate_L250_N5000_counters.md")
+
+
+def measured_counters(kernel_prefix):
+    """HBM bytes per launch and MFMA busy fraction of a kernel, from the
+    committed rocprofv3 counter summary (FETCH_SIZE / WRITE_SIZE scaled by a
+    1 GiB copy); None where the summary has no row for the kernel."""
+    try:
+        lines = open(COUNTERS_FILE).read().splitlines()
+    except OSError:
+        return None, None
+    for ln in lines:
+        cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0].strip("`").startswith(kernel_prefix):
+            busy = float(cells[4]) if cells[4] else None
+            return float(cells[3]), busy
+    return None, None
+
+
 def make_x(msa):
     from ccmpred_amd.pll import init_bias, nsingle_padded
     L = msa.shape[1]
@@ -213,6 +232,7 @@ def main():
     gemm = max(("gemm_fwd", "gemm_grad"), key=lambda k: bf16["step_ms"][k])
     gemm_ms = bf16["step_ms"][gemm]
     achieved = flops / (gemm_ms * 1e-3) / 1e12
+    traffic, mfma_busy = measured_counters("void k_gemm_onehot<%d," % (0 if gemm == "gemm_fwd" else 1))
     line = {
         "metric": "PLL f+grad evals/s at L=250,N=5000: BF16-MFMA vs FP32 GPU vs host CPU",
         "value": round(bf16["evals_per_s"], 2),
@@ -231,8 +251,10 @@ def main():
                    "lambda_pair": 0.2 * (L - 1), "x": "init_bias + symmetric N(0,0.01^2) couplings"},
         "roofline": {"bound": "mfma", "kernel": gemm, "achieved": round(achieved, 2),
                      "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
-                     "algorithmic_flops": flops, "avg_ms": round(gemm_ms, 4)},
+                     "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": traffic,
+                     "algorithmic_flops": flops, "avg_ms": round(gemm_ms, 4),
+                     "mfma_busy": mfma_busy,
+                     "counters": os.path.relpath(COUNTERS_FILE, ROOT) if traffic else None},
         "fp32_evals_per_s": round(fp32["evals_per_s"], 2),
         "bf16_over_fp32": round(bf16["evals_per_s"] / fp32["evals_per_s"], 3),
         "bf16_step_ms": {k: round(v, 4) for k, v in bf16["step_ms"].items()},

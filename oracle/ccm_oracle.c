@@ -34,6 +34,111 @@
 #undef EXPR
 #undef LOGR
 
+/*
+ * FP64 evaluator for large alignments, OpenMP-parallel: the model of
+ * evaluate_cpu.c:16-166 (same logits, log-partition, objective, node and
+ * pair gradient, symmetrisation, self-edge zeroing and L2 terms) with the
+ * work split the way evaluate_cpu_omp.c:18-189 splits it -- per-sequence
+ * logits in parallel over sequences, the raw pair-gradient rows in parallel
+ * over the coupling column k -- over blocks of 256 sequences.  Only the
+ * FP64 summation order differs from ora_evaluate_f64, which it matches to
+ * rounding; it exists so that FP64 reference values at L = 600, N = 20000
+ * take seconds on a multi-core host.
+ */
+double ora_evaluate_f64_par(const unsigned char *msa, const double *weights, int ncol, int nrow,
+                            double lambda_single, double lambda_pair, const double *x, double *g)
+{
+	enum { BLK = 256 };
+	const long L = ncol;
+	const long C = ORA_NA * L;
+	const long nsingle = L * (ORA_NA - 1);
+	const long nsp = ora_nsingle_padded(ncol);
+	const long nvp = ora_nvar_padded(ncol);
+	const double *x1 = x;
+	const double *x2 = x + nsp;
+	double *g1 = g;
+	double *g2 = g + nsp;
+	double *raw = (double *)calloc((size_t)(nvp - nsp), sizeof(double));
+	double *D = (double *)malloc(sizeof(double) * BLK * (size_t)C);
+	double f = 0;
+	memset(g, 0, sizeof(double) * (size_t)nvp);
+
+	for (long n0 = 0; n0 < nrow; n0 += BLK) {
+		const long nb = nrow - n0 < BLK ? nrow - n0 : BLK;
+		double fb = 0;
+		/* D[i][c] = w_n (P_n(c) - [x_nj = a]), c = a*L + j */
+		#pragma omp parallel for reduction(+ : fb) schedule(dynamic, 4)
+		for (long i = 0; i < nb; i++) {
+			const long n = n0 + i;
+			const double wn = weights[n];
+			const unsigned char *seq = msa + n * L;
+			double *lg = D + i * C;
+			for (long a = 0; a < ORA_NA - 1; a++)
+				for (long s = 0; s < L; s++)
+					lg[a * L + s] = x1[a * L + s];
+			for (long s = 0; s < L; s++)
+				lg[(ORA_NA - 1) * L + s] = 0;
+			for (long k = 0; k < L; k++) {
+				const double *row = x2 + ORA_X2(seq[k], k, 0, 0, L);
+				for (long c = 0; c < C; c++)
+					lg[c] += row[c];
+			}
+			for (long s = 0; s < L; s++) {
+				double z = 0;
+				for (long a = 0; a < ORA_NA; a++)
+					z += exp(lg[a * L + s]);
+				const double lse = log(z);
+				fb += wn * (lse - lg[seq[s] * L + s]);
+				for (long a = 0; a < ORA_NA; a++)
+					lg[a * L + s] = wn * exp(lg[a * L + s] - lse);
+				lg[seq[s] * L + s] -= wn;
+			}
+		}
+		f += fb;
+		/* node gradient: column sums over the block, a < 20 */
+		#pragma omp parallel for
+		for (long c = 0; c < nsingle; c++) {
+			double t = 0;
+			for (long i = 0; i < nb; i++)
+				t += D[i * C + c];
+			g1[c] += t;
+		}
+		/* raw pair gradient rows (x_nk, k) */
+		#pragma omp parallel for schedule(dynamic, 1)
+		for (long k = 0; k < L; k++)
+			for (long i = 0; i < nb; i++) {
+				double *rg = raw + ORA_X2(msa[(n0 + i) * L + k], k, 0, 0, L);
+				const double *d = D + i * C;
+				for (long c = 0; c < C; c++)
+					rg[c] += d[c];
+			}
+	}
+
+	/* g2 = raw + raw^T, zero self-edge blocks */
+	#pragma omp parallel for
+	for (long b = 0; b < ORA_NA; b++)
+		for (long k = 0; k < L; k++)
+			for (long a = 0; a < ORA_NA; a++)
+				for (long j = 0; j < L; j++)
+					g2[ORA_X2(b, k, a, j, L)] =
+					    k == j ? 0.0 : raw[ORA_X2(b, k, a, j, L)] + raw[ORA_X2(a, j, b, k, L)];
+
+	/* L2: real singles, then the whole padded pair block */
+	double reg = 0;
+	for (long v = 0; v < nsingle; v++) {
+		reg += lambda_single * x[v] * x[v];
+		g[v] += 2 * lambda_single * x[v];
+	}
+	#pragma omp parallel for reduction(+ : reg)
+	for (long v = nsp; v < nvp; v++) {
+		reg += 0.5 * lambda_pair * x[v] * x[v];
+		g[v] += 2 * lambda_pair * x[v];
+	}
+	free(raw);
+	free(D);
+	return f + reg;
+}
+
 /* ---- reweighting (reweighting.c:9-67) ---------------------------------- */
 
 /* reweighting.c:10: threshold (float) times ncol is a float product; its

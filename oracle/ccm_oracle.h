@@ -69,6 +69,10 @@ float ora_evaluate_f32(const unsigned char *msa, const float *w, int ncol, int n
 double ora_evaluate_f64(const unsigned char *msa, const double *w, int ncol, int nrow,
                         double lambda_single, double lambda_pair, const double *x,
                         double *g);
+/* FP64 evaluator parallel over sequence blocks and coupling columns (same
+ * model as ora_evaluate_f64, summation order differs only in rounding). */
+double ora_evaluate_f64_par(const unsigned char *msa, const double *weights, int ncol, int nrow,
+                            double lambda_single, double lambda_pair, const double *x, double *g);
 void ora_forward_lse_f32(const unsigned char *msa, int ncol, int nrow, const float *x,
                          float *lse);
 void ora_forward_lse_f64(const unsigned char *msa, int ncol, int nrow, const double *x,

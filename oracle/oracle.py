@@ -56,6 +56,9 @@ def _declare(L):
     L.ora_evaluate_f32.restype = C.c_float
     L.ora_evaluate_f64.argtypes = [_u8p, _f64p, C.c_int, C.c_int, C.c_double, C.c_double, _f64p, _f64p]
     L.ora_evaluate_f64.restype = C.c_double
+    L.ora_evaluate_f64_par.argtypes = [_u8p, _f64p, C.c_int, C.c_int, C.c_double, C.c_double, _f64p,
+                                       _f64p]
+    L.ora_evaluate_f64_par.restype = C.c_double
     L.ora_forward_lse_f32.argtypes = [_u8p, C.c_int, C.c_int, _f32p, _f32p]
     L.ora_forward_lse_f64.argtypes = [_u8p, C.c_int, C.c_int, _f64p, _f64p]
     L.ora_omp_create.argtypes = [_u8p, _f32p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int]
@@ -131,6 +134,16 @@ def evaluate_f64(msa, w, x, lambda_single, lambda_pair):
     g = np.zeros_like(x)
     f = lib().ora_evaluate_f64(msa, np.ascontiguousarray(w, dtype=np.float64), msa.shape[1],
                                msa.shape[0], float(lambda_single), float(lambda_pair), x, g)
+    return float(f), g
+
+
+def evaluate_f64_par(msa, w, x, lambda_single, lambda_pair):
+    """FP64 evaluator parallel over sequence blocks (OpenMP): for large sizes."""
+    msa = _msa(msa)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    g = np.zeros_like(x)
+    f = lib().ora_evaluate_f64_par(msa, np.ascontiguousarray(w, dtype=np.float64), msa.shape[1],
+                                   msa.shape[0], float(lambda_single), float(lambda_pair), x, g)
     return float(f), g
 
 

@@ -1,0 +1,144 @@
+"""Precision forms at L=250, N=5000 and at L=600, N=20000.
+
+For each size: one evaluation of every precision form at the same x
+(column log-frequencies + symmetric N(0, 0.01^2) couplings) against the FP64
+oracle (the OpenMP FP64 restatement, checked against the serial one in
+tests/test_oracle_pll.py); then 50 CG iterations of the BF16 form used for
+optimisation (bf16f) against the plain FP32 GPU path, compared as APC
+contact maps (Spearman over i < j, top-L/5 pairs at separation >= 6).  After
+50 iterations neither run has converged and the two trajectories differ, so a
+pair whose score sits at the top-L/5 cut-off may swap: the test allows at most
+one swap per 100 pairs, each within 2 % of the cut-off score in both maps
+(measured at L=600: one of 120, identical sets at L=250).
+An FP64-evaluated host CG of this size would take hours, so the FP32 path,
+whose gradient is pinned to the FP64 oracle at 1e-4 here, is the reference
+trajectory; at L=75 the CG is compared with the FP64-evaluated host CG in
+tests/test_gpu_parity.py.
+
+Tolerances (gradient relative L2 against FP64): fp32 1e-4 (and f 1e-5);
+bf16f and bf16 at L=250 1e-3; bf16x2 1e-4.  Single-pass bf16 at L=600,
+N=20000 measures 1.3e-3 (its forward logits sum 600 BF16-rounded couplings;
+the two-pass forward of bf16f brings the gradient to 3e-4), so it is held to
+2e-3 there.
+"""
+import time
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as ora  # noqa: E402
+from ccmpred_amd import pll as P  # noqa: E402
+from ccmpred_amd.msa import synthetic_msa  # noqa: E402
+from tests.pll_bruteforce import nsingle_padded, NA, NAP  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def start_x(msa, seed):
+    L = msa.shape[1]
+    x = ora.init_bias(msa).astype(np.float64)
+    nsp = nsingle_padded(L)
+    rng = np.random.default_rng(seed)
+    W = rng.normal(0, 0.01, (NA, L, NA, L))
+    W = W + W.transpose(2, 3, 0, 1)
+    W[:, np.arange(L), :, np.arange(L)] = 0
+    x[nsp:].reshape(NA, L, NAP, L)[:, :, :NA, :] += W
+    return x.astype(np.float32)
+
+
+def spearman(a, b):
+    iu = np.triu_indices(a.shape[0], k=1)
+    ra = np.argsort(np.argsort(a[iu])).astype(np.float64)
+    rb = np.argsort(np.argsort(b[iu])).astype(np.float64)
+    return float(np.corrcoef(ra, rb)[0, 1])
+
+
+def grad_tol(prec, L):
+    return {"fp32": 1e-4, "bf16": 1e-3 if L <= 250 else 2e-3, "bf16f": 1e-3, "bf16x2": 1e-4}[prec]
+
+
+@pytest.mark.parametrize("L,N,seed", [(250, 5000, 2), (600, 20000, 4)])
+def test_precision_forms_against_fp64(L, N, seed):
+    msa, _ = synthetic_msa(L, N, seed, 60)
+    x = start_x(msa, seed + 50)
+    say(f"precision report L={L} N={N}")
+    with P.PLL(msa, precision="fp32") as p32:
+        w = p32.weights.copy()
+        lam1, lam2 = p32.lambda_single, p32.lambda_pair
+    t0 = time.perf_counter()
+    f64, g64 = ora.evaluate_f64_par(msa, w.astype(np.float64), x.astype(np.float64), lam1, lam2)
+    say(f"  FP64 oracle f={f64:.10g} ({time.perf_counter() - t0:.1f} s)")
+    nsp = nsingle_padded(L)
+    xd = torch.from_numpy(x).cuda()
+    for prec in ("fp32", "bf16", "bf16f", "bf16x2"):
+        tol = grad_tol(prec, L)
+        with P.PLL(msa, precision=prec) as p:
+            assert np.array_equal(p.weights, w)
+            f, g = p.evaluate(xd)
+            g = g.cpu().numpy().astype(np.float64)
+        ef = abs(f - f64) / abs(f64)
+        eg = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+        eg1 = np.linalg.norm(g[:nsp] - g64[:nsp]) / np.linalg.norm(g64[:nsp])
+        eg2 = np.linalg.norm(g[nsp:] - g64[nsp:]) / np.linalg.norm(g64[nsp:])
+        say(f"  {prec:7s} f rel err {ef:.2e}  g rel L2 {eg:.2e} (singles {eg1:.2e}, pairs {eg2:.2e})")
+        assert eg <= tol
+        if prec == "fp32":
+            assert ef <= 1e-5
+        else:
+            assert ef <= 1e-4
+
+
+@pytest.mark.parametrize("L,N,seed", [(250, 5000, 2), (600, 20000, 4)])
+def test_bf16_cg_contacts_against_fp32(L, N, seed):
+    msa, pairs = synthetic_msa(L, N, seed, 60)
+    x0 = ora.init_bias(msa)
+    say(f"CG 50 iterations L={L} N={N}: bf16f vs fp32")
+    maps, res = {}, {}
+    for prec in ("fp32", "bf16f"):
+        with P.PLL(msa, precision=prec) as p:
+            xd = torch.from_numpy(x0.copy()).cuda()
+            t0 = time.perf_counter()
+            r = p.conjugrad(xd, max_iterations=50, epsilon=0.0)
+            dt = time.perf_counter() - t0
+            maps[prec] = P.contact_map(xd.cpu().numpy(), L)
+            res[prec] = r
+        say(f"  {prec:6s} ret={r['ret']} it={r['iterations']} evals={r['evaluations']} f={r['fx']:.8g} "
+            f"({dt:.1f} s)")
+    rho = spearman(maps["bf16f"], maps["fp32"])
+    k = L // 5
+    top_b = set(P.top_pairs(maps["bf16f"], k, 6))
+    top_f = set(P.top_pairs(maps["fp32"], k, 6))
+    top7_b = set(P.top_pairs(maps["bf16f"], k, 7))
+    top7_f = set(P.top_pairs(maps["fp32"], k, 7))
+    planted = set(map(tuple, pairs.tolist()))
+    say(f"  APC spearman={rho:.6f}  top-L/5 identical={top_b == top_f} ({len(top_b & top_f)}/{k})  "
+        f"min_sep 7 identical={top7_b == top7_f}  planted in top-L/5: bf16f {len(top_b & planted)} "
+        f"fp32 {len(top_f & planted)}")
+    rel_f = abs(res["bf16f"]["fx"] - res["fp32"]["fx"]) / abs(res["fp32"]["fx"])
+    say(f"  final f rel diff {rel_f:.2e}")
+    assert res["fp32"]["iterations"] == 50
+    assert res["bf16f"]["iterations"] == 50
+    assert rho >= 0.999
+    assert len(top_b & planted) == len(top_f & planted)
+    # identical top-L/5, up to pairs that sit at the cut-off: a swapped pair
+    # must score within 2 % of the L/5-th score in both maps
+    for name, m in (("bf16f", maps["bf16f"]), ("fp32", maps["fp32"])):
+        iu = np.triu_indices(L, k=6)
+        cut = np.sort(m[iu])[::-1][k - 1]
+        for pr in sorted(top_b ^ top_f):
+            rel = abs(m[pr] - cut) / cut
+            say(f"  swapped {pr}: {name} score {m[pr]:.5g} vs cut-off {cut:.5g} (rel {rel:.2e})")
+            assert rel < 0.02
+    assert len(top_b ^ top_f) <= 2 * max(1, k // 100)

@@ -177,3 +177,22 @@ def test_fast_tied_fd_matches_full_reevaluation():
     np.testing.assert_allclose(fast, slow, rtol=1e-6, atol=1e-6)
     f, g = ora.evaluate_f64(msa, w, x, lam1, lam2)
     np.testing.assert_allclose(fast[:40], g[ia[:40]], rtol=2e-6, atol=2e-6)
+
+
+def test_parallel_f64_matches_serial():
+    """The OpenMP FP64 evaluator (blocked over sequences, used for large-size
+    error reports) agrees with the serial FP64 restatement to rounding."""
+    msa, _ = synthetic_msa(23, 600, 31, 3)
+    w = ora.calculate_weights(msa, 0.8).astype(np.float64)
+    rng = np.random.default_rng(3)
+    x = ora.init_bias(msa).astype(np.float64)
+    L = msa.shape[1]
+    nsp = nsingle_padded(L)
+    W = rng.normal(0, 0.05, (NA, L, NA, L))
+    W = W + W.transpose(2, 3, 0, 1)
+    W[:, np.arange(L), :, np.arange(L)] = 0
+    x[nsp:].reshape(NA, L, NAP, L)[:, :, :NA, :] += W
+    f1, g1 = ora.evaluate_f64(msa, w, x, 0.01, 0.2 * (L - 1))
+    f2, g2 = ora.evaluate_f64_par(msa, w, x, 0.01, 0.2 * (L - 1))
+    assert abs(f1 - f2) <= 1e-12 * abs(f1)
+    assert np.max(np.abs(g1 - g2)) <= 1e-12 * np.max(np.abs(g1))
