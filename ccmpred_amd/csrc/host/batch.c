@@ -37,11 +37,11 @@
 
 #include "state.h"
 
-#define NVAR_MAX 16 /* GEMM launches (tile variant x K step) per contraction */
+#define NVAR_MAX 16 /* GEMM launches (one per tile variant) per contraction */
 
 typedef struct {
 	ccm_group g;
-	int variant, tk; /* GEMM groups only */
+	int variant; /* GEMM groups only */
 } bgroup;
 
 struct ccm_batch {
@@ -188,24 +188,22 @@ static int build_tables(ccm_batch *b)
 	    add_group(&t, &b->g_vec, b->h_probs, all, b->n, K_VEC, 0, 0))
 		goto out;
 	b->n_fwd = b->n_grad = 0;
-	/* one GEMM launch per (tile variant, K step) in use */
+	/* one GEMM launch per tile variant in use */
 	for (int mode = 0; mode < 2; mode++) {
 		const int *var = mode == 0 ? b->var_fwd : b->var_grad;
-		for (int v = 0; v < ccm_gemm_variants(); v++)
-			for (int tk = 64; tk <= 128; tk *= 2) {
-				int np = 0;
-				for (int i = 0; i < b->n; i++)
-					if (var[i] == v && ccm_gemm_tk(mode, b->h_probs[i].Lp) == tk)
-						list[np++] = i;
-				if (!np)
-					continue;
-				if ((mode == 0 ? b->n_fwd : b->n_grad) >= NVAR_MAX)
-					goto out;
-				bgroup *bg = mode == 0 ? &b->g_fwd[b->n_fwd++] : &b->g_grad[b->n_grad++];
-				if (add_group(&t, bg, b->h_probs, list, np, -1, mode, v))
-					goto out;
-				bg->tk = tk;
-			}
+		for (int v = 0; v < ccm_gemm_variants(); v++) {
+			int np = 0;
+			for (int i = 0; i < b->n; i++)
+				if (var[i] == v)
+					list[np++] = i;
+			if (!np)
+				continue;
+			if ((mode == 0 ? b->n_fwd : b->n_grad) >= NVAR_MAX)
+				goto out;
+			bgroup *bg = mode == 0 ? &b->g_fwd[b->n_fwd++] : &b->g_grad[b->n_grad++];
+			if (add_group(&t, bg, b->h_probs, list, np, -1, mode, v))
+				goto out;
+		}
 	}
 	if (b->d_tab && b->tab_len < t.len) {
 		(void)hipFree(b->d_tab);
@@ -459,9 +457,8 @@ static int enqueue_gemms(ccm_batch *b, int mode)
 	const int ng = mode == 0 ? b->n_fwd : b->n_grad;
 	const bgroup *gs = mode == 0 ? b->g_fwd : b->g_grad;
 	for (int i = 0; i < ng; i++) {
-		int r = mode == 0
-		            ? ccm_launch_gemm_fwd_grouped(b->d_probs, gs[i].g, gs[i].variant, gs[i].tk, b->s)
-		            : ccm_launch_gemm_grad_grouped(b->d_probs, gs[i].g, gs[i].variant, gs[i].tk, b->s);
+		int r = mode == 0 ? ccm_launch_gemm_fwd_grouped(b->d_probs, gs[i].g, gs[i].variant, b->s)
+		                  : ccm_launch_gemm_grad_grouped(b->d_probs, gs[i].g, gs[i].variant, b->s);
 		if (r)
 			return r;
 	}

@@ -265,6 +265,274 @@ __device__ __forceinline__ void gemm_onehot_tile(const uint16_t* __restrict__ A,
   }
 }
 
+// ---- the same contraction with direct-to-LDS staging ----------------------
+//
+// The dense operand (and, for the gradient, the one-hot bytes) go from global
+// memory straight into LDS with global_load_lds_dwordx4: each wave
+// instruction fills 1 KB of LDS, lane i taking the 16 bytes at offset 16 i.
+// S stages are in flight, so a step's loads are issued S-1 steps before they
+// are used and no registers hold them.  LDS rows are unpadded; bank conflicts
+// are avoided by an XOR swizzle of the 16-byte granules of a row (the loading
+// lane fetches the granule that belongs at its position, the reading lane
+// looks it up through the same XOR).  Waits are explicit vmcnt counts and the
+// step barrier carries no memory fence, so later stages stay in flight across
+// it.  Same K order as gemm_onehot_tile: the results are bit-identical.
+
+// s_waitcnt with only the vector-memory count limited to N (gfx9 encoding:
+// vmcnt bits [3:0] and [15:14], expcnt [6:4] and lgkmcnt [11:8] at maximum)
+template <int N>
+__device__ __forceinline__ void ccm_wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// workgroup barrier for LDS hand-over: wait for this wave's LDS operations,
+// then s_barrier; the memory clobber keeps the compiler from moving LDS
+// accesses across it, and no vector-memory wait is implied
+__device__ __forceinline__ void ccm_lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// physical granule of logical granule g in row r of a dense stage with G
+// granules per row (G = 8: rows of 128 bytes; G = 16: rows of 256 bytes)
+template <int G>
+__device__ __forceinline__ int swz_a(int r, int g) {
+  return G == 8 ? (g ^ ((r >> 1) & 7)) : (g ^ (r & 15));
+}
+// physical granule of logical granule g of column c of a byte stage with GB
+// granules per column (GB = 4: 64 bytes per column)
+template <int GB>
+__device__ __forceinline__ int swz_b(int c, int g) {
+  return g ^ ((c >> 2) & (GB - 1));
+}
+
+// LDS stages of the direct-to-LDS kernel: one __shared__ object per stage
+// (dense tile, then the byte tile), so that the compiler can tell a stage
+// being read from the stages being filled and does not drain all
+// outstanding loads before every LDS read
+template <int BYTES, int J>
+__device__ __forceinline__ uint8_t* dtl_stage() {
+  __shared__ __attribute__((aligned(1024))) uint8_t s[BYTES];
+  return s;
+}
+
+template <int MODE, int WM, int WN, int RB, int CB, int TK, int S>
+__device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict__ A, long lda, int M,
+                                                     int Kb, int passes,
+                                                     const uint8_t* __restrict__ bytes, long ldbytes,
+                                                     int Lp, int Ncol, float* __restrict__ C, long ldc,
+                                                     int accumulate, int mt, int nt) {
+  constexpr int NW = WM * WN;
+  constexpr int TM = WM * RB * 32, TN = WN * CB * 32;
+  constexpr int G = TK / 8;                   // 16-byte granules per dense row
+  constexpr int A_INS = TM * G / (64 * NW);   // DTL instructions per wave and stage
+  constexpr int GB = TK / 16;                 // 16-byte granules per byte column
+  constexpr int B_INS = MODE == 1 ? TN * GB / (64 * NW) : 0;
+  constexpr int TNB = MODE == 1 ? TN * TK : 0;  // bytes of a byte stage
+  constexpr int HALF = TK / 2, NB16 = HALF / 16, SUBS = TK / 16;
+  constexpr int PER_STEP = A_INS + B_INS;     // vector-memory instructions per step
+  static_assert((TM * G) % (64 * NW) == 0, "dense loader mapping");
+  static_assert(MODE == 0 || (TN * GB) % (64 * NW) == 0, "byte loader mapping");
+  static_assert(MODE == 0 || TK == 64, "gradient K step");
+  static_assert(S == 3, "stages");
+  static_assert((S - 2) * PER_STEP < 64, "vmcnt range");
+
+  const int m0 = mt * TM, n0 = nt * TN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+  const int l32 = lane & 31, q = lane >> 5;
+  // the LDS operand of a DTL load must be wave-uniform
+  const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+
+  // per lane and DTL instruction: source row pointer with its granule
+  const uint16_t* asrc[A_INS];
+#pragma unroll
+  for (int i = 0; i < A_INS; i++) {
+    const int p = (wave * A_INS + i) * 64 + lane;
+    const int r = p / G, g = swz_a<G>(r, p % G);
+    const int row = m0 + r < M ? m0 + r : M - 1;  // rows past M are read but never stored
+    asrc[i] = A + (long)row * lda + g * 8;
+  }
+  const uint8_t* bsrc[B_INS > 0 ? B_INS : 1];
+#pragma unroll
+  for (int i = 0; i < B_INS; i++) {
+    const int p = (wave * B_INS + i) * 64 + lane;
+    const int c = p / GB, g = swz_b<GB>(c, p % GB);
+    const int kk = (n0 + c) % Lp;  // msapt row of the column; bb >= 21 past the real rows
+    bsrc[i] = bytes + (long)kk * ldbytes + g * 16;
+  }
+
+  // per lane and column block: the one-hot selector
+  int colv[CB];
+  uint32_t bval[CB];
+  const uint8_t* bptr[CB];
+#pragma unroll
+  for (int cb = 0; cb < CB; cb++) {
+    const int col = n0 + (wn * CB + cb) * 32 + l32;
+    colv[cb] = col;
+    if (MODE == 0) {
+      bval[cb] = 0;
+      bptr[cb] = bytes + (long)(col < Ncol ? col : Ncol - 1) * ldbytes + q * HALF;
+    } else {
+      bval[cb] = (uint32_t)(col / Lp);
+      bptr[cb] = nullptr;
+    }
+  }
+
+  floatx16 acc[RB][CB];
+#pragma unroll
+  for (int i = 0; i < RB; i++)
+#pragma unroll
+    for (int j = 0; j < CB; j++) acc[i][j] = (floatx16)0.0f;
+
+  const int kblocks = (MODE == 0 ? Lp : Kb) / TK;
+  const int inner = MODE == 0 ? passes * CCM_NA : 1;
+  const int nsteps = MODE == 0 ? kblocks * inner : passes * kblocks;
+  auto acol = [&](int st) -> long {
+    if (MODE == 0) {
+      const int kc = st / inner, r = st - kc * inner, pass = r / CCM_NA, bb = r - pass * CCM_NA;
+      return (long)pass * Kb + (long)bb * Lp + (long)kc * TK;
+    } else {
+      const int pass = st / kblocks, kc = st - pass * kblocks;
+      return (long)pass * Kb + (long)kc * TK;
+    }
+  };
+  constexpr int A_BYTES = TM * TK * 2;
+  // DTL loads of step st into the stage at sp (dense tile, then bytes)
+  auto issue = [&](uint8_t* __restrict__ sp, int st) {
+    uint16_t* as = reinterpret_cast<uint16_t*>(sp);
+    uint8_t* bs = sp + A_BYTES;
+    const long c = acol(st);
+#pragma unroll
+    for (int i = 0; i < A_INS; i++)
+      __builtin_amdgcn_global_load_lds((const void*)(asrc[i] + c),
+                                       (__attribute__((address_space(3))) void*)(as + (wave_u * A_INS + i) * 512),
+                                       16, 0, 0);
+    if constexpr (MODE == 1) {
+      const long nb = (long)(st % kblocks) * TK;
+#pragma unroll
+      for (int i = 0; i < B_INS; i++)
+        __builtin_amdgcn_global_load_lds((const void*)(bsrc[i] + nb),
+                                         (__attribute__((address_space(3))) void*)(bs + (wave_u * B_INS + i) * 1024),
+                                         16, 0, 0);
+    }
+  };
+  auto loadb = [&](uint4(&bv)[CB][NB16], int kc) {
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+      for (int i = 0; i < NB16; i++)
+        bv[cb][i] = *reinterpret_cast<const uint4*>(bptr[cb] + (long)kc * TK + 16 * i);
+  };
+  uint4 bcur[CB][MODE == 0 ? NB16 : 1];
+  auto compute = [&](const uint8_t* __restrict__ sp, uint32_t bsel) {
+    const uint16_t* as = reinterpret_cast<const uint16_t*>(sp);
+    const uint8_t* bs = sp + A_BYTES;
+#pragma unroll
+    for (int sub = 0; sub < SUBS; sub++) {
+      bf16x8_t bfrag[CB];
+#pragma unroll
+      for (int cb = 0; cb < CB; cb++) {
+        uint4 w;
+        if constexpr (MODE == 0) {
+          w = bcur[cb][sub >> 1];
+        } else {
+          const int c = (wn * CB + cb) * 32 + l32;
+          const int g = q * (GB / 2) + (sub >> 1);
+          w = *reinterpret_cast<const uint4*>(&bs[c * TK + swz_b<GB>(c, g) * 16]);
+        }
+        const uint2 v = (sub & 1) ? make_uint2(w.z, w.w) : make_uint2(w.x, w.y);
+        bfrag[cb] = onehot8(v, MODE == 0 ? bsel : bval[cb]);
+      }
+#pragma unroll
+      for (int rb = 0; rb < RB; rb++) {
+        const int r = (wm * RB + rb) * 32 + l32;
+        const int g = q * SUBS + sub;
+        const uint4 av = *reinterpret_cast<const uint4*>(&as[r * TK + swz_a<G>(r, g) * 8]);
+        const bf16x8_t afrag = __builtin_bit_cast(bf16x8_t, av);
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++)
+          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afrag, bfrag[cb], acc[rb][cb], 0, 0, 0);
+      }
+    }
+  };
+  // one step: wait for its loads, hand the stage over, refill the stage
+  // consumed by the previous step, compute.  The stages are distinct
+  // restrict-qualified pointers, which tells the compiler that the reads of
+  // the current stage do not depend on the loads filling the next one.
+  auto step = [&](const uint8_t* __restrict__ cur, uint8_t* __restrict__ nxt, int st) {
+    if (st + S - 2 < nsteps)
+      ccm_wait_vmcnt<(S - 2) * PER_STEP>();
+    else
+      ccm_wait_vmcnt<0>();
+    ccm_lds_barrier();
+    if (st + S - 1 < nsteps) issue(nxt, st + S - 1);
+    uint32_t bsel = 0;
+    if constexpr (MODE == 0) {
+      const int r = st % inner;
+      bsel = (uint32_t)(r - (r / CCM_NA) * CCM_NA);
+    }
+    compute(cur, bsel);
+  };
+
+  constexpr int STAGE_BYTES = A_BYTES + TNB;
+  uint8_t* s0 = dtl_stage<STAGE_BYTES, 0>();
+  uint8_t* s1 = dtl_stage<STAGE_BYTES, 1>();
+  uint8_t* s2 = dtl_stage<STAGE_BYTES, 2>();
+  issue(s0, 0);
+  if (nsteps > 1) issue(s1, 1);
+  if constexpr (MODE == 0) loadb(bcur, 0);
+  // the prologue's loads are complete before the loop, so that the only
+  // loads in flight inside it are those the steps issue
+  ccm_wait_vmcnt<0>();
+  if constexpr (MODE == 0) {
+    // 21 * passes steps per K block, a multiple of the three stages; the
+    // next block's bytes are loaded at the end of each block (the last
+    // block reloads its own)
+    for (int kc = 0; kc < kblocks; kc++) {
+      for (int r = 0; r < inner; r += 3) {
+        const int st = kc * inner + r;
+        step(s0, s2, st);
+        step(s1, s0, st + 1);
+        step(s2, s1, st + 2);
+      }
+      loadb(bcur, kc + 1 < kblocks ? kc + 1 : kc);
+    }
+  } else {
+    for (int st = 0; st < nsteps; st += 3) {
+      step(s0, s2, st);
+      if (st + 1 < nsteps) step(s1, s0, st + 1);
+      if (st + 2 < nsteps) step(s2, s1, st + 2);
+    }
+  }
+
+#pragma unroll
+  for (int rb = 0; rb < RB; rb++) {
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++) {
+      const int col = colv[cb];
+      if (col >= Ncol) continue;
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int row = m0 + (wm * RB + rb) * 32 + (i >> 2) * 8 + 4 * q + (i & 3);
+        if (row < M) {
+          float* dst = C + (long)row * ldc + col;
+          *dst = accumulate ? *dst + acc[rb][cb][i] : acc[rb][cb][i];
+        }
+      }
+    }
+  }
+}
+
+template <int MODE, int WM, int WN, int RB, int CB, int TK, int S, int OCC>
+__global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot_dtl(
+    const uint16_t* __restrict__ A, long lda, int M, int Kb, int passes,
+    const uint8_t* __restrict__ bytes, long ldbytes, int Lp, int Ncol, float* __restrict__ C,
+    long ldc, int accumulate, uint32_t tiles_n, uint32_t tiles, uint32_t total) {
+  const uint32_t qid = ccm_xcd_remap(blockIdx.x, total);
+  if (qid >= tiles) return;
+  gemm_onehot_tile_dtl<MODE, WM, WN, RB, CB, TK, S>(A, lda, M, Kb, passes, bytes, ldbytes, Lp, Ncol, C,
+                                                    ldc, accumulate, qid / tiles_n, qid % tiles_n);
+}
+
 template <int MODE, int WM, int WN, int RB, int CB, int TK, int OCC>
 __global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot(
     const uint16_t* __restrict__ A, long lda, int M, int Kb, int passes,
@@ -303,6 +571,28 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot_grouped(
   }
 }
 
+template <int MODE, int WM, int WN, int RB, int CB, int TK, int S, int OCC>
+__global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot_dtl_grouped(
+    const ccm_bprob* __restrict__ probs, const uint32_t* __restrict__ offs,
+    const int32_t* __restrict__ pidx, int np, uint32_t total) {
+  constexpr int TN = WN * CB * 32;
+  const uint32_t qid = ccm_xcd_remap(blockIdx.x, total);
+  if (qid >= offs[np]) return;
+  const int gi = ccm_group_find(offs, np, qid);
+  const ccm_bprob& B = probs[pidx[gi]];
+  if (!B.ctl->eval_on) return;
+  const uint32_t t = qid - offs[gi];
+  if constexpr (MODE == 0) {
+    const uint32_t tiles_n = (uint32_t)((B.ncols + TN - 1) / TN);
+    gemm_onehot_tile_dtl<0, WM, WN, RB, CB, TK, S>(B.wt, B.ldwt, B.P, B.P, B.passes_fwd, B.msap, B.Lp, B.Lp,
+                                                   B.ncols, B.zt, B.ncols, 0, t / tiles_n, t % tiles_n);
+  } else {
+    const uint32_t tiles_n = (uint32_t)((B.P + TN - 1) / TN);
+    gemm_onehot_tile_dtl<1, WM, WN, RB, CB, TK, S>(B.dt, B.lddt, B.P, B.ncols, B.passes, B.msapt, B.ldmt,
+                                                   B.Lp, B.P, B.gt, B.P, 0, t / tiles_n, t % tiles_n);
+  }
+}
+
 static inline uint32_t round8(uint32_t v) { return (v + 7u) & ~7u; }
 
 extern "C" {
@@ -320,73 +610,94 @@ int ccm_launch_convert_wt(const float* d_x, int L, int Lp, uint16_t* d_wt, long 
 
 }  // extern "C"
 
-// Tile variants (the K step is not part of a variant: it follows from the
-// contraction and the shape, see gemm_tk, so that every variant sums each
-// output in the same order and gives bit-identical results):
-//   {waves M x N, 32 x 32 blocks per wave M x N, workgroups per CU (one
-//   with the K step 128, whose LDS stages take 139 KB for 256 rows)}
-//   0: 2x4, 4x2, 1 -> 256 x 256, 8 waves
-//   1: 2x2, 4x2, 2 -> 256 x 128, 4 waves
-//   2: 2x2, 2x2, 2 -> 128 x 128, 4 waves
-#define GM_VARIANTS(X) \
-  X(0, 2, 4, 4, 2, 1)  \
-  X(1, 2, 2, 4, 2, 2)  \
-  X(2, 2, 2, 2, 2, 2)
-#define GM_NVAR 3
-static const int gm_tm[GM_NVAR] = {256, 256, 128};
-static const int gm_tn[GM_NVAR] = {256, 128, 128};
+// Tile variants.  Every variant runs the contraction in K steps of 64 in
+// the same order (for the forward: blocks of 64 coupling columns outermost,
+// then pass and residue), so all variants give bit-identical results.
+//   {waves M x N, 32 x 32 blocks per wave M x N, workgroups per CU, staging}
+//   0: 2x4, 4x2, 1, registers -> 256 x 256, 8 waves
+//   1: 2x2, 4x2, 2, registers -> 256 x 128, 4 waves
+//   2: 2x2, 2x2, 2, registers -> 128 x 128, 4 waves
+//   3: 2x4, 4x2, 1, direct    -> 256 x 256, 8 waves
+//   4: 2x2, 4x2, 1, direct    -> 256 x 128, 4 waves
+//   5: 2x2, 2x2, 2, direct    -> 128 x 128, 4 waves
+// "registers": next step prefetched through registers, two LDS stages;
+// "direct": direct-to-LDS loads, three stages in flight.
+#define GM_VARIANTS(X)   \
+  X(0, 2, 4, 4, 2, 1, 0) \
+  X(1, 2, 2, 4, 2, 2, 0) \
+  X(2, 2, 2, 2, 2, 2, 0) \
+  X(3, 2, 4, 4, 2, 1, 1) \
+  X(4, 2, 2, 4, 2, 1, 1) \
+  X(5, 2, 2, 2, 2, 2, 1)
+#define GM_NVAR 6
+#define GM_TK 64
+static const int gm_tm[GM_NVAR] = {256, 256, 128, 256, 256, 128};
+static const int gm_tn[GM_NVAR] = {256, 128, 128, 256, 128, 128};
+static const int gm_wgpc[GM_NVAR] = {1, 2, 2, 1, 1, 2};
+// throughput of a fully loaded CU relative to variant 3 (measured at
+// L = 250, N = 5000 and L = 300, N = 3000 on MI355X)
+static const float gm_eff[GM_NVAR] = {0.66f, 0.70f, 0.70f, 1.0f, 0.65f, 0.88f};
 
-// K step: the forward takes 128 where the padded column count allows (its
-// one-hot bytes are reused over 21 steps, so longer steps only cut barrier
-// and loop costs); the gradient loads its bytes every step and runs best at 64
-static int gemm_tk(int mode, int Lp) { return mode == 0 && Lp % 128 == 0 ? 128 : 64; }
+// kernel of a variant (only the staging form in use is instantiated)
+template <int MODE, int WM, int WN, int RB, int CB, int OCC, int DTL, bool GROUPED>
+static const void* gm_kernel() {
+  if constexpr (GROUPED) {
+    if constexpr (DTL)
+      return (const void*)k_gemm_onehot_dtl_grouped<MODE, WM, WN, RB, CB, GM_TK, 3, OCC>;
+    else
+      return (const void*)k_gemm_onehot_grouped<MODE, WM, WN, RB, CB, GM_TK, OCC>;
+  } else {
+    if constexpr (DTL)
+      return (const void*)k_gemm_onehot_dtl<MODE, WM, WN, RB, CB, GM_TK, 3, OCC>;
+    else
+      return (const void*)k_gemm_onehot<MODE, WM, WN, RB, CB, GM_TK, OCC>;
+  }
+}
 
 template <int MODE>
 static int launch_gemm(int variant, const uint16_t* A, long lda, int M, int Kb, int passes,
                        const uint8_t* bytes, long ldbytes, int Lp, int Ncol, float* C, long ldc,
                        int accumulate, hipStream_t s) {
   if (variant < 0 || variant >= GM_NVAR) return CCM_ERR_ARG;
-  const int tk = gemm_tk(MODE, Lp);
-  if ((MODE == 0 && Lp % tk) || Kb % tk) return CCM_ERR_ARG;
+  if ((MODE == 0 && Lp % GM_TK) || Kb % GM_TK) return CCM_ERR_ARG;
   const uint32_t tiles_m = (M + gm_tm[variant] - 1) / gm_tm[variant];
   const uint32_t tiles_n = (Ncol + gm_tn[variant] - 1) / gm_tn[variant];
   const uint32_t tiles = tiles_m * tiles_n, total = round8(tiles);
-#define GM_LAUNCH(TK, WM, WN, RB, CB, OCC)                                                             \
-  hipLaunchKernelGGL((k_gemm_onehot<MODE, WM, WN, RB, CB, TK, OCC>), dim3(total), dim3(64 * WM * WN), 0, s, \
-                     A, lda, M, Kb, passes, bytes, ldbytes, Lp, Ncol, C, ldc, accumulate, tiles_n, tiles, total)
-#define GM_CASE(V, WM, WN, RB, CB, OCC)                 \
-  case V:                                               \
-    if (MODE == 0 && tk == 128)                         \
-      GM_LAUNCH((MODE == 0 ? 128 : 64), WM, WN, RB, CB, 1); \
-    else                                                \
-      GM_LAUNCH(64, WM, WN, RB, CB, OCC);               \
+  const void* fn = nullptr;
+  int nthreads = 0;
+#define GM_CASE(V, WM, WN, RB, CB, OCC, DTL)                                   \
+  case V:                                                                      \
+    fn = gm_kernel<MODE, WM, WN, RB, CB, OCC, DTL, false>();                  \
+    nthreads = 64 * WM * WN;                                                   \
     break;
   switch (variant) { GM_VARIANTS(GM_CASE) }
 #undef GM_CASE
-#undef GM_LAUNCH
+  void* args[] = {(void*)&A,     (void*)&lda,   (void*)&M,  (void*)&Kb,   (void*)&passes,
+                  (void*)&bytes, (void*)&ldbytes, (void*)&Lp, (void*)&Ncol, (void*)&C,
+                  (void*)&ldc,   (void*)&accumulate, (void*)&tiles_n, (void*)&tiles, (void*)&total};
+  if (hipLaunchKernel(fn, dim3(total), dim3(nthreads), args, 0, s) != hipSuccess) return CCM_ERR_LAUNCH;
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }
 
-// grouped: every alignment of the group shares the K step tk
 template <int MODE>
-static int launch_gemm_grouped(int variant, int tk, const ccm_bprob* probs, const ccm_group& g,
-                               hipStream_t s) {
-  if (variant < 0 || variant >= GM_NVAR || g.np <= 0 || (tk != 64 && tk != 128)) return CCM_ERR_ARG;
+static int launch_gemm_grouped(int variant, const ccm_bprob* probs, const ccm_group& g, hipStream_t s) {
+  if (variant < 0 || variant >= GM_NVAR || g.np <= 0) return CCM_ERR_ARG;
   const uint32_t total = round8(g.total);
-#define GG_LAUNCH(TK, WM, WN, RB, CB, OCC)                                                              \
-  hipLaunchKernelGGL((k_gemm_onehot_grouped<MODE, WM, WN, RB, CB, TK, OCC>), dim3(total), dim3(64 * WM * WN), \
-                     0, s, probs, g.offs, g.pidx, g.np, total)
-#define GG_CASE(V, WM, WN, RB, CB, OCC)                 \
-  case V:                                               \
-    if (MODE == 0 && tk == 128)                         \
-      GG_LAUNCH((MODE == 0 ? 128 : 64), WM, WN, RB, CB, 1); \
-    else                                                \
-      GG_LAUNCH(64, WM, WN, RB, CB, OCC);               \
+  const void* fn = nullptr;
+  int nthreads = 0;
+#define GG_CASE(V, WM, WN, RB, CB, OCC, DTL)                                                    \
+  case V:                                                                                       \
+    fn = gm_kernel<MODE, WM, WN, RB, CB, OCC, DTL, true>();                                     \
+    nthreads = 64 * WM * WN;                                                                    \
     break;
   switch (variant) { GM_VARIANTS(GG_CASE) }
 #undef GG_CASE
-#undef GG_LAUNCH
+  const uint32_t* offs = g.offs;
+  const int32_t* pidx = g.pidx;
+  int np = g.np;
+  void* args[] = {(void*)&probs, (void*)&offs, (void*)&pidx, (void*)&np, (void*)&total};
+  if (hipLaunchKernel(fn, dim3(total), dim3(nthreads), args, 0, s) != hipSuccess) return CCM_ERR_LAUNCH;
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }
@@ -396,13 +707,39 @@ static uint32_t gemm_tiles(int variant, int M, int Ncol) {
   return (uint32_t)(((M + gm_tm[variant] - 1) / gm_tm[variant]) * ((Ncol + gm_tn[variant] - 1) / gm_tn[variant]));
 }
 
-// default variant: the largest tile that still gives most CUs a workgroup
-// (forward: 256 x 256; gradient, measured faster with 4-wave workgroups:
-// 256 x 128), then smaller tiles
-static int gemm_default_variant(int mode, int M, int Ncol) {
-  if (mode == 0 && gemm_tiles(0, M, Ncol) >= 192) return 0;
-  if (gemm_tiles(1, M, Ncol) >= 192) return 1;
-  return 2;
+static int device_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+      cus = v;
+    else
+      cus = 256;
+  }
+  return cus;
+}
+
+// default variant: the direct-to-LDS tile with the least estimated time,
+// waves of workgroups over the CUs times the work of one wave per CU over
+// the variant's relative throughput (the last, partial wave costs a full one)
+static int gemm_default_variant(int M, int Ncol) {
+  const int cus = device_cus();
+  int best = 3;
+  float best_t = 0.f;
+  const int cand[2] = {3, 5};
+  for (int i = 0; i < 2; i++) {
+    const int v = cand[i];
+    const uint32_t tiles = gemm_tiles(v, M, Ncol);
+    const uint32_t slots = (uint32_t)(cus * gm_wgpc[v]);
+    const float waves = (float)((tiles + slots - 1) / slots);
+    const float t = waves * (float)(gm_tm[v] * gm_tn[v] * gm_wgpc[v]) / gm_eff[v];
+    if (i == 0 || t < best_t) {
+      best_t = t;
+      best = v;
+    }
+  }
+  return best;
 }
 
 extern "C" {
@@ -415,17 +752,17 @@ uint32_t ccm_gemm_tile_count(int variant, int M, int Ncol) {
   return gemm_tiles(variant, M, Ncol);
 }
 
-// whether a variant fits a contraction: the forward (mode 0) needs Lp a
-// multiple of 64, the gradient (mode 1) kcols a multiple of 64
+// whether a variant fits a contraction: the forward (mode 0) needs Lp, the
+// gradient (mode 1) kcols, a multiple of the K step 64
 int ccm_gemm_variant_ok(int variant, int mode, int Lp, int kcols) {
   if (variant < 0 || variant >= GM_NVAR) return 0;
-  return mode == 0 ? Lp % gemm_tk(0, Lp) == 0 : kcols % gemm_tk(1, Lp) == 0;
+  return mode == 0 ? Lp % GM_TK == 0 : kcols % GM_TK == 0;
 }
 
 // default variant of the forward (mode 0) or gradient (mode 1) contraction
 int ccm_gemm_default_variant_for(int mode, int Lp, int ncols) {
   const int P = CCM_NA * Lp;
-  return gemm_default_variant(mode, P, mode == 0 ? ncols : P);
+  return gemm_default_variant(P, mode == 0 ? ncols : P);
 }
 
 int ccm_convert_wt_blocks(int Lp) {
@@ -442,18 +779,15 @@ int ccm_launch_convert_wt_grouped(const ccm_bprob* d_probs, ccm_group g, hipStre
   return CCM_OK;
 }
 
-// tk: the K step of every alignment of the group (ccm_gemm_tk)
-int ccm_launch_gemm_fwd_grouped(const ccm_bprob* d_probs, ccm_group g, int variant, int tk, hipStream_t s) {
-  return launch_gemm_grouped<0>(variant, tk, d_probs, g, s);
+int ccm_launch_gemm_fwd_grouped(const ccm_bprob* d_probs, ccm_group g, int variant, hipStream_t s) {
+  return launch_gemm_grouped<0>(variant, d_probs, g, s);
 }
 
-int ccm_launch_gemm_grad_grouped(const ccm_bprob* d_probs, ccm_group g, int variant, int tk, hipStream_t s) {
-  return launch_gemm_grouped<1>(variant, tk, d_probs, g, s);
+int ccm_launch_gemm_grad_grouped(const ccm_bprob* d_probs, ccm_group g, int variant, hipStream_t s) {
+  return launch_gemm_grouped<1>(variant, d_probs, g, s);
 }
 
-int ccm_gemm_tk(int mode, int Lp) { return gemm_tk(mode, Lp); }
-
-int ccm_gemm_default_variant(int M, int Ncol) { return gemm_default_variant(0, M, Ncol); }
+int ccm_gemm_default_variant(int M, int Ncol) { return gemm_default_variant(M, Ncol); }
 
 // Zt[c][n] (P x ldc floats, ldc >= ncols) = sum_r Wt[c][r] [x_nk = b] for the
 // ncols (multiple of 128) sequences whose padded MSA rows start at d_msap;
@@ -463,7 +797,7 @@ int ccm_launch_gemm_fwd(const uint16_t* d_wt, long lda, int Lp, int passes, cons
   const int P = CCM_NA * Lp;
   if (Lp % 64 || ncols % 128 || ncols <= 0 || lda < (long)passes * P || ldc < ncols || (lda % 8))
     return CCM_ERR_ARG;
-  if (variant < 0) variant = gemm_default_variant(0, P, ncols);
+  if (variant < 0) variant = gemm_default_variant(P, ncols);
   return launch_gemm<0>(variant, d_wt, lda, P, P, passes, d_msap, (long)Lp, Lp, ncols, d_zt, ldc, 0, s);
 }
 
@@ -476,7 +810,7 @@ int ccm_launch_gemm_grad(const uint16_t* d_dt, long lda, int Lp, int passes, con
   const int P = CCM_NA * Lp;
   if (Lp % 64 || kcols % 64 || kcols <= 0 || lda < (long)passes * kcols || ldc < P || (lda % 8))
     return CCM_ERR_ARG;
-  if (variant < 0) variant = gemm_default_variant(1, P, P);
+  if (variant < 0) variant = gemm_default_variant(P, P);
   return launch_gemm<1>(variant, d_dt, lda, P, kcols, passes, d_msapt, ldmt, Lp, P, d_gt, ldc,
                         accumulate, s);
 }

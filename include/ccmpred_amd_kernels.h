@@ -102,13 +102,8 @@ int ccm_gemm_default_variant_for(int mode, int Lp, int ncols);
 int ccm_dots_blocks(long n);
 int ccm_cg_vec_blocks(long n);
 int ccm_launch_convert_wt_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
-/* K step of the forward (mode 0) or gradient (mode 1) contraction; every
- * alignment of a grouped GEMM launch must share it */
-int ccm_gemm_tk(int mode, int Lp);
-int ccm_launch_gemm_fwd_grouped(const ccm_bprob *d_probs, ccm_group g, int variant, int tk,
-                                ccm_stream_t s);
-int ccm_launch_gemm_grad_grouped(const ccm_bprob *d_probs, ccm_group g, int variant, int tk,
-                                 ccm_stream_t s);
+int ccm_launch_gemm_fwd_grouped(const ccm_bprob *d_probs, ccm_group g, int variant, ccm_stream_t s);
+int ccm_launch_gemm_grad_grouped(const ccm_bprob *d_probs, ccm_group g, int variant, ccm_stream_t s);
 int ccm_launch_softmax_cols_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
 int ccm_launch_pair_epilogue_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
 int ccm_launch_singles_finalize_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);

@@ -63,11 +63,14 @@ def test_batch_evaluate_bitwise(prec):
             b.set_x(i, x)
         f = b.evaluate()
         gb = [b.get_g(i) for i in range(b.n)]
-        # another tile assignment: the smallest tiles everywhere
+        # other tile assignments: register-staged and direct-to-LDS tiles
         nv = b.variants()
-        b.set_variants(np.full(b.n, 2), np.full(b.n, 2))
+        b.set_variants(np.full(b.n, 2), np.full(b.n, 0))
         f2 = b.evaluate()
         gb2 = [b.get_g(i) for i in range(b.n)]
+        b.set_variants(np.full(b.n, 5), np.full(b.n, 4))
+        f3 = b.evaluate()
+        gb3 = [b.get_g(i) for i in range(b.n)]
         b.set_variants(*nv)
         for i, (m, x) in enumerate(zip(msas, xs)):
             with P.PLL(m, precision=prec) as p:
@@ -76,9 +79,10 @@ def test_batch_evaluate_bitwise(prec):
                 gs = gs.cpu().numpy()
             say(f"  [{i}] L={m.shape[1]} N={m.shape[0]} f batch={f[i]:.12g} single={fs:.12g} "
                 f"max|dg|={np.max(np.abs(gb[i] - gs)):.3g}")
-            assert f[i] == fs and f2[i] == fs
+            assert f[i] == fs and f2[i] == fs and f3[i] == fs
             assert np.array_equal(gb[i], gs)
             assert np.array_equal(gb2[i], gs)
+            assert np.array_equal(gb3[i], gs)
 
 
 def test_batch_solve_matches_single():
