@@ -232,7 +232,10 @@ def main():
     gemm = max(("gemm_fwd", "gemm_grad"), key=lambda k: bf16["step_ms"][k])
     gemm_ms = bf16["step_ms"][gemm]
     achieved = flops / (gemm_ms * 1e-3) / 1e12
-    traffic, mfma_busy = measured_counters("void k_gemm_onehot<%d," % (0 if gemm == "gemm_fwd" else 1))
+    mode = 0 if gemm == "gemm_fwd" else 1
+    traffic, mfma_busy = measured_counters("void k_gemm_onehot_dtl<%d," % mode)
+    if traffic is None:
+        traffic, mfma_busy = measured_counters("void k_gemm_onehot<%d," % mode)
     line = {
         "metric": "PLL f+grad evals/s at L=250,N=5000: BF16-MFMA vs FP32 GPU vs host CPU",
         "value": round(bf16["evals_per_s"], 2),

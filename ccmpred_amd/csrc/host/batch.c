@@ -12,7 +12,7 @@
  *   gemm_fwd     one launch per forward tile variant in use
  *   softmax_cols
  *   gemm_grad    one launch per gradient tile variant in use
- *   pair_epilogue, singles_finalize, sum3 (f into the control block)
+ *   assemble     pair epilogue and singles; sum3: f into the control block
  *   dots         <g,s>, <g,g>, <g,gprev>
  *   cg_ctrl      line-search / direction decisions of ccm_conjugrad_gpu_ex
  *
@@ -58,7 +58,7 @@ struct ccm_batch {
 	/* group tables (device) and the groups that index them */
 	uint32_t *d_tab;
 	size_t tab_len;
-	bgroup g_conv, g_soft, g_epi, g_sing, g_all, g_dots, g_vec;
+	bgroup g_conv, g_soft, g_asm, g_all, g_dots, g_vec;
 	bgroup g_fwd[NVAR_MAX], g_grad[NVAR_MAX];
 	int n_fwd, n_grad;
 	int32_t *d_nactive, *h_nactive;
@@ -95,15 +95,14 @@ static hipError_t balloc(ccm_batch *b, void **p, size_t bytes)
 
 /* ---- group tables ------------------------------------------------------ */
 
-enum { K_CONV, K_SOFT, K_EPI, K_SING, K_ALL, K_DOTS, K_VEC };
+enum { K_CONV, K_SOFT, K_ASM, K_ALL, K_DOTS, K_VEC };
 
 static uint32_t blocks_of(const ccm_bprob *p, int kind)
 {
 	switch (kind) {
 	case K_CONV: return (uint32_t)ccm_convert_wt_blocks(p->Lp);
 	case K_SOFT: return (uint32_t)ccm_softmax_cols_blocks(p->L, p->N);
-	case K_EPI: return (uint32_t)ccm_pair_epilogue_blocks(p->L);
-	case K_SING: return (uint32_t)ccm_singles_blocks(p->L);
+	case K_ASM: return (uint32_t)ccm_assemble_blocks(p->L);
 	case K_ALL: return 1;
 	case K_DOTS: return (uint32_t)p->n_dots;
 	default: return (uint32_t)ccm_cg_vec_blocks(p->nvp);
@@ -181,8 +180,7 @@ static int build_tables(ccm_batch *b)
 		all[i] = i;
 	if (add_group(&t, &b->g_conv, b->h_probs, all, b->n, K_CONV, 0, 0) ||
 	    add_group(&t, &b->g_soft, b->h_probs, all, b->n, K_SOFT, 0, 0) ||
-	    add_group(&t, &b->g_epi, b->h_probs, all, b->n, K_EPI, 0, 0) ||
-	    add_group(&t, &b->g_sing, b->h_probs, all, b->n, K_SING, 0, 0) ||
+	    add_group(&t, &b->g_asm, b->h_probs, all, b->n, K_ASM, 0, 0) ||
 	    add_group(&t, &b->g_all, b->h_probs, all, b->n, K_ALL, 0, 0) ||
 	    add_group(&t, &b->g_dots, b->h_probs, all, b->n, K_DOTS, 0, 0) ||
 	    add_group(&t, &b->g_vec, b->h_probs, all, b->n, K_VEC, 0, 0))
@@ -217,8 +215,7 @@ static int build_tables(ccm_batch *b)
 	}
 	if (hipMemcpy(b->d_tab, t.h, sizeof(uint32_t) * t.len, hipMemcpyHostToDevice) != hipSuccess)
 		goto out;
-	bgroup *fixed[] = {&b->g_conv, &b->g_soft, &b->g_epi, &b->g_sing, &b->g_all, &b->g_dots,
-	                   &b->g_vec};
+	bgroup *fixed[] = {&b->g_conv, &b->g_soft, &b->g_asm, &b->g_all, &b->g_dots, &b->g_vec};
 	for (size_t i = 0; i < sizeof(fixed) / sizeof(fixed[0]); i++)
 		fix_group(fixed[i], b->d_tab);
 	for (int i = 0; i < b->n_fwd; i++)
@@ -471,8 +468,7 @@ static int enqueue_eval(ccm_batch *b)
 	KCHK(enqueue_gemms(b, 0));
 	KCHK(ccm_launch_softmax_cols_grouped(b->d_probs, b->g_soft.g, b->s));
 	KCHK(enqueue_gemms(b, 1));
-	KCHK(ccm_launch_pair_epilogue_grouped(b->d_probs, b->g_epi.g, b->s));
-	KCHK(ccm_launch_singles_finalize_grouped(b->d_probs, b->g_sing.g, b->s));
+	KCHK(ccm_launch_assemble_grouped(b->d_probs, b->g_asm.g, b->s));
 	KCHK(ccm_launch_sum3_grouped(b->d_probs, b->g_all.g, b->s));
 	return 0;
 fail:

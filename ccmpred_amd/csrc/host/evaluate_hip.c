@@ -459,18 +459,17 @@ fail:
 	return -1;
 }
 
+/* g from the raw pair gradient and the node-gradient partials, and the
+ * objective (softmax partials + regularisation) into d_f, in one launch */
 static int assemble_gradient(ccm_hip_state *st, const float *d_x, float *d_g)
 {
 	hipStream_t s = st->stream;
 	const int L = st->L;
-	if (st->cfg.precision == CCM_PRECISION_FP32)
-		KCHK(ccm_launch_pair_epilogue(st->d_graw, st->ldg, L, NULL, d_x, d_g, L, st->lam2,
-		                              st->d_regpair, s));
-	else
-		KCHK(ccm_launch_pair_epilogue(st->d_gt, (long)st->P, st->Lp, st->d_hsym, d_x, d_g, L,
-		                              st->lam2, st->d_regpair, s));
-	KCHK(ccm_launch_singles_finalize(st->d_g1part, st->n_g1blk, d_x, d_g, L, st->lam1,
-	                                 st->d_regsingle, s));
+	const int fp32 = st->cfg.precision == CCM_PRECISION_FP32;
+	KCHK(ccm_launch_assemble(fp32 ? st->d_graw : st->d_gt, fp32 ? st->ldg : (long)st->P,
+	                         fp32 ? L : st->Lp, fp32 ? NULL : st->d_hsym, d_x, d_g, L, st->lam1,
+	                         st->lam2, st->d_g1part, st->n_g1blk, st->d_fpart, st->n_fpart,
+	                         st->d_regpair, st->d_regsingle, st->d_f, s));
 	return 0;
 fail:
 	return -1;
@@ -505,18 +504,8 @@ int ccm_hip_evaluate(void *instance, const float *d_x, float *d_g, int nvar, dou
 	if (run_chunks(st, d_x, RUN_FULL, &step))
 		goto fail;
 	MARK(step++);
-	if (st->cfg.precision == CCM_PRECISION_FP32) {
-		KCHK(ccm_launch_pair_epilogue(st->d_graw, st->ldg, st->L, NULL, d_x, d_g, st->L,
-		                              st->lam2, st->d_regpair, s));
-	} else {
-		KCHK(ccm_launch_pair_epilogue(st->d_gt, (long)st->P, st->Lp, st->d_hsym, d_x, d_g,
-		                              st->L, st->lam2, st->d_regpair, s));
-	}
-	MARK(step++);
-	KCHK(ccm_launch_singles_finalize(st->d_g1part, st->n_g1blk, d_x, d_g, st->L, st->lam1,
-	                                 st->d_regsingle, s));
-	KCHK(ccm_launch_sum3(st->d_fpart, st->n_fpart, st->d_regpair, ccm_pair_epilogue_blocks(st->L),
-	                     st->d_regsingle, st->n_regsingle, st->d_f, s));
+	if (assemble_gradient(st, d_x, d_g))
+		goto fail;
 	MARK(step);
 	HIPCHK(hipMemcpyAsync(st->h_f, st->d_f, sizeof(double), hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));

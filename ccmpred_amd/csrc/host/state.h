@@ -13,7 +13,7 @@
 
 #include "ccmpred_amd.h"
 
-#define CCM_TIMING_STEPS 6
+#define CCM_TIMING_STEPS 5
 
 typedef struct ccm_hip_state {
 	ccm_hip_config cfg;

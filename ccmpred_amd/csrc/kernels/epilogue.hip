@@ -235,6 +235,86 @@ __global__ __launch_bounds__(256) void k_sum3_grouped(const ccm_bprob* __restric
   sum3_block(B.fpart, B.n_fpart, B.regpair, B.n_regpair, B.regsingle, B.n_regsingle, &B.ctl->fx);
 }
 
+// ---- gradient assembly: pair tiles and singles in one launch --------------
+//
+// Blocks [0, npair) assemble pair tiles, blocks [npair, npair + nsing) the
+// singles; each leaves its regularisation partial.  The objective's final
+// sum is a separate one-block launch (k_sum3): a last-block reduction here
+// would need a device-scope fence in every block, which writes back the L2
+// of the block's XCD and costs more than the launch it saves.
+extern "C" int ccm_pair_epilogue_blocks(int L);
+extern "C" int ccm_singles_blocks(int L);
+
+__device__ __forceinline__ void assemble_block(const float* __restrict__ M, long ldm, int Lm,
+                                               const float* __restrict__ hsym,
+                                               const float* __restrict__ x, float* __restrict__ g,
+                                               int L, long nsp, float lam1, float lam2,
+                                               const double* __restrict__ g1part, int n_g1blk,
+                                               double* __restrict__ regpair,
+                                               double* __restrict__ regsingle, uint32_t blk) {
+  const uint32_t tiles_c = (uint32_t)((CCM_NAP * L + EP_T - 1) / EP_T);
+  const uint32_t npair = (uint32_t)((CCM_NA * L + EP_T - 1) / EP_T) * tiles_c;
+  if (blk < npair)
+    pair_epilogue_block(M, ldm, Lm, hsym, x + nsp, g + nsp, L, lam2, tiles_c, regpair, blk);
+  else
+    singles_block(g1part, n_g1blk, x, g, L, nsp, lam1, regsingle, blk - npair);
+}
+
+__global__ __launch_bounds__(256) void k_assemble(const float* __restrict__ M, long ldm, int Lm,
+                                                  const float* __restrict__ hsym,
+                                                  const float* __restrict__ x, float* __restrict__ g,
+                                                  int L, long nsp, float lam1, float lam2,
+                                                  const double* __restrict__ g1part, int n_g1blk,
+                                                  double* __restrict__ regpair,
+                                                  double* __restrict__ regsingle) {
+  assemble_block(M, ldm, Lm, hsym, x, g, L, nsp, lam1, lam2, g1part, n_g1blk, regpair, regsingle,
+                 blockIdx.x);
+}
+
+// BF16 layout of every alignment of a group: assemble_blocks(L) blocks each
+__global__ __launch_bounds__(256) void k_assemble_grouped(const ccm_bprob* __restrict__ probs,
+                                                          const uint32_t* __restrict__ offs,
+                                                          const int32_t* __restrict__ pidx, int np) {
+  const int gi = ccm_group_find(offs, np, blockIdx.x);
+  const ccm_bprob& B = probs[pidx[gi]];
+  if (!B.ctl->eval_on) return;
+  assemble_block(B.gt, B.P, B.Lp, B.hsym, B.x, B.g, B.L, B.nsp, B.lam1, B.lam2, B.g1part, B.n_g1blk,
+                 B.regpair, B.regsingle, blockIdx.x - offs[gi]);
+}
+
+extern "C" {
+
+// blocks of the fused assembly: pair tiles, then singles
+int ccm_assemble_blocks(int L) { return ccm_pair_epilogue_blocks(L) + ccm_singles_blocks(L); }
+
+// g (pairs and singles) from the raw pair gradient M (FP32 layout Lm = L,
+// or the padded BF16 layout Lm = Lp with hsym), the node-gradient partials
+// and the regularisation, then *d_f = sum of the softmax partials and all
+// regularisation partials (second, one-block launch).  d_regpair:
+// ccm_pair_epilogue_blocks(L) doubles, d_regsingle: ccm_singles_blocks(L).
+int ccm_launch_assemble(const float* d_m, long ldm, int Lm, const float* d_hsym, const float* d_x,
+                        float* d_g, int L, float lam1, float lam2, const double* d_g1part, int n_g1blk,
+                        const double* d_fpart, int n_fpart, double* d_regpair, double* d_regsingle,
+                        double* d_f, hipStream_t s) {
+  if (L <= 0) return CCM_ERR_ARG;
+  hipLaunchKernelGGL(k_assemble, dim3(ccm_assemble_blocks(L)), dim3(256), 0, s, d_m, ldm, Lm, d_hsym, d_x,
+                     d_g, L, ccm_nsp(L), lam1, lam2, d_g1part, n_g1blk, d_regpair, d_regsingle);
+  CCM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum3, dim3(1), dim3(256), 0, s, d_fpart, n_fpart, d_regpair, ccm_pair_epilogue_blocks(L),
+                     d_regsingle, ccm_singles_blocks(L), d_f);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_assemble_grouped(const ccm_bprob* d_probs, ccm_group g, hipStream_t s) {
+  if (g.np <= 0 || g.total == 0) return CCM_ERR_ARG;
+  hipLaunchKernelGGL(k_assemble_grouped, dim3(g.total), dim3(256), 0, s, d_probs, g.offs, g.pidx, g.np);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+}  // extern "C"
+
 extern "C" {
 
 int ccm_pair_epilogue_blocks(int L) {

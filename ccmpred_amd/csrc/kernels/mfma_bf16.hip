@@ -423,31 +423,54 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
         bv[cb][i] = *reinterpret_cast<const uint4*>(bptr[cb] + (long)kc * TK + 16 * i);
   };
   uint4 bcur[CB][MODE == 0 ? NB16 : 1];
+  // A step's sub-steps are software-pipelined: the LDS fragments of sub-step
+  // s+1 (and the byte granule of the next pair of sub-steps) are requested
+  // before the MFMAs of sub-step s, so that LDS latency overlaps the matrix
+  // pipe instead of preceding it.
   auto compute = [&](const uint8_t* __restrict__ sp, uint32_t bsel) {
     const uint16_t* as = reinterpret_cast<const uint16_t*>(sp);
     const uint8_t* bs = sp + A_BYTES;
+    auto lda_frags = [&](uint4(&a)[RB], int sub) {
+#pragma unroll
+      for (int rb = 0; rb < RB; rb++) {
+        const int r = (wm * RB + rb) * 32 + l32;
+        const int g = q * SUBS + sub;
+        a[rb] = *reinterpret_cast<const uint4*>(&as[r * TK + swz_a<G>(r, g) * 8]);
+      }
+    };
+    // the 16 bytes that serve sub-steps 2*pair and 2*pair + 1
+    auto ld_bytes = [&](uint4(&w)[CB], int pair) {
+#pragma unroll
+      for (int cb = 0; cb < CB; cb++) {
+        if constexpr (MODE == 0) {
+          w[cb] = bcur[cb][pair];
+        } else {
+          const int c = (wn * CB + cb) * 32 + l32;
+          const int g = q * (GB / 2) + pair;
+          w[cb] = *reinterpret_cast<const uint4*>(&bs[c * TK + swz_b<GB>(c, g) * 16]);
+        }
+      }
+    };
+    uint4 af[2][RB];
+    uint4 bw[2][CB];
+    lda_frags(af[0], 0);
+    ld_bytes(bw[0], 0);
 #pragma unroll
     for (int sub = 0; sub < SUBS; sub++) {
+      if (sub + 1 < SUBS) {
+        lda_frags(af[(sub + 1) & 1], sub + 1);
+        if (((sub + 1) & 1) == 0) ld_bytes(bw[((sub + 1) >> 1) & 1], (sub + 1) >> 1);
+      }
       bf16x8_t bfrag[CB];
 #pragma unroll
       for (int cb = 0; cb < CB; cb++) {
-        uint4 w;
-        if constexpr (MODE == 0) {
-          w = bcur[cb][sub >> 1];
-        } else {
-          const int c = (wn * CB + cb) * 32 + l32;
-          const int g = q * (GB / 2) + (sub >> 1);
-          w = *reinterpret_cast<const uint4*>(&bs[c * TK + swz_b<GB>(c, g) * 16]);
-        }
+        const uint4 w = bw[(sub >> 1) & 1][cb];
         const uint2 v = (sub & 1) ? make_uint2(w.z, w.w) : make_uint2(w.x, w.y);
         bfrag[cb] = onehot8(v, MODE == 0 ? bsel : bval[cb]);
       }
 #pragma unroll
       for (int rb = 0; rb < RB; rb++) {
-        const int r = (wm * RB + rb) * 32 + l32;
-        const int g = q * SUBS + sub;
-        const uint4 av = *reinterpret_cast<const uint4*>(&as[r * TK + swz_a<G>(r, g) * 8]);
-        const bf16x8_t afrag = __builtin_bit_cast(bf16x8_t, av);
+        const bf16x8_t afrag = __builtin_bit_cast(bf16x8_t, af[sub & 1][rb]);
 #pragma unroll
         for (int cb = 0; cb < CB; cb++)
           acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afrag, bfrag[cb], acc[rb][cb], 0, 0, 0);

@@ -134,20 +134,25 @@ __device__ __forceinline__ void softmax_cols_block(const float* __restrict__ Zt,
       z[a] = v;
       m = fmaxf(m, v);
     }
-    float s = 0.f;
-#pragma unroll
-    for (int a = 0; a < CCM_NA; a++) s += expf(z[a] - m);
-    const float lse = m + logf(s);
     const int x = msapt[(long)j * ldmt + n];
     const float wn = w[n];
     float zx = 0.f;
 #pragma unroll
     for (int a = 0; a < CCM_NA; a++) zx = (a == x) ? z[a] : zx;
+    // e_a = exp(z_a - m) once; P_a = e_a / sum
+    float s = 0.f;
+#pragma unroll
+    for (int a = 0; a < CCM_NA; a++) {
+      z[a] = expf(z[a] - m);
+      s += z[a];
+    }
+    const float lse = m + logf(s);
     fl += (double)(wn * (lse - zx));
     if (mode) {
+      const float ws = wn / s;
 #pragma unroll
       for (int a = 0; a < CCM_NA; a++) {
-        const float qv = wn * expf(z[a] - lse);
+        const float qv = z[a] * ws;
         const uint16_t hi = ccm_f2bf(qv);
         uint16_t* dst = dcol + (long)a * Lp * lddt;
         dst[0] = hi;

@@ -16,8 +16,8 @@ from . import _lib
 
 # precision form -> (precision enum, gradient-operand passes, forward-operand passes)
 PRECISIONS = {"fp32": (0, 1, 1), "bf16": (1, 1, 1), "bf16x2": (1, 2, 2), "bf16f": (1, 1, 2)}
-FP32_STEPS = ["gather_fwd", "softmax_rows", "gather_grad", "pair_epilogue", "singles_sum"]
-BF16_STEPS = ["convert_wt", "gemm_fwd", "softmax_cols", "gemm_grad", "pair_epilogue", "singles_sum"]
+FP32_STEPS = ["gather_fwd", "softmax_rows", "gather_grad", "assemble"]
+BF16_STEPS = ["convert_wt", "gemm_fwd", "softmax_cols", "gemm_grad", "assemble"]
 
 
 def nsingle_padded(ncol: int) -> int:
@@ -176,8 +176,8 @@ class PLL:
         _lib.lib().ccm_hip_set_timing(self._uptr, 1 if on else 0)
 
     def timing(self):
-        out = (C.c_double * 6)()
-        n = _lib.lib().ccm_hip_get_timing(self._uptr, out, 6)
+        out = (C.c_double * 5)()
+        n = _lib.lib().ccm_hip_get_timing(self._uptr, out, 5)
         names = FP32_STEPS if self.precision == "fp32" else BF16_STEPS
         return n, {k: out[i] for i, k in enumerate(names)}
 

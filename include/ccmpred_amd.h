@@ -114,8 +114,8 @@ void *ccm_hip_stream(void *instance);
  * counterpart in the reference).  set_timing(on=1) creates the events and
  * clears the accumulators; get_timing fills up to n averages in ms and
  * returns the number of timed evaluations.  Steps, FP32 path: gather_fwd,
- * softmax, gather_grad, pair epilogue, singles+sum; BF16 path: convert_wt,
- * gemm_fwd, softmax, gemm_grad, pair epilogue, singles+sum. */
+ * softmax, gather_grad, assembly (pair epilogue, singles and the objective's
+ * sum); BF16 path: convert_wt, gemm_fwd, softmax, gemm_grad, assembly. */
 int ccm_hip_set_timing(void *instance, int on);
 int ccm_hip_get_timing(void *instance, double *avg_ms, int n);
 

@@ -79,6 +79,13 @@ int ccm_launch_reg_only(const float *d_x, int L, float lam1, float lam2, double 
                         ccm_stream_t s);
 int ccm_launch_sum3(const double *a, int na, const double *b, int nb, const double *c, int nc,
                     double *out, ccm_stream_t s);
+/* pair epilogue and singles in one launch, then the objective's final sum
+ * (fixed order) into *d_f */
+int ccm_assemble_blocks(int L);
+int ccm_launch_assemble(const float *d_m, long ldm, int Lm, const float *d_hsym, const float *d_x,
+                        float *d_g, int L, float lam1, float lam2, const double *d_g1part, int n_g1blk,
+                        const double *d_fpart, int n_fpart, double *d_regpair, double *d_regsingle,
+                        double *d_f, ccm_stream_t s);
 
 /* conjugate-gradient vector arithmetic */
 int ccm_dots_partial_len(void);
@@ -106,6 +113,7 @@ int ccm_launch_gemm_fwd_grouped(const ccm_bprob *d_probs, ccm_group g, int varia
 int ccm_launch_gemm_grad_grouped(const ccm_bprob *d_probs, ccm_group g, int variant, ccm_stream_t s);
 int ccm_launch_softmax_cols_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
 int ccm_launch_pair_epilogue_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
+int ccm_launch_assemble_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
 int ccm_launch_singles_finalize_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
 int ccm_launch_sum3_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
 int ccm_launch_dots_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_group gfinal, ccm_stream_t s);
