@@ -5,66 +5,122 @@
 //   g2 = 0 for self-edges (k == j) and for padding (a >= 21),
 //   then g2 += 2 lambda_pair x and reg += 0.5 lambda_pair x^2 over the whole
 //   padded block.  M is the raw (unsymmetrised) pair gradient in either the
-//   FP32 layout (Lm = L) or the padded BF16 layout (Lm = Lp).
+//   FP32 layout (Lm = L, M(b,k,a,j) = M[b*L + k][a*L + j]) or the padded
+//   BF16 layout (Lm = Lp, M(b,k,a,j) = M[a*Lp + j][b*Lp + k]); only the sum
+//   of both orientations enters g2, so either works.
 // Singles (evaluate_cpu.c:102-109,153-156): g1 = sum of per-block partials
 //   + 2 lambda_single V for the 20L real singles, 0 on the padding.
 #include "common.h"
 
 #define EP_T 64
 
-__device__ __forceinline__ void pair_epilogue_block(const float* __restrict__ M, long ldm, int Lm,
-                                                    const float* __restrict__ hsym,
-                                                    const float* __restrict__ x2,
-                                                    float* __restrict__ g2, int L, float lam2,
-                                                    uint32_t tiles_c, double* __restrict__ regpart,
-                                                    uint32_t blk) {
-  __shared__ float T[EP_T][EP_T + 1];
+// ---- symmetric form: one block per pair of mirrored tiles ------------------
+//
+// In the x2 index space (rows R = b*L + k, columns C = a*L + j, both below
+// 21L) g2[R][C] = M[phi(C)][phi(R)] + M[phi(R)][phi(C)] + hsym[R][C] + 2 l x[R][C],
+// phi(a*L + j) = a*Lm + j, which is symmetric in (R, C) apart from hsym and
+// x.  A block takes the tile pair (I, J), I <= J, of 64 x 64 tiles: it
+// stages M[phi(I)][phi(J)] and M[phi(J)][phi(I)] in LDS once and writes
+// both output tiles, so M is read once.  Blocks past the tile pairs write
+// the padding columns C in [64 NT, 32L) (2 l x only).
+__device__ __forceinline__ int ccm_phi(int X, int L, int Lm) {
+  const int a = X / L;
+  return a * Lm + (X - a * L);
+}
+
+// tile pairs (ti <= tj) of NT tiles, row by row
+__device__ __forceinline__ void ccm_tri_pair(uint32_t idx, int NT, int& ti, int& tj) {
+  // ti: largest t with t*NT - t*(t-1)/2 <= idx
+  const double n = (double)NT + 0.5;
+  int t = (int)(n - sqrt(n * n - 2.0 * (double)idx));
+  if (t < 0) t = 0;
+  if (t > NT - 1) t = NT - 1;
+  while (t > 0 && (uint32_t)(t * NT - t * (t - 1) / 2) > idx) t--;
+  while (t + 1 < NT && (uint32_t)((t + 1) * NT - (t + 1) * t / 2) <= idx) t++;
+  ti = t;
+  tj = t + (int)(idx - (uint32_t)(t * NT - t * (t - 1) / 2));
+}
+
+__device__ __forceinline__ int ccm_pair_tiles(int L) { return (CCM_NA * L + EP_T - 1) / EP_T; }
+
+__device__ __forceinline__ void pair_sym_block(const float* __restrict__ M, long ldm, int Lm,
+                                               const float* __restrict__ hsym,
+                                               const float* __restrict__ x2, float* __restrict__ g2,
+                                               int L, float lam2, double* __restrict__ regpart,
+                                               uint32_t blk) {
+  constexpr int RPT = EP_T / 4;  // rows per thread: 4 waves, rows wv, wv + 4, ...
+  __shared__ float D1[EP_T][EP_T + 1];
+  __shared__ float D2[EP_T][EP_T + 1];
   __shared__ double red[4];
-  const int tr = blk / tiles_c, tc = blk % tiles_c;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int C21 = CCM_NA * L, ld2 = CCM_NAP * L;
-  const float invL = 1.0f / (float)L;
-
-  // transposed operand M(a,j,b,k): rows of the LDS tile follow the output
-  // column (a,j), columns follow the output row (b,k)
-  {
-    const int R = tr * EP_T + lane;
-    const int b = __float2int_rd(((float)R + 0.5f) * invL);
-    const int k = R - b * L;
-    for (int cc = wv; cc < EP_T; cc += 4) {
-      const int Cp = tc * EP_T + cc;
-      float v = 0.f;
-      if (Cp < C21 && R < C21) {
-        const int a = __float2int_rd(((float)Cp + 0.5f) * invL);
-        const int j = Cp - a * L;
-        v = M[(long)(a * Lm + j) * ldm + b * Lm + k];
-      }
-      T[cc][lane] = v;
-    }
-  }
-  __syncthreads();
-
-  const int Cp = tc * EP_T + lane;
-  const int a = __float2int_rd(((float)Cp + 0.5f) * invL);
-  const int j = Cp - a * L;
-  const bool pair = Cp < C21;
-  const bool inrange = Cp < ld2;
+  const int NT = ccm_pair_tiles(L);
+  const uint32_t npairs = (uint32_t)(NT * (NT + 1) / 2);
   const float two_l = 2.0f * lam2, half_l = 0.5f * lam2;
   double reg = 0.0;
-  for (int rr = wv; rr < EP_T; rr += 4) {
-    const int R = tr * EP_T + rr;
-    if (R >= C21 || !inrange) continue;
-    const int b = __float2int_rd(((float)R + 0.5f) * invL);
-    const int k = R - b * L;
-    const long o = (long)R * ld2 + Cp;
-    const float xv = x2[o];
-    float v = 0.f;
-    if (pair && k != j) {
-      v = M[(long)(b * Lm + k) * ldm + a * Lm + j] + T[lane][rr];
-      if (hsym) v += hsym[o];
+  // one output tile: rows R0 + rr, column C; S(rr) from the staged blocks.
+  // All loads of the tile are issued before any result is formed.
+  auto out_tile = [&](int R0, int C, bool pair_col, int jC, bool first) {
+    float xv[RPT], hv[RPT];
+#pragma unroll
+    for (int i = 0; i < RPT; i++) {
+      const int R = R0 + wv + 4 * i;
+      const long o = (long)R * ld2 + C;
+      const bool ok = R < C21;
+      xv[i] = ok ? x2[o] : 0.f;
+      hv[i] = (ok && pair_col && hsym) ? hsym[o] : 0.f;
     }
-    g2[o] = v + two_l * xv;
-    reg += (double)(half_l * xv * xv);
+#pragma unroll
+    for (int i = 0; i < RPT; i++) {
+      const int rr = wv + 4 * i, R = R0 + rr;
+      if (R >= C21) continue;
+      const int kR = R - (R / L) * L;
+      float v = 0.f;
+      if (pair_col && kR != jC)
+        v = (first ? D1[rr][lane] + D2[lane][rr] : D2[rr][lane] + D1[lane][rr]) + hv[i];
+      g2[(long)R * ld2 + C] = v + two_l * xv[i];
+      reg += (double)(half_l * xv[i] * xv[i]);
+    }
+  };
+  if (blk < npairs) {
+    int ti, tj;
+    ccm_tri_pair(blk, NT, ti, tj);
+    // D1[r][c] = M[phi(ti*64 + r)][phi(tj*64 + c)], D2[r][c] = M[phi(tj*64 + r)][phi(ti*64 + c)]
+    {
+      const int Ci = ti * EP_T + lane, Cj = tj * EP_T + lane;
+      const int pci = Ci < C21 ? ccm_phi(Ci, L, Lm) : 0, pcj = Cj < C21 ? ccm_phi(Cj, L, Lm) : 0;
+      float d1[RPT], d2[RPT];
+#pragma unroll
+      for (int i = 0; i < RPT; i++) {
+        const int rr = wv + 4 * i;
+        const int Ri = ti * EP_T + rr, Rj = tj * EP_T + rr;
+        d1[i] = (Ri < C21 && Cj < C21) ? M[(long)ccm_phi(Ri, L, Lm) * ldm + pcj] : 0.f;
+        d2[i] = (Rj < C21 && Ci < C21) ? M[(long)ccm_phi(Rj, L, Lm) * ldm + pci] : 0.f;
+      }
+#pragma unroll
+      for (int i = 0; i < RPT; i++) {
+        D1[wv + 4 * i][lane] = d1[i];
+        D2[wv + 4 * i][lane] = d2[i];
+      }
+    }
+    __syncthreads();
+    // tile (ti, tj): rows ti*64 + rr, column tj*64 + lane
+    {
+      const int C = tj * EP_T + lane;
+      if (C < ld2) out_tile(ti * EP_T, C, C < C21, C - (C / L) * L, true);
+    }
+    // mirrored tile (tj, ti): rows tj*64 + rr, column ti*64 + lane (below C21)
+    if (ti != tj) {
+      const int C = ti * EP_T + lane;
+      out_tile(tj * EP_T, C, true, C - (C / L) * L, false);
+    }
+  } else {
+    // padding columns past the tiles: row tile tr, columns 64 NT + 64 tc + lane
+    const int npc = (ld2 - NT * EP_T + EP_T - 1) / EP_T;
+    const uint32_t q = blk - npairs;
+    const int tr = (int)(q / (uint32_t)npc), tc = (int)(q % (uint32_t)npc);
+    const int C = NT * EP_T + tc * EP_T + lane;
+    if (C < ld2) out_tile(tr * EP_T, C, false, 0, true);
   }
   reg = ccm_wave_sum_f64(reg);
   if (lane == 0) red[wv] = reg;
@@ -72,12 +128,19 @@ __device__ __forceinline__ void pair_epilogue_block(const float* __restrict__ M,
   if (threadIdx.x == 0) regpart[blk] = red[0] + red[1] + red[2] + red[3];
 }
 
+// blocks of the symmetric pair assembly: tile pairs, then padding tiles
+__host__ __device__ __forceinline__ int ccm_pair_sym_blocks(int L) {
+  const int NT = (CCM_NA * L + EP_T - 1) / EP_T;
+  const int npc = (CCM_NAP * L - NT * EP_T + EP_T - 1) / EP_T;
+  return NT * (NT + 1) / 2 + NT * (npc > 0 ? npc : 0);
+}
+
 __global__ __launch_bounds__(256) void k_pair_epilogue(const float* __restrict__ M, long ldm, int Lm,
                                                        const float* __restrict__ hsym,
                                                        const float* __restrict__ x2,
                                                        float* __restrict__ g2, int L, float lam2,
-                                                       uint32_t tiles_c, double* __restrict__ regpart) {
-  pair_epilogue_block(M, ldm, Lm, hsym, x2, g2, L, lam2, tiles_c, regpart, blockIdx.x);
+                                                       double* __restrict__ regpart) {
+  pair_sym_block(M, ldm, Lm, hsym, x2, g2, L, lam2, regpart, blockIdx.x);
 }
 
 // BF16 layout of every alignment of a group: pair_epilogue_blocks(L) blocks each
@@ -87,9 +150,8 @@ __global__ __launch_bounds__(256) void k_pair_epilogue_grouped(const ccm_bprob* 
   const int gi = ccm_group_find(offs, np, blockIdx.x);
   const ccm_bprob& B = probs[pidx[gi]];
   if (!B.ctl->eval_on) return;
-  const uint32_t tc = (uint32_t)((CCM_NAP * B.L + EP_T - 1) / EP_T);
-  pair_epilogue_block(B.gt, B.P, B.Lp, B.hsym, B.x + B.nsp, B.g + B.nsp, B.L, B.lam2, tc, B.regpair,
-                      blockIdx.x - offs[gi]);
+  pair_sym_block(B.gt, B.P, B.Lp, B.hsym, B.x + B.nsp, B.g + B.nsp, B.L, B.lam2, B.regpair,
+                 blockIdx.x - offs[gi]);
 }
 
 // symmetrised -2 H2 in the x2 layout with zero self-edge blocks and padding:
@@ -252,10 +314,9 @@ __device__ __forceinline__ void assemble_block(const float* __restrict__ M, long
                                                const double* __restrict__ g1part, int n_g1blk,
                                                double* __restrict__ regpair,
                                                double* __restrict__ regsingle, uint32_t blk) {
-  const uint32_t tiles_c = (uint32_t)((CCM_NAP * L + EP_T - 1) / EP_T);
-  const uint32_t npair = (uint32_t)((CCM_NA * L + EP_T - 1) / EP_T) * tiles_c;
+  const uint32_t npair = (uint32_t)ccm_pair_sym_blocks(L);
   if (blk < npair)
-    pair_epilogue_block(M, ldm, Lm, hsym, x + nsp, g + nsp, L, lam2, tiles_c, regpair, blk);
+    pair_sym_block(M, ldm, Lm, hsym, x + nsp, g + nsp, L, lam2, regpair, blk);
   else
     singles_block(g1part, n_g1blk, x, g, L, nsp, lam1, regsingle, blk - npair);
 }
@@ -317,19 +378,15 @@ int ccm_launch_assemble_grouped(const ccm_bprob* d_probs, ccm_group g, hipStream
 
 extern "C" {
 
-int ccm_pair_epilogue_blocks(int L) {
-  const int tr = (CCM_NA * L + EP_T - 1) / EP_T, tc = (CCM_NAP * L + EP_T - 1) / EP_T;
-  return tr * tc;
-}
+int ccm_pair_epilogue_blocks(int L) { return ccm_pair_sym_blocks(L); }
 
 int ccm_singles_blocks(int L) { return (int)((ccm_nsp(L) + 255) / 256); }
 
 int ccm_launch_pair_epilogue(const float* d_m, long ldm, int Lm, const float* d_hsym, const float* d_x,
                              float* d_g, int L, float lam2, double* d_regpart, hipStream_t s) {
   const long nsp = ccm_nsp(L);
-  const uint32_t tr = (CCM_NA * L + EP_T - 1) / EP_T, tc = (CCM_NAP * L + EP_T - 1) / EP_T;
-  hipLaunchKernelGGL(k_pair_epilogue, dim3(tr * tc), dim3(256), 0, s, d_m, ldm, Lm, d_hsym, d_x + nsp,
-                     d_g + nsp, L, lam2, tc, d_regpart);
+  hipLaunchKernelGGL(k_pair_epilogue, dim3(ccm_pair_sym_blocks(L)), dim3(256), 0, s, d_m, ldm, Lm, d_hsym,
+                     d_x + nsp, d_g + nsp, L, lam2, d_regpart);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }
