@@ -23,33 +23,46 @@ __device__ __forceinline__ void convert_wt_block(const float* __restrict__ x2, i
                                                  uint16_t* __restrict__ wt, long lda, int passes,
                                                  uint32_t tiles_r, uint32_t blk) {
   __shared__ float T[64][65];
-  const int P = CCM_NA * Lp;
+  const int P = CCM_NA * Lp;  // a multiple of 64: the tiles cover it exactly
   const int tc = blk / tiles_r, tr = blk % tiles_r;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long ld2 = (long)CCM_NAP * L;
   {
+    // T[rr][cc] = W at coupling row r = tr*64 + rr, output row c = tc*64 + cc;
+    // the 16 loads of a thread are issued before they are stored
     const int c = tc * 64 + lane;
     const int a = c / Lp, j = c % Lp;
-    const bool cok = c < P && a < CCM_NA && j < L;
-    for (int rr = wv; rr < 64; rr += 4) {
-      const int r = tr * 64 + rr;
+    const bool cok = a < CCM_NA && j < L;
+    float v[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int r = tr * 64 + wv + 4 * i;
       const int b = r / Lp, k = r % Lp;
-      float v = 0.f;
-      if (cok && r < P && k < L) v = x2[(long)(b * L + k) * ld2 + a * L + j];
-      T[rr][lane] = v;
+      v[i] = (cok && k < L) ? x2[(long)(b * L + k) * ld2 + a * L + j] : 0.f;
     }
+#pragma unroll
+    for (int i = 0; i < 16; i++) T[wv + 4 * i][lane] = v[i];
   }
   __syncthreads();
-  const int r = tr * 64 + lane;
-  if (r < P) {
-    for (int cc = wv; cc < 64; cc += 4) {
-      const int c = tc * 64 + cc;
-      if (c >= P) break;
-      const float v = T[lane][cc];
-      const uint16_t hi = ccm_f2bf(v);
-      wt[(long)c * lda + r] = hi;
-      if (passes > 1) wt[(long)c * lda + P + r] = ccm_f2bf(v - ccm_bf2f(hi));
-    }
+  // each thread converts 16 consecutive r of one output row c and stores
+  // them as two 16-byte vectors (and the residual half likewise)
+  const int cl = threadIdx.x >> 2, qr = threadIdx.x & 3;
+  const int c = tc * 64 + cl, r0 = tr * 64 + qr * 16;
+  uint32_t hi[8], lo[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const float v0 = T[qr * 16 + 2 * i][cl], v1 = T[qr * 16 + 2 * i + 1][cl];
+    const uint16_t h0 = ccm_f2bf(v0), h1 = ccm_f2bf(v1);
+    hi[i] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+    lo[i] = (uint32_t)ccm_f2bf(v0 - ccm_bf2f(h0)) | ((uint32_t)ccm_f2bf(v1 - ccm_bf2f(h1)) << 16);
+  }
+  uint4* dst = reinterpret_cast<uint4*>(wt + (long)c * lda + r0);
+  dst[0] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+  dst[1] = make_uint4(hi[4], hi[5], hi[6], hi[7]);
+  if (passes > 1) {
+    uint4* dlo = reinterpret_cast<uint4*>(wt + (long)c * lda + P + r0);
+    dlo[0] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+    dlo[1] = make_uint4(lo[4], lo[5], lo[6], lo[7]);
   }
 }
 
@@ -623,7 +636,7 @@ extern "C" {
 int ccm_launch_convert_wt(const float* d_x, int L, int Lp, uint16_t* d_wt, long lda, int passes,
                           hipStream_t s) {
   const int P = CCM_NA * Lp;
-  if (Lp % 32 || Lp < L || lda < (long)passes * P) return CCM_ERR_ARG;
+  if (Lp % 64 || Lp < L || lda < (long)passes * P || lda % 8) return CCM_ERR_ARG;
   const uint32_t tiles = (uint32_t)((P + 63) / 64);
   hipLaunchKernelGGL(k_convert_wt, dim3(tiles * tiles), dim3(256), 0, s, d_x + ccm_nsp(L), L, Lp,
                      d_wt, lda, passes, tiles);

@@ -110,13 +110,14 @@ __device__ __forceinline__ void softmax_cols_block(const float* __restrict__ Zt,
 #pragma unroll
   for (int a = 0; a < 20; a++) g1[a] = 0.f;
   double fl = 0.0;
+  // two sequences per iteration, so that the loads of both are in flight
+#pragma unroll 2
   for (int r = 0; r < SC_R; r++) {
     const int n = bx * (256 * SC_R) + r * 256 + threadIdx.x;
-    if (n >= ncols) break;
     uint16_t* dcol = Dt + (long)j * lddt + n;
     if (n >= Nc) {
       // padded columns of Dt stay zero
-      if (mode) {
+      if (mode && n < ncols) {
 #pragma unroll
         for (int a = 0; a < CCM_NA; a++) {
           dcol[(long)a * Lp * lddt] = 0;
