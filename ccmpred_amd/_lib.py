@@ -10,7 +10,8 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libccmpred_amd.so")
+# CCMPRED_AMD_LIB selects another build of the library (same ABI)
+LIB_PATH = os.environ.get("CCMPRED_AMD_LIB") or os.path.join(_HERE, "lib", "libccmpred_amd.so")
 BIN_PATH = os.path.join(_HERE, "bin", "ccmpred")
 
 _lib = None
