@@ -83,18 +83,47 @@ __global__ __launch_bounds__(256) void k_convert_wt_grouped(const ccm_bprob* __r
                    blockIdx.x - offs[gi]);
 }
 
-__device__ __forceinline__ uint32_t oh_pair(uint32_t lo_byte, uint32_t hi_byte, uint32_t b) {
-  return (lo_byte == b ? 0x3F80u : 0u) | (hi_byte == b ? 0x3F800000u : 0u);
+// One-hot fragments from MSA bytes with packed 16-bit integer arithmetic.
+// A "spread" dword holds two bytes in the low bytes of its 16-bit halves;
+// after XOR with the residue (in both halves) a half is zero exactly where
+// the byte matched, and min(t, 1) * 0xC080 + 0x3F80 (mod 2^16) maps 0 to
+// bf16 1.0 (0x3F80) and 1 to 0.  The min is written as the instruction
+// because the compiler otherwise turns the sequence back into per-half
+// compares and selects; the multiply-add becomes one v_pk_mad_u16.
+typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t oh_halves(uint32_t t) {
+  uint32_t m;
+  asm("v_pk_min_u16 %0, %1, %2" : "=v"(m) : "v"(t), "v"(0x00010001u));
+  const u16x2_t r = __builtin_bit_cast(u16x2_t, m) * (u16x2_t)(0xC080u) + (u16x2_t)(0x3F80u);
+  return __builtin_bit_cast(uint32_t, r);
 }
+
+// bytes 0,1 (sel 0) or 2,3 (sel 1) of v into the low bytes of the halves
+__device__ __forceinline__ uint32_t spread_lo(uint32_t v) { return __builtin_amdgcn_perm(0u, v, 0x0c010c00u); }
+__device__ __forceinline__ uint32_t spread_hi(uint32_t v) { return __builtin_amdgcn_perm(0u, v, 0x0c030c02u); }
 
 // 8 MSA bytes -> 8 bf16 one-hot values ([byte == b] ? 1 : 0), element i in
 // the i-th bf16 slot of the fragment
 __device__ __forceinline__ bf16x8_t onehot8(uint2 v, uint32_t b) {
+  const uint32_t p = b * 0x01010101u;
+  const uint32_t t0 = v.x ^ p, t1 = v.y ^ p;
   uint4 o;
-  o.x = oh_pair(v.x & 0xffu, (v.x >> 8) & 0xffu, b);
-  o.y = oh_pair((v.x >> 16) & 0xffu, v.x >> 24, b);
-  o.z = oh_pair(v.y & 0xffu, (v.y >> 8) & 0xffu, b);
-  o.w = oh_pair((v.y >> 16) & 0xffu, v.y >> 24, b);
+  o.x = oh_halves(spread_lo(t0));
+  o.y = oh_halves(spread_hi(t0));
+  o.z = oh_halves(spread_lo(t1));
+  o.w = oh_halves(spread_hi(t1));
+  return __builtin_bit_cast(bf16x8_t, o);
+}
+
+// the same from 8 bytes already spread into four dwords; p2 = b * 0x00010001
+__device__ __forceinline__ bf16x8_t onehot8_spread(uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3,
+                                                   uint32_t p2) {
+  uint4 o;
+  o.x = oh_halves(s0 ^ p2);
+  o.y = oh_halves(s1 ^ p2);
+  o.z = oh_halves(s2 ^ p2);
+  o.w = oh_halves(s3 ^ p2);
   return __builtin_bit_cast(bf16x8_t, o);
 }
 
@@ -436,6 +465,25 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
         bv[cb][i] = *reinterpret_cast<const uint4*>(bptr[cb] + (long)kc * TK + 16 * i);
   };
   uint4 bcur[CB][MODE == 0 ? NB16 : 1];
+  // MODE 0: the block's bytes spread for onehot8_spread, once per K block
+  // (at its first step, so that the byte loads overlap the previous steps)
+  uint32_t bsp[CB][MODE == 0 ? NB16 * 8 : 1];
+  auto spread_bytes = [&]() {
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+      for (int i = 0; i < (MODE == 0 ? NB16 : 0); i++) {
+        const uint4 w = bcur[cb][i];
+        bsp[cb][i * 8 + 0] = spread_lo(w.x);
+        bsp[cb][i * 8 + 1] = spread_hi(w.x);
+        bsp[cb][i * 8 + 2] = spread_lo(w.y);
+        bsp[cb][i * 8 + 3] = spread_hi(w.y);
+        bsp[cb][i * 8 + 4] = spread_lo(w.z);
+        bsp[cb][i * 8 + 5] = spread_hi(w.z);
+        bsp[cb][i * 8 + 6] = spread_lo(w.w);
+        bsp[cb][i * 8 + 7] = spread_hi(w.w);
+      }
+  };
   // A step's sub-steps are software-pipelined: the LDS fragments of sub-step
   // s+1 (and the byte granule of the next pair of sub-steps) are requested
   // before the MFMAs of sub-step s, so that LDS latency overlaps the matrix
@@ -451,13 +499,12 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
         a[rb] = *reinterpret_cast<const uint4*>(&as[r * TK + swz_a<G>(r, g) * 8]);
       }
     };
-    // the 16 bytes that serve sub-steps 2*pair and 2*pair + 1
+    // MODE 1: the 16 staged bytes that serve sub-steps 2*pair and 2*pair + 1
+    // (MODE 0 reads the spread bytes held in registers)
     auto ld_bytes = [&](uint4(&w)[CB], int pair) {
+      if constexpr (MODE == 1) {
 #pragma unroll
-      for (int cb = 0; cb < CB; cb++) {
-        if constexpr (MODE == 0) {
-          w[cb] = bcur[cb][pair];
-        } else {
+        for (int cb = 0; cb < CB; cb++) {
           const int c = (wn * CB + cb) * 32 + l32;
           const int g = q * (GB / 2) + pair;
           w[cb] = *reinterpret_cast<const uint4*>(&bs[c * TK + swz_b<GB>(c, g) * 16]);
@@ -477,9 +524,15 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
       bf16x8_t bfrag[CB];
 #pragma unroll
       for (int cb = 0; cb < CB; cb++) {
-        const uint4 w = bw[(sub >> 1) & 1][cb];
-        const uint2 v = (sub & 1) ? make_uint2(w.z, w.w) : make_uint2(w.x, w.y);
-        bfrag[cb] = onehot8(v, MODE == 0 ? bsel : bval[cb]);
+        if constexpr (MODE == 0) {
+          const int d = (sub >> 1) * 8 + (sub & 1) * 4;
+          bfrag[cb] = onehot8_spread(bsp[cb][d], bsp[cb][d + 1], bsp[cb][d + 2], bsp[cb][d + 3],
+                                     bsel * 0x00010001u);
+        } else {
+          const uint4 w = bw[(sub >> 1) & 1][cb];
+          const uint2 v = (sub & 1) ? make_uint2(w.z, w.w) : make_uint2(w.x, w.y);
+          bfrag[cb] = onehot8(v, bval[cb]);
+        }
       }
 #pragma unroll
       for (int rb = 0; rb < RB; rb++) {
@@ -505,6 +558,7 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
     if constexpr (MODE == 0) {
       const int r = st % inner;
       bsel = (uint32_t)(r - (r / CCM_NA) * CCM_NA);
+      if (r == 0) spread_bytes();
     }
     compute(cur, bsel);
   };
