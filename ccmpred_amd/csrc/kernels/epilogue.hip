@@ -20,9 +20,10 @@
 // 21L) g2[R][C] = M[phi(C)][phi(R)] + M[phi(R)][phi(C)] + hsym[R][C] + 2 l x[R][C],
 // phi(a*L + j) = a*Lm + j, which is symmetric in (R, C) apart from hsym and
 // x.  A block takes the tile pair (I, J), I <= J, of 64 x 64 tiles: it
-// stages M[phi(I)][phi(J)] and M[phi(J)][phi(I)] in LDS once and writes
-// both output tiles, so M is read once.  Blocks past the tile pairs write
-// the padding columns C in [64 NT, 32L) (2 l x only).
+// stages M[phi(I)][phi(J)] + hsym[I][J] and M[phi(J)][phi(I)] in LDS once
+// and writes both output tiles, so M is read once and, hsym being
+// symmetric, half of hsym.  Blocks past the tile pairs write the padding
+// columns C in [64 NT, 32L) (2 l x only).
 __device__ __forceinline__ int ccm_phi(int X, int L, int Lm) {
   const int a = X / L;
   return a * Lm + (X - a * L);
@@ -61,14 +62,11 @@ __device__ __forceinline__ void pair_sym_block(const float* __restrict__ M, long
   // one output tile: rows R0 + rr, column C; S(rr) from the staged blocks.
   // All loads of the tile are issued before any result is formed.
   auto out_tile = [&](int R0, int C, bool pair_col, int jC, bool first) {
-    float xv[RPT], hv[RPT];
+    float xv[RPT];
 #pragma unroll
     for (int i = 0; i < RPT; i++) {
       const int R = R0 + wv + 4 * i;
-      const long o = (long)R * ld2 + C;
-      const bool ok = R < C21;
-      xv[i] = ok ? x2[o] : 0.f;
-      hv[i] = (ok && pair_col && hsym) ? hsym[o] : 0.f;
+      xv[i] = R < C21 ? x2[(long)R * ld2 + C] : 0.f;
     }
 #pragma unroll
     for (int i = 0; i < RPT; i++) {
@@ -77,7 +75,7 @@ __device__ __forceinline__ void pair_sym_block(const float* __restrict__ M, long
       const int kR = R - (R / L) * L;
       float v = 0.f;
       if (pair_col && kR != jC)
-        v = (first ? D1[rr][lane] + D2[lane][rr] : D2[rr][lane] + D1[lane][rr]) + hv[i];
+        v = first ? D1[rr][lane] + D2[lane][rr] : D2[rr][lane] + D1[lane][rr];
       g2[(long)R * ld2 + C] = v + two_l * xv[i];
       reg += (double)(half_l * xv[i] * xv[i]);
     }
@@ -85,22 +83,29 @@ __device__ __forceinline__ void pair_sym_block(const float* __restrict__ M, long
   if (blk < npairs) {
     int ti, tj;
     ccm_tri_pair(blk, NT, ti, tj);
-    // D1[r][c] = M[phi(ti*64 + r)][phi(tj*64 + c)], D2[r][c] = M[phi(tj*64 + r)][phi(ti*64 + c)]
+    // D1[r][c] = M[phi(ti*64 + r)][phi(tj*64 + c)] + hsym[ti*64 + r][tj*64 + c],
+    // D2[r][c] = M[phi(tj*64 + r)][phi(ti*64 + c)] (hsym split when ti == tj)
     {
       const int Ci = ti * EP_T + lane, Cj = tj * EP_T + lane;
       const int pci = Ci < C21 ? ccm_phi(Ci, L, Lm) : 0, pcj = Cj < C21 ? ccm_phi(Cj, L, Lm) : 0;
-      float d1[RPT], d2[RPT];
+      float d1[RPT], d2[RPT], h1[RPT];
 #pragma unroll
       for (int i = 0; i < RPT; i++) {
         const int rr = wv + 4 * i;
         const int Ri = ti * EP_T + rr, Rj = tj * EP_T + rr;
-        d1[i] = (Ri < C21 && Cj < C21) ? M[(long)ccm_phi(Ri, L, Lm) * ldm + pcj] : 0.f;
+        const bool ok1 = Ri < C21 && Cj < C21;
+        d1[i] = ok1 ? M[(long)ccm_phi(Ri, L, Lm) * ldm + pcj] : 0.f;
+        h1[i] = (ok1 && hsym) ? hsym[(long)Ri * ld2 + Cj] : 0.f;
         d2[i] = (Rj < C21 && Ci < C21) ? M[(long)ccm_phi(Rj, L, Lm) * ldm + pci] : 0.f;
       }
+      // a diagonal pair (one tile, mirrored onto itself) gives half of hsym
+      // to each staged block, so that both orientations add the same two
+      // operands and g2 stays exactly symmetric
+      const float h1s = ti == tj ? 0.5f : 1.0f, h2s = ti == tj ? 0.5f : 0.0f;
 #pragma unroll
       for (int i = 0; i < RPT; i++) {
-        D1[wv + 4 * i][lane] = d1[i];
-        D2[wv + 4 * i][lane] = d2[i];
+        D1[wv + 4 * i][lane] = d1[i] + h1s * h1[i];
+        D2[wv + 4 * i][lane] = d2[i] + h2s * h1[i];
       }
     }
     __syncthreads();
@@ -248,8 +253,11 @@ __global__ __launch_bounds__(256) void k_reg_only(const float* __restrict__ x, l
   if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
-// Neumaier-compensated FP64 sum of up to three partial arrays, fixed order.
-// out[0] = total (double)
+// Neumaier-compensated FP64 sum of up to three partial arrays, fixed order:
+// thread t sums elements t, t + 256, ... of the concatenation a|b|c (eight
+// loads issued before their adds), then wave 0 folds the 256 thread sums,
+// four per lane, and a lane tree combines the 64 (sum, compensation) pairs
+// with exact two-sums.  out[0] = total (double)
 __device__ __forceinline__ void sum3_block(const double* __restrict__ a, int na,
                                            const double* __restrict__ b, int nb,
                                            const double* __restrict__ c, int nc,
@@ -264,20 +272,39 @@ __device__ __forceinline__ void sum3_block(const double* __restrict__ a, int na,
       comp += (v - t) + s;
     s = t;
   };
-  for (int i = threadIdx.x; i < na; i += 256) add(a[i]);
-  for (int i = threadIdx.x; i < nb; i += 256) add(b[i]);
-  for (int i = threadIdx.x; i < nc; i += 256) add(c[i]);
+  const int n = na + nb + nc;
+  for (int i0 = threadIdx.x; i0 < n; i0 += 256 * 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int i = i0 + u * 256;
+      v[u] = i < na ? a[i] : i < na + nb ? b[i - na] : i < n ? c[i - na - nb] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (i0 + u * 256 < n) add(v[u]);
+  }
   sh[threadIdx.x] = s;
   shc[threadIdx.x] = comp;
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < 64) {
+    const int l = threadIdx.x;
     s = 0.0;
     comp = 0.0;
-    for (int i = 0; i < 256; i++) {
-      add(sh[i]);
-      comp += shc[i];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      add(sh[4 * l + u]);
+      comp += shc[4 * l + u];
     }
-    out[0] = s + comp;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const double s2 = __shfl_down(s, off, 64), c2 = __shfl_down(comp, off, 64);
+      const double t = s + s2, bp = t - s;
+      const double e = (s - (t - bp)) + (s2 - bp);
+      s = t;
+      comp = comp + c2 + e;
+    }
+    if (l == 0) out[0] = s + comp;
   }
 }
 

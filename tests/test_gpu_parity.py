@@ -121,6 +121,34 @@ def test_fp32_deterministic_and_structured():
         assert torch.all(g1[20 * L: nsp] == 0)
 
 
+@pytest.mark.parametrize("prec", ["bf16", "bf16f"])
+def test_bf16_deterministic_and_structured(prec):
+    """The BF16 forms at L=250, N=5000: bitwise repeatable, pair gradient
+    exactly symmetric (the constant count block is added so that both
+    orientations sum the same operands), padding 2*lam2*x, self-edges 0."""
+    L, N = 250, 5000
+    say(f"{prec} structure L={L} N={N}")
+    msa, _ = synthetic_msa(L, N, 2, 60)
+    x = make_x(msa, 7, 0.01)
+    nsp = nsingle_padded(L)
+    # nonzero padding couplings: they enter only through 2*lam2*x
+    x[nsp:].reshape(NA, L, NAP, L)[:, :, NA:, :] = np.random.default_rng(8).normal(
+        0, 0.01, (NA, L, NAP - NA, L)).astype(np.float32)
+    with P.PLL(msa, precision=prec) as p:
+        xd = torch.from_numpy(x).cuda()
+        f1, g1 = p.evaluate(xd)
+        f2, g2 = p.evaluate(xd)
+        assert f1 == f2 and torch.equal(g1, g2)
+        G = g1[nsp:].view(NA, L, NAP, L)
+        T = G[:, :, :NA, :]
+        assert torch.equal(T, T.permute(2, 3, 0, 1))
+        X = xd[nsp:].view(NA, L, NAP, L)
+        assert torch.equal(G[:, :, NA:, :], (2.0 * np.float32(p.lambda_pair)) * X[:, :, NA:, :])
+        idx = torch.arange(L, device=G.device)
+        assert torch.all(T[:, idx, :, idx] == 0)
+        assert torch.all(g1[20 * L: nsp] == 0)
+
+
 def test_x_zero_closed_form_full_size():
     """At x = 0: f = L ln21 sum(w), g1 = sum(w)/21 - C, g2 = (C_kb + C_ja)/21 - 2 H2
     with H2 = A^T diag(w) A formed independently from the one-hot MSA."""
