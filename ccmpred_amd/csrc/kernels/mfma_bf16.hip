@@ -553,13 +553,18 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
     else
       ccm_wait_vmcnt<0>();
     ccm_lds_barrier();
-    if (st + S - 1 < nsteps) issue(nxt, st + S - 1);
     uint32_t bsel = 0;
     if constexpr (MODE == 0) {
-      const int r = st % inner;
+      // first step of a K block: spread its bytes (loaded a block earlier,
+      // retired by the wait above) and request the next block's
+      const int kc = st / inner, r = st - kc * inner;
       bsel = (uint32_t)(r - (r / CCM_NA) * CCM_NA);
-      if (r == 0) spread_bytes();
+      if (r == 0) {
+        spread_bytes();
+        if (kc + 1 < kblocks) loadb(bcur, kc + 1);
+      }
     }
+    if (st + S - 1 < nsteps) issue(nxt, st + S - 1);
     compute(cur, bsel);
   };
 
@@ -573,25 +578,14 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
   // the prologue's loads are complete before the loop, so that the only
   // loads in flight inside it are those the steps issue
   ccm_wait_vmcnt<0>();
-  if constexpr (MODE == 0) {
-    // 21 * passes steps per K block, a multiple of the three stages; the
-    // next block's bytes are loaded at the end of each block (the last
-    // block reloads its own)
-    for (int kc = 0; kc < kblocks; kc++) {
-      for (int r = 0; r < inner; r += 3) {
-        const int st = kc * inner + r;
-        step(s0, s2, st);
-        step(s1, s0, st + 1);
-        step(s2, s1, st + 2);
-      }
-      loadb(bcur, kc + 1 < kblocks ? kc + 1 : kc);
-    }
-  } else {
-    for (int st = 0; st < nsteps; st += 3) {
-      step(s0, s2, st);
-      if (st + 1 < nsteps) step(s1, s0, st + 1);
-      if (st + 2 < nsteps) step(s2, s1, st + 2);
-    }
+  // one flat loop over the steps, three (one per stage) per iteration: a
+  // nested loop over K blocks made the compiler lose track of which stage
+  // the loads in flight at the inner loop's head write, and wait for all of
+  // them before the first step's LDS reads
+  for (int st = 0; st < nsteps; st += 3) {
+    step(s0, s2, st);
+    if (st + 1 < nsteps) step(s1, s0, st + 1);
+    if (st + 2 < nsteps) step(s2, s1, st + 2);
   }
 
 #pragma unroll
