@@ -87,11 +87,12 @@ __global__ __launch_bounds__(256) void k_softmax_rows(float* __restrict__ ZD, lo
   }
 }
 
-#define SC_R 8  // sequences per thread in the transposed variant (2048 per block)
+#define SC_SEQ 512  // sequences per block in the transposed variant
 
-// One block: column j (blockIdx.y) for 2048 consecutive sequences; each
-// thread takes 8 sequences 256 apart and accumulates its g1 terms before a
-// single wave reduction per residue.
+// One block: column j (blockIdx.y) for 512 consecutive sequences; thread t
+// takes the adjacent pair n, n+1 (n = 512 bx + 2t), so that its logits are
+// 8-byte loads and its BF16 results 4-byte stores, and accumulates its g1
+// terms before a single wave reduction per residue.
 __device__ __forceinline__ void softmax_cols_block(const float* __restrict__ Zt, long ldzt,
                                                    const float* __restrict__ x1,
                                                    const uint8_t* __restrict__ msapt, long ldmt,
@@ -103,62 +104,67 @@ __device__ __forceinline__ void softmax_cols_block(const float* __restrict__ Zt,
   __shared__ float g1red[4][20];
   __shared__ double fred[4];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  float v1[20];
-#pragma unroll
-  for (int a = 0; a < 20; a++) v1[a] = x1[(long)a * L + j];
+  const int n = bx * SC_SEQ + 2 * threadIdx.x;
   float g1[20];
 #pragma unroll
   for (int a = 0; a < 20; a++) g1[a] = 0.f;
   double fl = 0.0;
-  // two sequences per iteration, so that the loads of both are in flight
-#pragma unroll 2
-  for (int r = 0; r < SC_R; r++) {
-    const int n = bx * (256 * SC_R) + r * 256 + threadIdx.x;
-    uint16_t* dcol = Dt + (long)j * lddt + n;
-    if (n >= Nc) {
-      // padded columns of Dt stay zero
-      if (mode && n < ncols) {
+  if (n < ncols) {
+    // both sequences' logits (singles added), loads first
+    float2 z[CCM_NA];
+    const float2* zc = reinterpret_cast<const float2*>(Zt + (long)j * ldzt + n);
+    const long zs = (long)Lp * ldzt / 2;
 #pragma unroll
-        for (int a = 0; a < CCM_NA; a++) {
-          dcol[(long)a * Lp * lddt] = 0;
-          if (passes > 1) dcol[(long)a * Lp * lddt + lo_off] = 0;
-        }
-      }
-      continue;
+    for (int a = 0; a < CCM_NA; a++) z[a] = zc[a * zs];
+    const bool ok0 = n < Nc, ok1 = n + 1 < Nc;
+    const int x0 = ok0 ? msapt[(long)j * ldmt + n] : 255;
+    const int x1v = ok1 ? msapt[(long)j * ldmt + n + 1] : 255;
+    const float w0 = ok0 ? w[n] : 0.f, w1 = ok1 ? w[n + 1] : 0.f;
+#pragma unroll
+    for (int a = 0; a < 20; a++) {
+      const float v = x1[(long)a * L + j];
+      z[a].x += v;
+      z[a].y += v;
     }
-    float z[CCM_NA];
-    float m = -INFINITY;
+    // padding sequences past Nc: finite placeholder logits, weight 0
 #pragma unroll
     for (int a = 0; a < CCM_NA; a++) {
-      float v = Zt[(long)(a * Lp + j) * ldzt + n];
-      if (a < 20) v += v1[a];
-      z[a] = v;
-      m = fmaxf(m, v);
+      z[a].x = ok0 ? z[a].x : 0.f;
+      z[a].y = ok1 ? z[a].y : 0.f;
     }
-    const int x = msapt[(long)j * ldmt + n];
-    const float wn = w[n];
-    float zx = 0.f;
+    float m0 = -INFINITY, m1 = -INFINITY, zx0 = 0.f, zx1 = 0.f;
 #pragma unroll
-    for (int a = 0; a < CCM_NA; a++) zx = (a == x) ? z[a] : zx;
+    for (int a = 0; a < CCM_NA; a++) {
+      m0 = fmaxf(m0, z[a].x);
+      m1 = fmaxf(m1, z[a].y);
+      zx0 = (a == x0) ? z[a].x : zx0;
+      zx1 = (a == x1v) ? z[a].y : zx1;
+    }
     // e_a = exp(z_a - m) once; P_a = e_a / sum
-    float s = 0.f;
+    float s0 = 0.f, s1 = 0.f;
 #pragma unroll
     for (int a = 0; a < CCM_NA; a++) {
-      z[a] = expf(z[a] - m);
-      s += z[a];
+      z[a].x = expf(z[a].x - m0);
+      z[a].y = expf(z[a].y - m1);
+      s0 += z[a].x;
+      s1 += z[a].y;
     }
-    const float lse = m + logf(s);
-    fl += (double)(wn * (lse - zx));
+    if (ok0) fl += (double)(w0 * (m0 + logf(s0) - zx0));
+    if (ok1) fl += (double)(w1 * (m1 + logf(s1) - zx1));
     if (mode) {
-      const float ws = wn / s;
+      // padded sequences (weight 0) store zeros
+      const float ws0 = w0 / s0, ws1 = w1 / s1;
+      uint32_t* dcol = reinterpret_cast<uint32_t*>(Dt + (long)j * lddt + n);
+      const long ds = (long)Lp * lddt / 2;
 #pragma unroll
       for (int a = 0; a < CCM_NA; a++) {
-        const float qv = z[a] * ws;
-        const uint16_t hi = ccm_f2bf(qv);
-        uint16_t* dst = dcol + (long)a * Lp * lddt;
-        dst[0] = hi;
-        if (passes > 1) dst[lo_off] = ccm_f2bf(qv - ccm_bf2f(hi));
-        if (a < 20) g1[a] += qv - ((a == x) ? wn : 0.f);
+        const float q0 = z[a].x * ws0, q1 = z[a].y * ws1;
+        const uint16_t h0 = ccm_f2bf(q0), h1 = ccm_f2bf(q1);
+        dcol[a * ds] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+        if (passes > 1)
+          dcol[a * ds + lo_off / 2] = (uint32_t)ccm_f2bf(q0 - ccm_bf2f(h0)) |
+                                      ((uint32_t)ccm_f2bf(q1 - ccm_bf2f(h1)) << 16);
+        if (a < 20) g1[a] += (q0 - ((a == x0) ? w0 : 0.f)) + (q1 - ((a == x1v) ? w1 : 0.f));
       }
     }
   }
@@ -200,7 +206,7 @@ __global__ __launch_bounds__(256) void k_softmax_cols_grouped(const ccm_bprob* _
   const int gi = ccm_group_find(offs, np, blockIdx.x);
   const ccm_bprob& B = probs[pidx[gi]];
   if (!B.ctl->eval_on) return;
-  const int nb = (B.ncols + 256 * SC_R - 1) / (256 * SC_R);
+  const int nb = (B.ncols + SC_SEQ - 1) / SC_SEQ;
   const int t = (int)(blockIdx.x - offs[gi]);
   softmax_cols_block(B.zt, B.ncols, B.x, B.msapt, B.ldmt, B.w, B.L, B.Lp, B.N, B.ncols, B.dt, B.lddt,
                      B.ncols, B.passes, 1, 0, B.fpart, B.g1part, t % nb, t / nb);
@@ -211,9 +217,9 @@ extern "C" {
 // partial-array geometry over the whole alignment (chunks are multiples of 256)
 int ccm_softmax_rows_blocks(int L, int N) { return ((L + 63) / 64) * ((N + 63) / 64); }
 int ccm_softmax_rows_nblk(int N) { return (N + 63) / 64; }
-int ccm_softmax_cols_blocks(int L, int N) { return ((N + 256 * SC_R - 1) / (256 * SC_R)) * L; }
-int ccm_softmax_cols_nblk(int N) { return (N + 256 * SC_R - 1) / (256 * SC_R); }
-int ccm_softmax_cols_granule(void) { return 256 * SC_R; }
+int ccm_softmax_cols_blocks(int L, int N) { return ((N + SC_SEQ - 1) / SC_SEQ) * L; }
+int ccm_softmax_cols_nblk(int N) { return (N + SC_SEQ - 1) / SC_SEQ; }
+int ccm_softmax_cols_granule(void) { return SC_SEQ; }
 
 // rows layout on sequences [n0, n0+Nc) of the alignment; pointers are
 // already offset to the chunk, n0 must be a multiple of 64
@@ -235,10 +241,12 @@ int ccm_launch_softmax_cols(const float* d_zt, long ldzt, const float* d_x1, con
                             long ldmt, const float* d_w, int L, int Lp, int Nc, int ncols, int n0,
                             uint16_t* d_dt, long lddt, long lo_off, int passes, int mode,
                             double* d_fpart, double* d_g1part, hipStream_t s) {
-  if (L <= 0 || Nc <= 0 || ncols < Nc || (n0 % (256 * SC_R))) return CCM_ERR_ARG;
-  const int nb = (ncols + 256 * SC_R - 1) / (256 * SC_R);
+  if (L <= 0 || Nc <= 0 || ncols < Nc || (n0 % SC_SEQ) || (ncols & 1) || (ldzt & 1) || (lddt & 1) ||
+      (lo_off & 1))
+    return CCM_ERR_ARG;
+  const int nb = (ncols + SC_SEQ - 1) / SC_SEQ;
   hipLaunchKernelGGL(k_softmax_cols, dim3(nb, L), dim3(256), 0, s, d_zt, ldzt, d_x1, d_msapt, ldmt, d_w, L,
-                     Lp, Nc, ncols, d_dt, lddt, lo_off, passes, mode, n0 / (256 * SC_R), d_fpart,
+                     Lp, Nc, ncols, d_dt, lddt, lo_off, passes, mode, n0 / SC_SEQ, d_fpart,
                      d_g1part);
   CCM_LAUNCH_CHECK();
   return CCM_OK;

@@ -68,7 +68,7 @@ typedef struct ccm_hip_config {
 	int bf16_passes; /* gradient contraction: 1 = BF16 operand, 2 = hi+lo BF16 split */
 	int bf16_fwd_passes; /* forward contraction, same meaning; 0 = as bf16_passes */
 	int verbose;     /* 1: print the reweighting summary like the reference */
-	int chunk_rows;  /* sequences per chunk (rounded up to 256 in FP32, 2048 in BF16); 0 = all */
+	int chunk_rows;  /* sequences per chunk (rounded up to 256 in FP32, 512 in BF16); 0 = all */
 	int gemm_variant; /* MFMA tile variant (0..ccm_gemm_variants()-1); -1 = default */
 } ccm_hip_config;
 

@@ -81,7 +81,7 @@ def test_recompute_matches_stored(prec):
     say(f"recompute vs stored {prec} L={L} N={N}")
     msa, _ = synthetic_msa(L, N, 5, 24)
     x = torch.from_numpy(make_x(msa, 56)).cuda()
-    # chunks of 2048 sequences: the BF16 softmax works on whole 2048-sequence blocks
+    # chunks of 2048 sequences (a multiple of the 512-sequence BF16 softmax block)
     with P.PLL(msa, precision=prec) as stored, P.PLL(msa, precision=prec, chunk_rows=2048) as chunked:
         xs = x.clone().requires_grad_(True)
         fs = pll_objective(xs, stored, recompute=False)
