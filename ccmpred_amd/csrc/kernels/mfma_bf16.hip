@@ -328,10 +328,18 @@ __device__ __forceinline__ void ccm_wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
-// workgroup barrier for LDS hand-over: wait for this wave's LDS operations,
-// then s_barrier; the memory clobber keeps the compiler from moving LDS
-// accesses across it, and no vector-memory wait is implied
-__device__ __forceinline__ void ccm_lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+// workgroup barrier for LDS hand-over: wait for this wave's LDS operations
+// (lgkmcnt(0)), then s_barrier, between two empty asm statements with a
+// memory clobber that keep the compiler from moving memory accesses across.
+// Written with the builtins, the LDS wait is visible to the compiler's own
+// wait insertion; no vector-memory wait is implied (a release fence would
+// add one for the direct-to-LDS loads of later stages still in flight).
+__device__ __forceinline__ void ccm_lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt, expcnt at their maxima
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
 
 // physical granule of logical granule g in row r of a dense stage with G
 // granules per row (G = 8: rows of 128 bytes; G = 16: rows of 256 bytes)
@@ -484,42 +492,51 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
         bsp[cb][i * 8 + 7] = spread_hi(w.w);
       }
   };
-  // A step's sub-steps are software-pipelined: the LDS fragments of sub-step
-  // s+1 (and the byte granule of the next pair of sub-steps) are requested
-  // before the MFMAs of sub-step s, so that LDS latency overlaps the matrix
-  // pipe instead of preceding it.
-  auto compute = [&](const uint8_t* __restrict__ sp, uint32_t bsel) {
+  // LDS fragments of the dense operand (and MODE 1's bytes) in two register
+  // buffers: sub-step s uses af[s & 1] and bw[(s >> 1) & 1]
+  uint4 af[2][RB];
+  uint4 bw[2][CB];
+  auto lda_frags = [&](const uint8_t* __restrict__ sp, uint4(&a)[RB], int sub) {
     const uint16_t* as = reinterpret_cast<const uint16_t*>(sp);
-    const uint8_t* bs = sp + A_BYTES;
-    auto lda_frags = [&](uint4(&a)[RB], int sub) {
 #pragma unroll
-      for (int rb = 0; rb < RB; rb++) {
-        const int r = (wm * RB + rb) * 32 + l32;
-        const int g = q * SUBS + sub;
-        a[rb] = *reinterpret_cast<const uint4*>(&as[r * TK + swz_a<G>(r, g) * 8]);
-      }
-    };
-    // MODE 1: the 16 staged bytes that serve sub-steps 2*pair and 2*pair + 1
-    // (MODE 0 reads the spread bytes held in registers)
-    auto ld_bytes = [&](uint4(&w)[CB], int pair) {
-      if constexpr (MODE == 1) {
+    for (int rb = 0; rb < RB; rb++) {
+      const int r = (wm * RB + rb) * 32 + l32;
+      const int g = q * SUBS + sub;
+      a[rb] = *reinterpret_cast<const uint4*>(&as[r * TK + swz_a<G>(r, g) * 8]);
+    }
+  };
+  // MODE 1: the 16 staged bytes that serve sub-steps 2*pair and 2*pair + 1
+  // (MODE 0 reads the spread bytes held in registers)
+  auto ld_bytes = [&](const uint8_t* __restrict__ sp, uint4(&w)[CB], int pair) {
+    if constexpr (MODE == 1) {
+      const uint8_t* bs = sp + A_BYTES;
 #pragma unroll
-        for (int cb = 0; cb < CB; cb++) {
-          const int c = (wn * CB + cb) * 32 + l32;
-          const int g = q * (GB / 2) + pair;
-          w[cb] = *reinterpret_cast<const uint4*>(&bs[c * TK + swz_b<GB>(c, g) * 16]);
-        }
+      for (int cb = 0; cb < CB; cb++) {
+        const int c = (wn * CB + cb) * 32 + l32;
+        const int g = q * (GB / 2) + pair;
+        w[cb] = *reinterpret_cast<const uint4*>(&bs[c * TK + swz_b<GB>(c, g) * 16]);
       }
-    };
-    uint4 af[2][RB];
-    uint4 bw[2][CB];
-    lda_frags(af[0], 0);
-    ld_bytes(bw[0], 0);
+    }
+  };
+  // One step on the stage at cur; nxt is the next step's stage.  The
+  // sub-steps are software-pipelined: the fragments of sub-step s+1 are
+  // requested before the MFMAs of sub-step s.  Before the MFMAs of the last
+  // sub-step, whose operands are then in registers, the step hands over:
+  // wait for the next stage's loads, barrier (after which no wave reads cur
+  // any more), refill cur with step st+3 and request the next step's first
+  // fragments from nxt.  Neither the barrier nor the next step's LDS latency
+  // then leaves the matrix pipe without queued work.
+  auto step = [&](uint8_t* cur, const uint8_t* nxt, int st) {
+    uint32_t bsel = 0;
+    if constexpr (MODE == 0) {
+      const int r = st % inner;
+      bsel = (uint32_t)(r - (r / CCM_NA) * CCM_NA);
+    }
 #pragma unroll
     for (int sub = 0; sub < SUBS; sub++) {
       if (sub + 1 < SUBS) {
-        lda_frags(af[(sub + 1) & 1], sub + 1);
-        if (((sub + 1) & 1) == 0) ld_bytes(bw[((sub + 1) >> 1) & 1], (sub + 1) >> 1);
+        lda_frags(cur, af[(sub + 1) & 1], sub + 1);
+        if (((sub + 1) & 1) == 0) ld_bytes(cur, bw[((sub + 1) >> 1) & 1], (sub + 1) >> 1);
       }
       bf16x8_t bfrag[CB];
 #pragma unroll
@@ -534,6 +551,31 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
           bfrag[cb] = onehot8(v, bval[cb]);
         }
       }
+      if (sub == SUBS - 1) {
+        // loads in flight: those of steps st+1 and st+2 (and, in MODE 0, a
+        // block's bytes issued before the latter).  The refill and the LDS
+        // reads below are unconditional: past the last step the refill
+        // repeats the last step's loads into a stage nobody reads and the
+        // reads go unused, so that no path without them makes the compiler
+        // wait for all loads before the LDS reads or this sub-step's MFMAs.
+        ccm_wait_vmcnt<PER_STEP>();
+        ccm_lds_barrier();
+        if constexpr (MODE == 0) {
+          // the next step opens a K block: spread its bytes (loaded a block
+          // earlier) and request the following block's
+          const int kn = (st + 1) / inner;
+          if (st + 1 < nsteps && st + 1 == kn * inner) {
+            spread_bytes();
+            if (kn + 1 < kblocks) loadb(bcur, kn + 1);
+          }
+        }
+        lda_frags(nxt, af[0], 0);
+        ld_bytes(nxt, bw[0], 0);
+        // keep the reads ahead of the refill: with the refill's loads in
+        // flight the compiler cannot tell the two stages apart and waits
+        __builtin_amdgcn_sched_barrier(0);
+        issue(cur, st + 3 < nsteps ? st + 3 : nsteps - 1);
+      }
 #pragma unroll
       for (int rb = 0; rb < RB; rb++) {
         const bf16x8_t afrag = __builtin_bit_cast(bf16x8_t, af[sub & 1][rb]);
@@ -543,50 +585,35 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
       }
     }
   };
-  // one step: wait for its loads, hand the stage over, refill the stage
-  // consumed by the previous step, compute.  The stages are distinct
-  // restrict-qualified pointers, which tells the compiler that the reads of
-  // the current stage do not depend on the loads filling the next one.
-  auto step = [&](const uint8_t* __restrict__ cur, uint8_t* __restrict__ nxt, int st) {
-    if (st + S - 2 < nsteps)
-      ccm_wait_vmcnt<(S - 2) * PER_STEP>();
-    else
-      ccm_wait_vmcnt<0>();
-    ccm_lds_barrier();
-    uint32_t bsel = 0;
-    if constexpr (MODE == 0) {
-      // first step of a K block: spread its bytes (loaded a block earlier,
-      // retired by the wait above) and request the next block's
-      const int kc = st / inner, r = st - kc * inner;
-      bsel = (uint32_t)(r - (r / CCM_NA) * CCM_NA);
-      if (r == 0) {
-        spread_bytes();
-        if (kc + 1 < kblocks) loadb(bcur, kc + 1);
-      }
-    }
-    if (st + S - 1 < nsteps) issue(nxt, st + S - 1);
-    compute(cur, bsel);
-  };
 
   constexpr int STAGE_BYTES = A_BYTES + TNB;
   uint8_t* s0 = dtl_stage<STAGE_BYTES, 0>();
   uint8_t* s1 = dtl_stage<STAGE_BYTES, 1>();
   uint8_t* s2 = dtl_stage<STAGE_BYTES, 2>();
+  // prologue: all three stages requested, drained, the first fragments read
   issue(s0, 0);
   if (nsteps > 1) issue(s1, 1);
+  if (nsteps > 2) issue(s2, 2);
   if constexpr (MODE == 0) loadb(bcur, 0);
-  // the prologue's loads are complete before the loop, so that the only
-  // loads in flight inside it are those the steps issue
   ccm_wait_vmcnt<0>();
+  ccm_lds_barrier();
+  if constexpr (MODE == 0) {
+    spread_bytes();
+    if (kblocks > 1) loadb(bcur, 1);
+  }
+  lda_frags(s0, af[0], 0);
+  ld_bytes(s0, bw[0], 0);
   // one flat loop over the steps, three (one per stage) per iteration: a
   // nested loop over K blocks made the compiler lose track of which stage
   // the loads in flight at the inner loop's head write, and wait for all of
   // them before the first step's LDS reads
   for (int st = 0; st < nsteps; st += 3) {
-    step(s0, s2, st);
-    if (st + 1 < nsteps) step(s1, s0, st + 1);
-    if (st + 2 < nsteps) step(s2, s1, st + 2);
+    step(s0, s1, st);
+    if (st + 1 < nsteps) step(s1, s2, st + 1);
+    if (st + 2 < nsteps) step(s2, s0, st + 2);
   }
+  // the refills past the last step complete before the workgroup ends
+  ccm_wait_vmcnt<0>();
 
 #pragma unroll
   for (int rb = 0; rb < RB; rb++) {
