@@ -254,15 +254,16 @@ __global__ __launch_bounds__(256) void k_reg_only(const float* __restrict__ x, l
 }
 
 // Neumaier-compensated FP64 sum of up to three partial arrays, fixed order:
-// thread t sums elements t, t + 256, ... of the concatenation a|b|c (eight
-// loads issued before their adds), then wave 0 folds the 256 thread sums,
-// four per lane, and a lane tree combines the 64 (sum, compensation) pairs
-// with exact two-sums.  out[0] = total (double)
+// thread t of SUM_NT sums elements t, t + SUM_NT, ... of the concatenation
+// a|b|c (eight loads issued before their adds), then wave 0 folds the
+// SUM_NT thread sums, SUM_NT/64 per lane, and a lane tree combines the 64
+// (sum, compensation) pairs with exact two-sums.  out[0] = total (double)
+#define SUM_NT 1024
 __device__ __forceinline__ void sum3_block(const double* __restrict__ a, int na,
                                            const double* __restrict__ b, int nb,
                                            const double* __restrict__ c, int nc,
                                            double* __restrict__ out) {
-  __shared__ double sh[256], shc[256];
+  __shared__ double sh[SUM_NT], shc[SUM_NT];
   double s = 0.0, comp = 0.0;
   auto add = [&](double v) {
     const double t = s + v;
@@ -273,28 +274,29 @@ __device__ __forceinline__ void sum3_block(const double* __restrict__ a, int na,
     s = t;
   };
   const int n = na + nb + nc;
-  for (int i0 = threadIdx.x; i0 < n; i0 += 256 * 8) {
+  for (int i0 = threadIdx.x; i0 < n; i0 += SUM_NT * 8) {
     double v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      const int i = i0 + u * 256;
+      const int i = i0 + u * SUM_NT;
       v[u] = i < na ? a[i] : i < na + nb ? b[i - na] : i < n ? c[i - na - nb] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < 8; u++)
-      if (i0 + u * 256 < n) add(v[u]);
+      if (i0 + u * SUM_NT < n) add(v[u]);
   }
   sh[threadIdx.x] = s;
   shc[threadIdx.x] = comp;
   __syncthreads();
   if (threadIdx.x < 64) {
+    constexpr int PER = SUM_NT / 64;
     const int l = threadIdx.x;
     s = 0.0;
     comp = 0.0;
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      add(sh[4 * l + u]);
-      comp += shc[4 * l + u];
+    for (int u = 0; u < PER; u++) {
+      add(sh[PER * l + u]);
+      comp += shc[PER * l + u];
     }
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
@@ -308,16 +310,16 @@ __device__ __forceinline__ void sum3_block(const double* __restrict__ a, int na,
   }
 }
 
-__global__ __launch_bounds__(256) void k_sum3(const double* __restrict__ a, int na,
-                                              const double* __restrict__ b, int nb,
-                                              const double* __restrict__ c, int nc,
-                                              double* __restrict__ out) {
+__global__ __launch_bounds__(SUM_NT) void k_sum3(const double* __restrict__ a, int na,
+                                                 const double* __restrict__ b, int nb,
+                                                 const double* __restrict__ c, int nc,
+                                                 double* __restrict__ out) {
   sum3_block(a, na, b, nb, c, nc, out);
 }
 
 // one block per alignment of a group: f = softmax partials + pair and
 // single regularisation partials, into the alignment's control block
-__global__ __launch_bounds__(256) void k_sum3_grouped(const ccm_bprob* __restrict__ probs,
+__global__ __launch_bounds__(SUM_NT) void k_sum3_grouped(const ccm_bprob* __restrict__ probs,
                                                       const int32_t* __restrict__ pidx) {
   const ccm_bprob& B = probs[pidx[blockIdx.x]];
   if (!B.ctl->eval_on) return;
@@ -388,7 +390,7 @@ int ccm_launch_assemble(const float* d_m, long ldm, int Lm, const float* d_hsym,
   hipLaunchKernelGGL(k_assemble, dim3(ccm_assemble_blocks(L)), dim3(256), 0, s, d_m, ldm, Lm, d_hsym, d_x,
                      d_g, L, ccm_nsp(L), lam1, lam2, d_g1part, n_g1blk, d_regpair, d_regsingle);
   CCM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_sum3, dim3(1), dim3(256), 0, s, d_fpart, n_fpart, d_regpair, ccm_pair_epilogue_blocks(L),
+  hipLaunchKernelGGL(k_sum3, dim3(1), dim3(SUM_NT), 0, s, d_fpart, n_fpart, d_regpair, ccm_pair_epilogue_blocks(L),
                      d_regsingle, ccm_singles_blocks(L), d_f);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
@@ -447,7 +449,7 @@ int ccm_launch_reg_only(const float* d_x, int L, float lam1, float lam2, double*
 
 int ccm_launch_sum3(const double* a, int na, const double* b, int nb, const double* c, int nc,
                     double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_sum3, dim3(1), dim3(256), 0, s, a, na, b, nb, c, nc, out);
+  hipLaunchKernelGGL(k_sum3, dim3(1), dim3(SUM_NT), 0, s, a, na, b, nb, c, nc, out);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }
@@ -470,7 +472,7 @@ int ccm_launch_singles_finalize_grouped(const ccm_bprob* d_probs, ccm_group g, h
 // g.total is ignored: one block per group entry
 int ccm_launch_sum3_grouped(const ccm_bprob* d_probs, ccm_group g, hipStream_t s) {
   if (g.np <= 0) return CCM_ERR_ARG;
-  hipLaunchKernelGGL(k_sum3_grouped, dim3(g.np), dim3(256), 0, s, d_probs, g.pidx);
+  hipLaunchKernelGGL(k_sum3_grouped, dim3(g.np), dim3(SUM_NT), 0, s, d_probs, g.pidx);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }
