@@ -15,8 +15,33 @@ __device__ __forceinline__ void dots_partial_block(const float* __restrict__ a,
                                                    uint32_t nblk) {
   __shared__ double red[3][4];
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  const long stride = (long)nblk * 256;
-  for (long i = (long)blk * 256 + threadIdx.x; i < n; i += stride) {
+  // 16-byte loads over the first 4*(n/4) elements when every vector is
+  // 16-byte aligned (else all elements take the scalar loop), then the
+  // tail; a second operand equal to a is not read again
+  const bool b1a = b1 == a;
+  const bool al = (((uintptr_t)a | (uintptr_t)b0 | (uintptr_t)(nd > 1 ? b1 : a) |
+                    (uintptr_t)(nd > 2 ? b2 : a)) & 15u) == 0;
+  const long n4 = al ? n >> 2 : 0, stride = (long)nblk * 256;
+  const float4* a4 = reinterpret_cast<const float4*>(a);
+  const float4* b04 = reinterpret_cast<const float4*>(b0);
+  const float4* b14 = reinterpret_cast<const float4*>(b1);
+  const float4* b24 = reinterpret_cast<const float4*>(b2);
+  for (long i = (long)blk * 256 + threadIdx.x; i < n4; i += stride) {
+    const float4 av = a4[i], v0 = b04[i];
+    s0 += (double)av.x * (double)v0.x + (double)av.y * (double)v0.y + (double)av.z * (double)v0.z +
+          (double)av.w * (double)v0.w;
+    if (nd > 1) {
+      const float4 v1 = b1a ? av : b14[i];
+      s1 += (double)av.x * (double)v1.x + (double)av.y * (double)v1.y + (double)av.z * (double)v1.z +
+            (double)av.w * (double)v1.w;
+    }
+    if (nd > 2) {
+      const float4 v2 = b24[i];
+      s2 += (double)av.x * (double)v2.x + (double)av.y * (double)v2.y + (double)av.z * (double)v2.z +
+            (double)av.w * (double)v2.w;
+    }
+  }
+  for (long i = 4 * n4 + (long)blk * 256 + threadIdx.x; i < n; i += stride) {
     const double av = a[i];
     s0 += av * (double)b0[i];
     if (nd > 1) s1 += av * (double)b1[i];
@@ -46,20 +71,37 @@ __global__ __launch_bounds__(256) void k_dots_partial(const float* __restrict__ 
   dots_partial_block(a, b0, b1, b2, n, nd, part, blockIdx.x, gridDim.x);
 }
 
+// fixed-order sum of each dot's block partials: thread t adds partials t,
+// t + 256, ... (all loads issued first), then a wave tree and the four
+// wave sums in order
 __device__ __forceinline__ void dots_final_block(const double* __restrict__ part, int nblk, int nd,
                                                  double* __restrict__ out) {
-  __shared__ double sh[256];
-  for (int d = 0; d < nd; d++) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += 256) s += part[(long)d * nblk + i];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = 0.0;
-      for (int i = 0; i < 256; i++) t += sh[i];
-      out[d] = t;
-    }
-    __syncthreads();
+  __shared__ double sh[3][4];
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int i0 = threadIdx.x; i0 < nblk; i0 += 256 * 4) {
+    double v[3][4];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int i = i0 + u * 256;
+        v[d][u] = (d < nd && i < nblk) ? part[(long)d * nblk + i] : 0.0;
+      }
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int u = 0; u < 4; u++) s[d] += v[d][u];
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    s[d] = ccm_wave_sum_f64(s[d]);
+    if (lane == 0) sh[d][wv] = s[d];
+  }
+  __syncthreads();
+  if (threadIdx.x < nd) {
+    const int d = threadIdx.x;
+    out[d] = ((sh[d][0] + sh[d][1]) + sh[d][2]) + sh[d][3];
   }
 }
 
