@@ -584,6 +584,8 @@ int ccm_batch_autotune(ccm_batch *b, int reps, double *ms_out)
 			goto out;
 		for (int v = 0; v < nv; v++) {
 			int any = 0;
+			if (!ccm_gemm_variant_direct(v))
+				continue;
 			for (int i = 0; i < n; i++) {
 				const ccm_bprob *p = &b->h_probs[i];
 				const int ok = ccm_gemm_variant_ok(v, mode, p->Lp, p->ncols);
@@ -618,6 +620,8 @@ int ccm_batch_autotune(ccm_batch *b, int reps, double *ms_out)
 			if (lo == hi)
 				continue;
 			for (int v = 0; v < nv; v++) {
+				if (!ccm_gemm_variant_direct(v))
+					continue;
 				memcpy(trial, cur, sizeof(int) * (size_t)n);
 				int any = 0;
 				for (int k = lo; k < hi; k++) {

@@ -745,6 +745,7 @@ int ccm_launch_convert_wt(const float* d_x, int L, int Lp, uint16_t* d_wt, long 
 static const int gm_tm[GM_NVAR] = {256, 256, 128, 256, 256, 128};
 static const int gm_tn[GM_NVAR] = {256, 128, 128, 256, 128, 128};
 static const int gm_wgpc[GM_NVAR] = {1, 2, 2, 1, 1, 2};
+static const int gm_direct[GM_NVAR] = {0, 0, 0, 1, 1, 1};
 // throughput of a fully loaded CU relative to variant 3 (measured at
 // L = 250, N = 5000 and L = 300, N = 3000 on MI355X)
 static const float gm_eff[GM_NVAR] = {0.66f, 0.70f, 0.70f, 1.0f, 0.65f, 0.88f};
@@ -856,6 +857,12 @@ static int gemm_default_variant(int M, int Ncol) {
 extern "C" {
 
 int ccm_gemm_variants(void) { return GM_NVAR; }
+
+// whether a variant stages through direct-to-LDS loads (the register-staged
+// ones are slower at every measured shape; the batched autotuner skips them)
+int ccm_gemm_variant_direct(int variant) {
+  return variant >= 0 && variant < GM_NVAR ? gm_direct[variant] : 0;
+}
 
 // tiles of a variant over an M x Ncol output (0 for an unknown variant)
 uint32_t ccm_gemm_tile_count(int variant, int M, int Ncol) {

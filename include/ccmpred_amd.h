@@ -190,8 +190,9 @@ size_t ccm_batch_device_bytes(const ccm_batch *b);
 int ccm_batch_set_variants(ccm_batch *b, const int *fwd, const int *grad);
 int ccm_batch_get_variants(const ccm_batch *b, int *fwd, int *grad);
 /* Autotuner: times the grouped contractions on the device (median of reps)
- * over tile assignments and keeps the fastest; ms[0..1] receive the tuned
- * forward and gradient times (may be NULL). */
+ * over assignments of the direct-to-LDS tile variants and keeps the
+ * fastest; ms[0..1] receive the tuned forward and gradient times (may be
+ * NULL). */
 int ccm_batch_autotune(ccm_batch *b, int reps, double *ms);
 
 /* One grouped evaluation of every alignment at its x: gradients into
