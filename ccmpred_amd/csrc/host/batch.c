@@ -41,7 +41,8 @@
 
 typedef struct {
 	ccm_group g;
-	int variant; /* GEMM groups only */
+	int variant;     /* GEMM groups only */
+	size_t qmap_pos; /* GEMM groups: table position of the tile map */
 } bgroup;
 
 struct ccm_batch {
@@ -129,6 +130,87 @@ static int tab_push(tabbuf *t, uint32_t v)
 	return 0;
 }
 
+/* A row panel of a grouped contraction: the tiles of one alignment that
+ * share a band of output rows (and so the same rows of the dense operand). */
+typedef struct {
+	double work;   /* tiles x K steps */
+	int prob, row; /* position in the group, tile row */
+} panel;
+
+static int panel_cmp(const void *a, const void *b)
+{
+	const panel *x = (const panel *)a, *y = (const panel *)b;
+	if (x->work != y->work)
+		return x->work > y->work ? -1 : 1;
+	if (x->prob != y->prob)
+		return x->prob < y->prob ? -1 : 1;
+	return x->row < y->row ? -1 : (x->row > y->row);
+}
+
+/* Tile map of a grouped contraction.  Workgroups are dispatched to the 8
+ * XCDs round-robin (workgroup b to XCD b % 8), and the tiles of a group
+ * differ in cost (their K length is the alignment's).  Whole row panels go
+ * to XCDs, longest first onto the least-loaded XCD, so that the XCDs get
+ * equal work, a panel's dense rows stay in one L2, and each XCD runs its
+ * heaviest tiles first; qmap[8 i + x] is XCD x's i-th tile (CCM_QMAP_IDLE
+ * past its list).  Returns the map length (the grid), 0 on failure. */
+static uint32_t push_tile_map(tabbuf *t, const ccm_bprob *probs, const int *list, int np, int mode,
+                              int variant, const uint32_t *offs)
+{
+	int tm, tn;
+	if (ccm_gemm_tile_shape(variant, &tm, &tn))
+		return 0;
+	size_t npan = 0;
+	for (int i = 0; i < np; i++)
+		npan += (size_t)((probs[list[i]].P + tm - 1) / tm);
+	const size_t ntiles = offs[np];
+	panel *pan = (panel *)malloc(sizeof(panel) * (npan ? npan : 1));
+	uint32_t *xl = (uint32_t *)malloc(sizeof(uint32_t) * 8 * (ntiles ? ntiles : 1));
+	uint32_t len = 0;
+	size_t cnt[8] = {0};
+	double load[8] = {0};
+	if (!pan || !xl)
+		goto out;
+	npan = 0;
+	for (int i = 0; i < np; i++) {
+		const ccm_bprob *p = &probs[list[i]];
+		const int ncol = mode == 0 ? p->ncols : p->P;
+		const int cols = (ncol + tn - 1) / tn, rows = (p->P + tm - 1) / tm;
+		const double steps = mode == 0 ? (double)p->passes_fwd * p->P / 64.0 : (double)p->passes * p->ncols / 64.0;
+		for (int r = 0; r < rows; r++) {
+			pan[npan].work = steps * cols;
+			pan[npan].prob = i;
+			pan[npan].row = r;
+			npan++;
+		}
+	}
+	qsort(pan, npan, sizeof(panel), panel_cmp);
+	for (size_t k = 0; k < npan; k++) {
+		int x = 0;
+		for (int y = 1; y < 8; y++)
+			if (load[y] < load[x])
+				x = y;
+		const ccm_bprob *p = &probs[list[pan[k].prob]];
+		const int ncol = mode == 0 ? p->ncols : p->P;
+		const int cols = (ncol + tn - 1) / tn;
+		for (int c = 0; c < cols; c++)
+			xl[(size_t)x * ntiles + cnt[x]++] = offs[pan[k].prob] + (uint32_t)(pan[k].row * cols + c);
+		load[x] += pan[k].work;
+	}
+	size_t mx = 0;
+	for (int x = 0; x < 8; x++)
+		mx = cnt[x] > mx ? cnt[x] : mx;
+	for (size_t i = 0; i < mx; i++)
+		for (int x = 0; x < 8; x++)
+			if (tab_push(t, i < cnt[x] ? xl[(size_t)x * ntiles + i] : CCM_QMAP_IDLE))
+				goto out;
+	len = (uint32_t)(8 * mx);
+out:
+	free(pan);
+	free(xl);
+	return len;
+}
+
 /* appends a group over the listed problems; offsets and indices are stored
  * as table positions and turned into device pointers once uploaded */
 static int add_group(tabbuf *t, bgroup *bg, const ccm_bprob *probs, const int *list, int np,
@@ -158,7 +240,23 @@ static int add_group(tabbuf *t, bgroup *bg, const ccm_bprob *probs, const int *l
 	bg->g.pidx = (const int32_t *)(uintptr_t)ix;
 	bg->g.np = np;
 	bg->g.total = acc;
+	bg->g.qmap = NULL;
 	bg->variant = variant;
+	bg->qmap_pos = 0;
+	if (kind < 0 && acc > 0) {
+		/* a copy of the offsets: pushing the map may move the table */
+		uint32_t *ofs = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)(np + 1));
+		if (!ofs)
+			return -1;
+		memcpy(ofs, t->h + o, sizeof(uint32_t) * (size_t)(np + 1));
+		const size_t qp = t->len;
+		const uint32_t grid = push_tile_map(t, probs, list, np, mode, variant, ofs);
+		free(ofs);
+		if (!grid)
+			return -1;
+		bg->qmap_pos = qp + 1; /* 0: no map */
+		bg->g.total = grid;
+	}
 	return 0;
 }
 
@@ -166,6 +264,7 @@ static void fix_group(bgroup *bg, const uint32_t *base)
 {
 	bg->g.offs = base + (uintptr_t)bg->g.offs;
 	bg->g.pidx = (const int32_t *)(base + (uintptr_t)bg->g.pidx);
+	bg->g.qmap = bg->qmap_pos ? base + (bg->qmap_pos - 1) : NULL;
 }
 
 static int build_tables(ccm_batch *b)

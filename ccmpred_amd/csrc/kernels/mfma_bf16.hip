@@ -660,12 +660,18 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot(
 // consecutive tiles (mostly of one alignment).
 //   MODE 0: Zt = Wt . onehot(msap)      (M = P, K = P, cols = ncols)
 //   MODE 1: Gt = Dt . onehot(msapt)^T   (M = P, K = ncols, cols = P)
+// logical tile of a grouped contraction's workgroup: from the launch's tile
+// map when it has one, else the XCD-contiguous order
+__device__ __forceinline__ uint32_t ccm_group_tile(const uint32_t* __restrict__ qmap, uint32_t total) {
+  return qmap ? qmap[blockIdx.x] : ccm_xcd_remap(blockIdx.x, total);
+}
+
 template <int MODE, int WM, int WN, int RB, int CB, int TK, int OCC>
 __global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot_grouped(
     const ccm_bprob* __restrict__ probs, const uint32_t* __restrict__ offs,
-    const int32_t* __restrict__ pidx, int np, uint32_t total) {
+    const int32_t* __restrict__ pidx, int np, uint32_t total, const uint32_t* __restrict__ qmap) {
   constexpr int TN = WN * CB * 32;
-  const uint32_t qid = ccm_xcd_remap(blockIdx.x, total);
+  const uint32_t qid = ccm_group_tile(qmap, total);
   if (qid >= offs[np]) return;
   const int gi = ccm_group_find(offs, np, qid);
   const ccm_bprob& B = probs[pidx[gi]];
@@ -685,9 +691,9 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot_grouped(
 template <int MODE, int WM, int WN, int RB, int CB, int TK, int S, int OCC>
 __global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot_dtl_grouped(
     const ccm_bprob* __restrict__ probs, const uint32_t* __restrict__ offs,
-    const int32_t* __restrict__ pidx, int np, uint32_t total) {
+    const int32_t* __restrict__ pidx, int np, uint32_t total, const uint32_t* __restrict__ qmap) {
   constexpr int TN = WN * CB * 32;
-  const uint32_t qid = ccm_xcd_remap(blockIdx.x, total);
+  const uint32_t qid = ccm_group_tile(qmap, total);
   if (qid >= offs[np]) return;
   const int gi = ccm_group_find(offs, np, qid);
   const ccm_bprob& B = probs[pidx[gi]];
@@ -795,7 +801,8 @@ static int launch_gemm(int variant, const uint16_t* A, long lda, int M, int Kb, 
 template <int MODE>
 static int launch_gemm_grouped(int variant, const ccm_bprob* probs, const ccm_group& g, hipStream_t s) {
   if (variant < 0 || variant >= GM_NVAR || g.np <= 0) return CCM_ERR_ARG;
-  const uint32_t total = round8(g.total);
+  // with a tile map the grid is the map; else padded to whole XCD rounds
+  const uint32_t total = g.qmap ? g.total : round8(g.total);
   const void* fn = nullptr;
   int nthreads = 0;
 #define GG_CASE(V, WM, WN, RB, CB, OCC, DTL)                                                    \
@@ -808,7 +815,8 @@ static int launch_gemm_grouped(int variant, const ccm_bprob* probs, const ccm_gr
   const uint32_t* offs = g.offs;
   const int32_t* pidx = g.pidx;
   int np = g.np;
-  void* args[] = {(void*)&probs, (void*)&offs, (void*)&pidx, (void*)&np, (void*)&total};
+  const uint32_t* qmap = g.qmap;
+  void* args[] = {(void*)&probs, (void*)&offs, (void*)&pidx, (void*)&np, (void*)&total, (void*)&qmap};
   if (hipLaunchKernel(fn, dim3(total), dim3(nthreads), args, 0, s) != hipSuccess) return CCM_ERR_LAUNCH;
   CCM_LAUNCH_CHECK();
   return CCM_OK;
@@ -857,6 +865,14 @@ static int gemm_default_variant(int M, int Ncol) {
 extern "C" {
 
 int ccm_gemm_variants(void) { return GM_NVAR; }
+
+// output tile of a variant (rows x columns); CCM_ERR_ARG for an unknown variant
+int ccm_gemm_tile_shape(int variant, int* tm, int* tn) {
+  if (variant < 0 || variant >= GM_NVAR) return CCM_ERR_ARG;
+  *tm = gm_tm[variant];
+  *tn = gm_tn[variant];
+  return CCM_OK;
+}
 
 // whether a variant stages through direct-to-LDS loads (the register-staged
 // ones are slower at every measured shape; the batched autotuner skips them)

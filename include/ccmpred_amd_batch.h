@@ -64,12 +64,16 @@ typedef struct ccm_bprob {
 
 /* A grouped launch: problems pidx[0..np) get consecutive block ranges
  * [offs[i], offs[i+1]); total is the grid size (offs[np], or more for
- * launches whose grid is padded). */
+ * launches whose grid is padded).  Contractions may carry qmap: workgroup
+ * b then takes logical tile qmap[b] (CCM_QMAP_IDLE: none) instead of the
+ * XCD-contiguous order, and total is the length of qmap. */
+#define CCM_QMAP_IDLE 0xFFFFFFFFu
 typedef struct ccm_group {
 	const uint32_t *offs;
 	const int32_t *pidx;
 	int32_t np;
 	uint32_t total;
+	const uint32_t *qmap;
 } ccm_group;
 
 /* conjugate-gradient parameters as the control kernel reads them */

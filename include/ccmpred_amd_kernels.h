@@ -106,6 +106,7 @@ int ccm_convert_wt_blocks(int Lp);
 uint32_t ccm_gemm_tile_count(int variant, int M, int Ncol);
 int ccm_gemm_variant_ok(int variant, int mode, int Lp, int kcols);
 int ccm_gemm_variant_direct(int variant);
+int ccm_gemm_tile_shape(int variant, int *tm, int *tn);
 int ccm_gemm_default_variant_for(int mode, int Lp, int ncols);
 int ccm_dots_blocks(long n);
 int ccm_cg_vec_blocks(long n);
