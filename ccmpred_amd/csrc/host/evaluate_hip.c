@@ -271,7 +271,8 @@ int init_hip_ex(void *instance, const ccm_hip_config *cfg_in)
 	HIPCHK(dalloc(st, (void **)&st->d_regpair, sizeof(double) * (size_t)st->n_regpair));
 	HIPCHK(dalloc(st, (void **)&st->d_regsingle, sizeof(double) * (size_t)st->n_regsingle));
 	HIPCHK(dalloc(st, (void **)&st->d_f, sizeof(double) * 4));
-	HIPCHK(hipHostMalloc((void **)&st->h_f, sizeof(double) * 4, hipHostMallocDefault));
+	HIPCHK(hipHostMalloc((void **)&st->h_f, sizeof(double) * 4, hipHostMallocMapped | hipHostMallocCoherent));
+	HIPCHK(hipHostGetDevicePointer((void **)&st->dh_f, st->h_f, 0));
 
 	if (cfg.precision == CCM_PRECISION_FP32) {
 		st->ldz = round_up((long)CCM_N_ALPHA * L, 64);
@@ -460,8 +461,9 @@ fail:
 }
 
 /* g from the raw pair gradient and the node-gradient partials, and the
- * objective (softmax partials + regularisation) into d_f, in one launch */
-static int assemble_gradient(ccm_hip_state *st, const float *d_x, float *d_g)
+ * objective (softmax partials + regularisation) into *f_out (device
+ * address) */
+static int assemble_gradient(ccm_hip_state *st, const float *d_x, float *d_g, double *f_out)
 {
 	hipStream_t s = st->stream;
 	const int L = st->L;
@@ -469,7 +471,7 @@ static int assemble_gradient(ccm_hip_state *st, const float *d_x, float *d_g)
 	KCHK(ccm_launch_assemble(fp32 ? st->d_graw : st->d_gt, fp32 ? st->ldg : (long)st->P,
 	                         fp32 ? L : st->Lp, fp32 ? NULL : st->d_hsym, d_x, d_g, L, st->lam1,
 	                         st->lam2, st->d_g1part, st->n_g1blk, st->d_fpart, st->n_fpart,
-	                         st->d_regpair, st->d_regsingle, st->d_f, s));
+	                         st->d_regpair, st->d_regsingle, f_out, s));
 	return 0;
 fail:
 	return -1;
@@ -504,10 +506,11 @@ int ccm_hip_evaluate(void *instance, const float *d_x, float *d_g, int nvar, dou
 	if (run_chunks(st, d_x, RUN_FULL, &step))
 		goto fail;
 	MARK(step++);
-	if (assemble_gradient(st, d_x, d_g))
+	/* the objective's last kernel writes f into mapped host memory: no
+	 * device-to-host copy before the synchronisation */
+	if (assemble_gradient(st, d_x, d_g, st->dh_f))
 		goto fail;
 	MARK(step);
-	HIPCHK(hipMemcpyAsync(st->h_f, st->d_f, sizeof(double), hipMemcpyDeviceToHost, s));
 	HIPCHK(hipStreamSynchronize(s));
 	*f = st->h_f[0];
 	if (st->timing) {
@@ -585,7 +588,7 @@ int ccm_hip_backward(void *instance, const float *d_x, float *d_g, int nvar)
 		fprintf(stderr, "ccmpred_amd: ccm_hip_backward without a preceding ccm_hip_forward\n");
 		return -1;
 	}
-	if (pair_gradient_from_forward(st) || assemble_gradient(st, d_x, d_g))
+	if (pair_gradient_from_forward(st) || assemble_gradient(st, d_x, d_g, st->d_f))
 		return -1;
 	if (hipStreamSynchronize(st->stream) != hipSuccess)
 		return -1;

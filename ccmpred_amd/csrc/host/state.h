@@ -51,7 +51,8 @@ typedef struct ccm_hip_state {
 	double *d_regsingle;
 	int n_regsingle;
 	double *d_f;
-	double *h_f;
+	double *h_f;  /* pinned, mapped and coherent host memory */
+	double *dh_f; /* its device address: the evaluation writes f there */
 	int forward_valid; /* ccm_hip_forward left probabilities for ccm_hip_backward */
 	/* optional per-step timing with HIP events on the instance stream */
 	int timing;
