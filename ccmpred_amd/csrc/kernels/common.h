@@ -53,11 +53,17 @@ __device__ __forceinline__ double ccm_block_sum_f64(double v, double* lds) {
 }
 
 // float -> bf16 (round to nearest even), as raw bits
+// float -> bf16 (round to nearest even), as raw bits: the gfx950 conversion
+// instruction v_cvt_pk_bf16_f32, one value or a packed pair (a in the low
+// half)
+typedef __bf16 ccm_bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float ccm_floatx2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t ccm_f2bf2(float a, float b) {
+  const ccm_floatx2_t v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, ccm_bf16x2_t));
+}
 __device__ __forceinline__ uint16_t ccm_f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  uint32_t r = 0x7FFFu + ((u >> 16) & 1u);
-  if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
-  return (uint16_t)((u + r) >> 16);
+  return __builtin_bit_cast(uint16_t, (__bf16)f);
 }
 
 __device__ __forceinline__ float ccm_bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }

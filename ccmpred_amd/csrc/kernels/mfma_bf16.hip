@@ -52,9 +52,8 @@ __device__ __forceinline__ void convert_wt_block(const float* __restrict__ x2, i
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     const float v0 = T[qr * 16 + 2 * i][cl], v1 = T[qr * 16 + 2 * i + 1][cl];
-    const uint16_t h0 = ccm_f2bf(v0), h1 = ccm_f2bf(v1);
-    hi[i] = (uint32_t)h0 | ((uint32_t)h1 << 16);
-    lo[i] = (uint32_t)ccm_f2bf(v0 - ccm_bf2f(h0)) | ((uint32_t)ccm_f2bf(v1 - ccm_bf2f(h1)) << 16);
+    hi[i] = ccm_f2bf2(v0, v1);
+    lo[i] = ccm_f2bf2(v0 - ccm_bf2f((uint16_t)hi[i]), v1 - ccm_bf2f((uint16_t)(hi[i] >> 16)));
   }
   uint4* dst = reinterpret_cast<uint4*>(wt + (long)c * lda + r0);
   dst[0] = make_uint4(hi[0], hi[1], hi[2], hi[3]);

@@ -140,17 +140,20 @@ __device__ __forceinline__ void softmax_cols_block(const float* __restrict__ Zt,
       zx0 = (a == x0) ? z[a].x : zx0;
       zx1 = (a == x1v) ? z[a].y : zx1;
     }
-    // e_a = exp(z_a - m) once; P_a = e_a / sum
+    // e_a = exp(z_a - m) once; P_a = e_a / sum.  The hardware exponential
+    // and logarithm (relative error ~1e-6 over the arguments that matter)
+    // are far inside the BF16 rounding of P; the FP32 path keeps the
+    // library functions.
     float s0 = 0.f, s1 = 0.f;
 #pragma unroll
     for (int a = 0; a < CCM_NA; a++) {
-      z[a].x = expf(z[a].x - m0);
-      z[a].y = expf(z[a].y - m1);
+      z[a].x = __expf(z[a].x - m0);
+      z[a].y = __expf(z[a].y - m1);
       s0 += z[a].x;
       s1 += z[a].y;
     }
-    if (ok0) fl += (double)(w0 * (m0 + logf(s0) - zx0));
-    if (ok1) fl += (double)(w1 * (m1 + logf(s1) - zx1));
+    if (ok0) fl += (double)(w0 * (m0 + __logf(s0) - zx0));
+    if (ok1) fl += (double)(w1 * (m1 + __logf(s1) - zx1));
     if (mode) {
       // padded sequences (weight 0) store zeros
       const float ws0 = w0 / s0, ws1 = w1 / s1;
@@ -159,11 +162,10 @@ __device__ __forceinline__ void softmax_cols_block(const float* __restrict__ Zt,
 #pragma unroll
       for (int a = 0; a < CCM_NA; a++) {
         const float q0 = z[a].x * ws0, q1 = z[a].y * ws1;
-        const uint16_t h0 = ccm_f2bf(q0), h1 = ccm_f2bf(q1);
-        dcol[a * ds] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+        const uint32_t h = ccm_f2bf2(q0, q1);
+        dcol[a * ds] = h;
         if (passes > 1)
-          dcol[a * ds + lo_off / 2] = (uint32_t)ccm_f2bf(q0 - ccm_bf2f(h0)) |
-                                      ((uint32_t)ccm_f2bf(q1 - ccm_bf2f(h1)) << 16);
+          dcol[a * ds + lo_off / 2] = ccm_f2bf2(q0 - ccm_bf2f((uint16_t)h), q1 - ccm_bf2f((uint16_t)(h >> 16)));
         if (a < 20) g1[a] += (q0 - ((a == x0) ? w0 : 0.f)) + (q1 - ((a == x1v) ? w1 : 0.f));
       }
     }
