@@ -29,16 +29,18 @@ __device__ __forceinline__ void convert_wt_block(const float* __restrict__ x2, i
   const long ld2 = (long)CCM_NAP * L;
   {
     // T[rr][cc] = W at coupling row r = tr*64 + rr, output row c = tc*64 + cc;
-    // the 16 loads of a thread are issued before they are stored
-    const int c = tc * 64 + lane;
-    const int a = c / Lp, j = c % Lp;
+    // Lp being a multiple of 64, a tile lies in one residue block b (rows)
+    // and a (columns), so the divisions are per tile; the 16 loads of a
+    // thread are issued before they are stored
+    const int a = (tc * 64) / Lp, j = tc * 64 - a * Lp + lane;
+    const int b = (tr * 64) / Lp, k0 = tr * 64 - b * Lp;
     const bool cok = a < CCM_NA && j < L;
+    const float* src = x2 + (long)(b * L + k0) * ld2 + a * L + j;
     float v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-      const int r = tr * 64 + wv + 4 * i;
-      const int b = r / Lp, k = r % Lp;
-      v[i] = (cok && k < L) ? x2[(long)(b * L + k) * ld2 + a * L + j] : 0.f;
+      const int rr = wv + 4 * i;
+      v[i] = (cok && k0 + rr < L) ? src[(long)rr * ld2] : 0.f;
     }
 #pragma unroll
     for (int i = 0; i < 16; i++) T[wv + 4 * i][lane] = v[i];
