@@ -13,6 +13,10 @@
 #include "common.h"
 
 #define EP_T 64
+// threads of an assembly block (pair tiles and singles): 8 waves, so that a
+// thread holds 8 rows of a tile and more waves fit on a SIMD
+#define EP_NTH 512
+#define EP_W (EP_NTH / 64)
 
 // ---- symmetric form: one block per pair of mirrored tiles ------------------
 //
@@ -49,10 +53,10 @@ __device__ __forceinline__ void pair_sym_block(const float* __restrict__ M, long
                                                const float* __restrict__ x2, float* __restrict__ g2,
                                                int L, float lam2, double* __restrict__ regpart,
                                                uint32_t blk) {
-  constexpr int RPT = EP_T / 4;  // rows per thread: 4 waves, rows wv, wv + 4, ...
+  constexpr int RPT = EP_T / EP_W;  // rows per thread: rows wv, wv + EP_W, ...
   __shared__ float D1[EP_T][EP_T + 1];
   __shared__ float D2[EP_T][EP_T + 1];
-  __shared__ double red[4];
+  __shared__ double red[EP_W];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int C21 = CCM_NA * L, ld2 = CCM_NAP * L;
   const int NT = ccm_pair_tiles(L);
@@ -65,12 +69,12 @@ __device__ __forceinline__ void pair_sym_block(const float* __restrict__ M, long
     float xv[RPT];
 #pragma unroll
     for (int i = 0; i < RPT; i++) {
-      const int R = R0 + wv + 4 * i;
+      const int R = R0 + wv + EP_W * i;
       xv[i] = R < C21 ? x2[(long)R * ld2 + C] : 0.f;
     }
 #pragma unroll
     for (int i = 0; i < RPT; i++) {
-      const int rr = wv + 4 * i, R = R0 + rr;
+      const int rr = wv + EP_W * i, R = R0 + rr;
       if (R >= C21) continue;
       const int kR = R - (R / L) * L;
       float v = 0.f;
@@ -91,7 +95,7 @@ __device__ __forceinline__ void pair_sym_block(const float* __restrict__ M, long
       float d1[RPT], d2[RPT], h1[RPT];
 #pragma unroll
       for (int i = 0; i < RPT; i++) {
-        const int rr = wv + 4 * i;
+        const int rr = wv + EP_W * i;
         const int Ri = ti * EP_T + rr, Rj = tj * EP_T + rr;
         const bool ok1 = Ri < C21 && Cj < C21;
         d1[i] = ok1 ? M[(long)ccm_phi(Ri, L, Lm) * ldm + pcj] : 0.f;
@@ -104,8 +108,8 @@ __device__ __forceinline__ void pair_sym_block(const float* __restrict__ M, long
       const float h1s = ti == tj ? 0.5f : 1.0f, h2s = ti == tj ? 0.5f : 0.0f;
 #pragma unroll
       for (int i = 0; i < RPT; i++) {
-        D1[wv + 4 * i][lane] = d1[i] + h1s * h1[i];
-        D2[wv + 4 * i][lane] = d2[i] + h2s * h1[i];
+        D1[wv + EP_W * i][lane] = d1[i] + h1s * h1[i];
+        D2[wv + EP_W * i][lane] = d2[i] + h2s * h1[i];
       }
     }
     __syncthreads();
@@ -130,7 +134,12 @@ __device__ __forceinline__ void pair_sym_block(const float* __restrict__ M, long
   reg = ccm_wave_sum_f64(reg);
   if (lane == 0) red[wv] = reg;
   __syncthreads();
-  if (threadIdx.x == 0) regpart[blk] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < EP_W; w++) t += red[w];
+    regpart[blk] = t;
+  }
 }
 
 // blocks of the symmetric pair assembly: tile pairs, then padding tiles
@@ -140,7 +149,7 @@ __host__ __device__ __forceinline__ int ccm_pair_sym_blocks(int L) {
   return NT * (NT + 1) / 2 + NT * (npc > 0 ? npc : 0);
 }
 
-__global__ __launch_bounds__(256) void k_pair_epilogue(const float* __restrict__ M, long ldm, int Lm,
+__global__ __launch_bounds__(EP_NTH) void k_pair_epilogue(const float* __restrict__ M, long ldm, int Lm,
                                                        const float* __restrict__ hsym,
                                                        const float* __restrict__ x2,
                                                        float* __restrict__ g2, int L, float lam2,
@@ -149,7 +158,7 @@ __global__ __launch_bounds__(256) void k_pair_epilogue(const float* __restrict__
 }
 
 // BF16 layout of every alignment of a group: pair_epilogue_blocks(L) blocks each
-__global__ __launch_bounds__(256) void k_pair_epilogue_grouped(const ccm_bprob* __restrict__ probs,
+__global__ __launch_bounds__(EP_NTH) void k_pair_epilogue_grouped(const ccm_bprob* __restrict__ probs,
                                                                const uint32_t* __restrict__ offs,
                                                                const int32_t* __restrict__ pidx, int np) {
   const int gi = ccm_group_find(offs, np, blockIdx.x);
@@ -194,8 +203,8 @@ __device__ __forceinline__ void singles_block(const double* __restrict__ g1part,
                                               const float* __restrict__ x1, float* __restrict__ g1,
                                               int L, long nsp, float lam1, double* __restrict__ regpart,
                                               uint32_t blk) {
-  __shared__ double red[4];
-  const long v = (long)blk * 256 + threadIdx.x;
+  __shared__ double red[EP_W];
+  const long v = (long)blk * EP_NTH + threadIdx.x;
   const long ns = 20L * L;
   double reg = 0.0;
   if (v < ns) {
@@ -211,10 +220,15 @@ __device__ __forceinline__ void singles_block(const double* __restrict__ g1part,
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) red[wv] = reg;
   __syncthreads();
-  if (threadIdx.x == 0) regpart[blk] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < EP_W; w++) t += red[w];
+    regpart[blk] = t;
+  }
 }
 
-__global__ __launch_bounds__(256) void k_singles_finalize(const double* __restrict__ g1part, int nblk,
+__global__ __launch_bounds__(EP_NTH) void k_singles_finalize(const double* __restrict__ g1part, int nblk,
                                                           const float* __restrict__ x1,
                                                           float* __restrict__ g1, int L, long nsp,
                                                           float lam1, double* __restrict__ regpart) {
@@ -222,7 +236,7 @@ __global__ __launch_bounds__(256) void k_singles_finalize(const double* __restri
 }
 
 // every alignment of a group: singles_blocks(L) blocks each
-__global__ __launch_bounds__(256) void k_singles_finalize_grouped(const ccm_bprob* __restrict__ probs,
+__global__ __launch_bounds__(EP_NTH) void k_singles_finalize_grouped(const ccm_bprob* __restrict__ probs,
                                                                   const uint32_t* __restrict__ offs,
                                                                   const int32_t* __restrict__ pidx,
                                                                   int np) {
@@ -350,7 +364,7 @@ __device__ __forceinline__ void assemble_block(const float* __restrict__ M, long
     singles_block(g1part, n_g1blk, x, g, L, nsp, lam1, regsingle, blk - npair);
 }
 
-__global__ __launch_bounds__(256) void k_assemble(const float* __restrict__ M, long ldm, int Lm,
+__global__ __launch_bounds__(EP_NTH) void k_assemble(const float* __restrict__ M, long ldm, int Lm,
                                                   const float* __restrict__ hsym,
                                                   const float* __restrict__ x, float* __restrict__ g,
                                                   int L, long nsp, float lam1, float lam2,
@@ -362,7 +376,7 @@ __global__ __launch_bounds__(256) void k_assemble(const float* __restrict__ M, l
 }
 
 // BF16 layout of every alignment of a group: assemble_blocks(L) blocks each
-__global__ __launch_bounds__(256) void k_assemble_grouped(const ccm_bprob* __restrict__ probs,
+__global__ __launch_bounds__(EP_NTH) void k_assemble_grouped(const ccm_bprob* __restrict__ probs,
                                                           const uint32_t* __restrict__ offs,
                                                           const int32_t* __restrict__ pidx, int np) {
   const int gi = ccm_group_find(offs, np, blockIdx.x);
@@ -387,7 +401,7 @@ int ccm_launch_assemble(const float* d_m, long ldm, int Lm, const float* d_hsym,
                         const double* d_fpart, int n_fpart, double* d_regpair, double* d_regsingle,
                         double* d_f, hipStream_t s) {
   if (L <= 0) return CCM_ERR_ARG;
-  hipLaunchKernelGGL(k_assemble, dim3(ccm_assemble_blocks(L)), dim3(256), 0, s, d_m, ldm, Lm, d_hsym, d_x,
+  hipLaunchKernelGGL(k_assemble, dim3(ccm_assemble_blocks(L)), dim3(EP_NTH), 0, s, d_m, ldm, Lm, d_hsym, d_x,
                      d_g, L, ccm_nsp(L), lam1, lam2, d_g1part, n_g1blk, d_regpair, d_regsingle);
   CCM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_sum3, dim3(1), dim3(SUM_NT), 0, s, d_fpart, n_fpart, d_regpair, ccm_pair_epilogue_blocks(L),
@@ -398,7 +412,7 @@ int ccm_launch_assemble(const float* d_m, long ldm, int Lm, const float* d_hsym,
 
 int ccm_launch_assemble_grouped(const ccm_bprob* d_probs, ccm_group g, hipStream_t s) {
   if (g.np <= 0 || g.total == 0) return CCM_ERR_ARG;
-  hipLaunchKernelGGL(k_assemble_grouped, dim3(g.total), dim3(256), 0, s, d_probs, g.offs, g.pidx, g.np);
+  hipLaunchKernelGGL(k_assemble_grouped, dim3(g.total), dim3(EP_NTH), 0, s, d_probs, g.offs, g.pidx, g.np);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }
@@ -409,12 +423,12 @@ extern "C" {
 
 int ccm_pair_epilogue_blocks(int L) { return ccm_pair_sym_blocks(L); }
 
-int ccm_singles_blocks(int L) { return (int)((ccm_nsp(L) + 255) / 256); }
+int ccm_singles_blocks(int L) { return (int)((ccm_nsp(L) + EP_NTH - 1) / EP_NTH); }
 
 int ccm_launch_pair_epilogue(const float* d_m, long ldm, int Lm, const float* d_hsym, const float* d_x,
                              float* d_g, int L, float lam2, double* d_regpart, hipStream_t s) {
   const long nsp = ccm_nsp(L);
-  hipLaunchKernelGGL(k_pair_epilogue, dim3(ccm_pair_sym_blocks(L)), dim3(256), 0, s, d_m, ldm, Lm, d_hsym,
+  hipLaunchKernelGGL(k_pair_epilogue, dim3(ccm_pair_sym_blocks(L)), dim3(EP_NTH), 0, s, d_m, ldm, Lm, d_hsym,
                      d_x + nsp, d_g + nsp, L, lam2, d_regpart);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
@@ -431,7 +445,7 @@ int ccm_launch_hsym(const float* d_mh, long ldm, float* d_hsym, int L, hipStream
 int ccm_launch_singles_finalize(const double* d_g1part, int nblk, const float* d_x, float* d_g, int L,
                                 float lam1, double* d_regpart, hipStream_t s) {
   const long nsp = ccm_nsp(L);
-  hipLaunchKernelGGL(k_singles_finalize, dim3((unsigned)((nsp + 255) / 256)), dim3(256), 0, s, d_g1part,
+  hipLaunchKernelGGL(k_singles_finalize, dim3((unsigned)ccm_singles_blocks(L)), dim3(EP_NTH), 0, s, d_g1part,
                      nblk, d_x, d_g, L, nsp, lam1, d_regpart);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
@@ -456,14 +470,14 @@ int ccm_launch_sum3(const double* a, int na, const double* b, int nb, const doub
 
 int ccm_launch_pair_epilogue_grouped(const ccm_bprob* d_probs, ccm_group g, hipStream_t s) {
   if (g.np <= 0 || g.total == 0) return CCM_ERR_ARG;
-  hipLaunchKernelGGL(k_pair_epilogue_grouped, dim3(g.total), dim3(256), 0, s, d_probs, g.offs, g.pidx, g.np);
+  hipLaunchKernelGGL(k_pair_epilogue_grouped, dim3(g.total), dim3(EP_NTH), 0, s, d_probs, g.offs, g.pidx, g.np);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }
 
 int ccm_launch_singles_finalize_grouped(const ccm_bprob* d_probs, ccm_group g, hipStream_t s) {
   if (g.np <= 0 || g.total == 0) return CCM_ERR_ARG;
-  hipLaunchKernelGGL(k_singles_finalize_grouped, dim3(g.total), dim3(256), 0, s, d_probs, g.offs, g.pidx,
+  hipLaunchKernelGGL(k_singles_finalize_grouped, dim3(g.total), dim3(EP_NTH), 0, s, d_probs, g.offs, g.pidx,
                      g.np);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
