@@ -355,6 +355,25 @@ __device__ __forceinline__ int swz_b(int c, int g) {
   return g ^ ((c >> 2) & (GB - 1));
 }
 
+// Buffer resources and the buffer form of the direct-to-LDS load.  The
+// resource type and builtins exist in the device compilation only; the
+// host pass, which never runs device code, sees inert stand-ins.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __amdgpu_buffer_rsrc_t ccm_rsrc_t;
+__device__ __forceinline__ ccm_rsrc_t ccm_make_rsrc(const void* base) {
+  // raw buffer, no stride, no bounds limit beyond the 31-bit offsets used
+  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, 0x7FFFFFFF, 0x00020000);
+}
+// 16 bytes per lane from base + voff + soff into LDS at lds + 16 * lane
+__device__ __forceinline__ void ccm_buffer_load_lds16(ccm_rsrc_t r, void* lds, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)lds, 16, voff, soff, 0, 0);
+}
+#else
+typedef const void* ccm_rsrc_t;
+__device__ __forceinline__ ccm_rsrc_t ccm_make_rsrc(const void* base) { return base; }
+__device__ __forceinline__ void ccm_buffer_load_lds16(ccm_rsrc_t, void*, uint32_t, uint32_t) {}
+#endif
+
 // LDS stages of the direct-to-LDS kernel: one __shared__ object per stage
 // (dense tile, then the byte tile), so that the compiler can tell a stage
 // being read from the stages being filled and does not drain all
@@ -393,22 +412,29 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
   // the LDS operand of a DTL load must be wave-uniform
   const int wave_u = __builtin_amdgcn_readfirstlane(wave);
 
-  // per lane and DTL instruction: source row pointer with its granule
-  const uint16_t* asrc[A_INS];
+  // The direct-to-LDS loads use the buffer form (buffer_load ... lds): the
+  // compiler counts the FLAT form (global_load_lds) against the LDS counter
+  // as well and then waits for every LDS read in flight (lgkmcnt(0)) where a
+  // count would do.  The dense operand's resource starts at the tile's first
+  // row, so the per-lane byte offsets (row, granule) fit 32 bits and stay
+  // fixed; a step's column offset goes in the scalar offset.
+  const ccm_rsrc_t arsrc = ccm_make_rsrc(A + (long)m0 * lda);
+  uint32_t aoff[A_INS];
 #pragma unroll
   for (int i = 0; i < A_INS; i++) {
     const int p = (wave * A_INS + i) * 64 + lane;
     const int r = p / G, g = swz_a<G>(r, p % G);
-    const int row = m0 + r < M ? m0 + r : M - 1;  // rows past M are read but never stored
-    asrc[i] = A + (long)row * lda + g * 8;
+    const int row = m0 + r < M ? r : M - 1 - m0;  // rows past M are read but never stored
+    aoff[i] = (uint32_t)(((long)row * lda + g * 8) * 2);
   }
-  const uint8_t* bsrc[B_INS > 0 ? B_INS : 1];
+  const ccm_rsrc_t brsrc = ccm_make_rsrc(bytes);
+  uint32_t boff[B_INS > 0 ? B_INS : 1];
 #pragma unroll
   for (int i = 0; i < B_INS; i++) {
     const int p = (wave * B_INS + i) * 64 + lane;
     const int c = p / GB, g = swz_b<GB>(c, p % GB);
     const int kk = (n0 + c) % Lp;  // msapt row of the column; bb >= 21 past the real rows
-    bsrc[i] = bytes + (long)kk * ldbytes + g * 16;
+    boff[i] = (uint32_t)((long)kk * ldbytes + g * 16);
   }
 
   // per lane and column block: the one-hot selector
@@ -451,19 +477,13 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
   auto issue = [&](uint8_t* __restrict__ sp, int st) {
     uint16_t* as = reinterpret_cast<uint16_t*>(sp);
     uint8_t* bs = sp + A_BYTES;
-    const long c = acol(st);
+    const uint32_t c = (uint32_t)(acol(st) * 2);
 #pragma unroll
-    for (int i = 0; i < A_INS; i++)
-      __builtin_amdgcn_global_load_lds((const void*)(asrc[i] + c),
-                                       (__attribute__((address_space(3))) void*)(as + (wave_u * A_INS + i) * 512),
-                                       16, 0, 0);
+    for (int i = 0; i < A_INS; i++) ccm_buffer_load_lds16(arsrc, as + (wave_u * A_INS + i) * 512, aoff[i], c);
     if constexpr (MODE == 1) {
-      const long nb = (long)(st % kblocks) * TK;
+      const uint32_t nb = (uint32_t)((st % kblocks) * TK);
 #pragma unroll
-      for (int i = 0; i < B_INS; i++)
-        __builtin_amdgcn_global_load_lds((const void*)(bsrc[i] + nb),
-                                         (__attribute__((address_space(3))) void*)(bs + (wave_u * B_INS + i) * 1024),
-                                         16, 0, 0);
+      for (int i = 0; i < B_INS; i++) ccm_buffer_load_lds16(brsrc, bs + (wave_u * B_INS + i) * 1024, boff[i], nb);
     }
   };
   auto loadb = [&](uint4(&bv)[CB][NB16], int kc) {
