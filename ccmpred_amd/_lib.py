@@ -58,7 +58,7 @@ EXPORTED = [
     "ccm_hip_backward", "ccm_hip_device_bytes", "ccm_reg_only_blocks", "ccm_launch_reg_only",
     "ccm_conjugrad_init", "ccm_conjugrad_gpu", "ccm_conjugrad_gpu_ex", "ccm_read_msa",
     "ccm_init_bias", "ccm_sum_submatrices", "ccm_apc", "ccm_normalize", "ccm_write_matrix",
-    "ccm_write_raw", "ccm_read_raw",
+    "ccm_write_raw", "ccm_read_raw", "ccm_write_raw_msgpack",
     "ccm_reweight_words", "ccm_launch_identity_counts", "ccm_launch_weights_from_counts",
     "ccm_launch_fill_f32", "ccm_launch_gather_fwd_f32", "ccm_launch_gather_grad_f32",
     "ccm_launch_neg_weighted_onehot", "ccm_seg_meta_bytes", "ccm_launch_gather_grad_seg_f32", "ccm_softmax_rows_blocks", "ccm_softmax_rows_nblk",

@@ -252,3 +252,106 @@ int ccm_read_raw(const char *filename, int ncol, float *x)
 	fclose(f);
 	return rc;
 }
+
+/* ---- raw couplings as MessagePack ("ccm-1" map) ----
+ * Each value takes the shortest MessagePack form for its size, as the
+ * msgpack-c packer that io.c:105-196 calls does: positive fixint / uint8 /
+ * uint16 / uint32 for non-negative ints, fixstr / str8, fixarray / array16 /
+ * array32, fixmap / map16 / map32, float32 (0xca) for every coupling. */
+static void mp_be(FILE *o, uint32_t v, int nbytes)
+{
+	for (int s = (nbytes - 1) * 8; s >= 0; s -= 8)
+		fputc((int)((v >> s) & 0xFF), o);
+}
+
+static void mp_head(FILE *o, uint32_t n, int fix_base, uint32_t fix_limit, int tag16, int tag32)
+{
+	if (n < fix_limit) {
+		fputc(fix_base | (int)n, o);
+	} else if (n < 65536u) {
+		fputc(tag16, o);
+		mp_be(o, n, 2);
+	} else {
+		fputc(tag32, o);
+		mp_be(o, n, 4);
+	}
+}
+
+static void mp_map(FILE *o, uint32_t n) { mp_head(o, n, 0x80, 16, 0xde, 0xdf); }
+static void mp_array(FILE *o, uint32_t n) { mp_head(o, n, 0x90, 16, 0xdc, 0xdd); }
+
+static void mp_str(FILE *o, const char *s)
+{
+	const uint32_t n = (uint32_t)strlen(s);
+	if (n < 32) {
+		fputc(0xa0 | (int)n, o);
+	} else if (n < 256) {
+		fputc(0xd9, o);
+		fputc((int)n, o);
+	} else {
+		mp_head(o, n, 0, 0, 0xda, 0xdb);
+	}
+	fwrite(s, 1, n, o);
+}
+
+/* non-negative ints only (column indices and ncol) */
+static void mp_uint(FILE *o, uint32_t v)
+{
+	if (v < 128u) {
+		fputc((int)v, o);
+	} else if (v < 256u) {
+		fputc(0xcc, o);
+		fputc((int)v, o);
+	} else if (v < 65536u) {
+		fputc(0xcd, o);
+		mp_be(o, v, 2);
+	} else {
+		fputc(0xce, o);
+		mp_be(o, v, 4);
+	}
+}
+
+static void mp_float(FILE *o, float f)
+{
+	uint32_t u;
+	memcpy(&u, &f, 4);
+	fputc(0xca, o);
+	mp_be(o, u, 4);
+}
+
+int ccm_write_raw_msgpack(FILE *out, const float *x, int ncol)
+{
+	if (!out || !x || ncol < 1)
+		return -1;
+	const long L = ncol;
+	const float *x2 = x + ccm_nsingle_padded(ncol);
+	char key[32];
+	mp_map(out, 4);
+	mp_str(out, "format");
+	mp_str(out, "ccm-1");
+	mp_str(out, "ncol");
+	mp_uint(out, (uint32_t)ncol);
+	mp_str(out, "x_single");
+	mp_array(out, (uint32_t)(L * (NA - 1)));
+	for (long i = 0; i < L; i++)
+		for (int a = 0; a < NA - 1; a++)
+			mp_float(out, x[(long)a * L + i]);
+	mp_str(out, "x_pair");
+	mp_map(out, (uint32_t)(L * (L - 1) / 2));
+	for (long i = 0; i < L; i++)
+		for (long j = i + 1; j < L; j++) {
+			snprintf(key, sizeof key, "%ld/%ld", i, j);
+			mp_str(out, key);
+			mp_map(out, 3);
+			mp_str(out, "i");
+			mp_uint(out, (uint32_t)i);
+			mp_str(out, "j");
+			mp_uint(out, (uint32_t)j);
+			mp_str(out, "x");
+			mp_array(out, NA * NA);
+			for (int a = 0; a < NA; a++)
+				for (int b = 0; b < NA; b++)
+					mp_float(out, x2[X2I(b, j, a, i, L)]);
+		}
+	return ferror(out) ? -1 : 0;
+}

@@ -44,6 +44,7 @@ static void usage(const char *exe, int full)
 		printf("\t-k LASTK  \tSet K parameter for convergence criterion to LASTK [default: 5]\n");
 		printf("\n\t-i INIFILE\tRead initial weights from INIFILE\n");
 		printf("\t-r RAWFILE\tStore raw prediction matrix in RAWFILE\n");
+		printf("\t-b BRAWFLE\tStore raw prediction matrix in msgpack format in BRAWFLE\n");
 		printf("\n\t-w IDTHRES\tSet sequence reweighting identity threshold to IDTHRES "
 		       "[default: 0.8]\n");
 		printf("\t-l LFACTOR\tSet pairwise regularization coefficients to LFACTOR * (L-1) "
@@ -59,7 +60,7 @@ static void usage(const char *exe, int full)
 
 int main(int argc, char **argv)
 {
-	const char *rawfile = NULL, *initfile = NULL, *msafile = NULL, *matfile = NULL;
+	const char *rawfile = NULL, *msgpackfile = NULL, *initfile = NULL, *msafile = NULL, *matfile = NULL;
 	int numiter = 250, use_apc = 1, use_norm = 0, device = 0, cg_k = 5;
 	float lambda_single = 0.01f, lambda_pair_factor = 0.2f, theta = 0.8f;
 	double cg_eps = 0.01;
@@ -71,7 +72,7 @@ int main(int argc, char **argv)
 		if (a[0] == '-' && a[1] && !a[2]) {
 			const char o = a[1];
 			const char *val = NULL;
-			if (strchr("dtnekirwlP", o)) {
+			if (strchr("dtnekirbwlP", o)) {
 				if (i + 1 >= argc)
 					usage(argv[0], 0);
 				val = argv[++i];
@@ -86,6 +87,7 @@ int main(int argc, char **argv)
 			case 'k': cg_k = atoi(val); break;
 			case 'i': initfile = val; break;
 			case 'r': rawfile = val; break;
+			case 'b': msgpackfile = val; break;
 			case 'w': theta = (float)atof(val); break;
 			case 'l': lambda_pair_factor = (float)atof(val); break;
 			case 'A': use_apc = 0; break;
@@ -233,6 +235,20 @@ int main(int argc, char **argv)
 		}
 		ccm_write_raw(rf, x, ncol);
 		fclose(rf);
+	}
+	if (msgpackfile) {
+		printf("Writing msgpack raw output to %s\n", msgpackfile);
+		FILE *bf = fopen(msgpackfile, "w");
+		if (!bf) {
+			printf("Cannot open %s for writing!\n\n", msgpackfile);
+			return 4;
+		}
+		const int wrc = ccm_write_raw_msgpack(bf, x, ncol);
+		fclose(bf);
+		if (wrc != 0) {
+			fprintf(stderr, "\nERROR: writing %s failed\n\n", msgpackfile);
+			return 4;
+		}
 	}
 	float *mat = (float *)malloc(sizeof(float) * (size_t)ncol * ncol);
 	ccm_sum_submatrices(x, mat, ncol);

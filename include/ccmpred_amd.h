@@ -232,6 +232,11 @@ void ccm_normalize(float *mat, int ncol);
 void ccm_write_matrix(FILE *out, const float *mat, int ncol, int nrow);
 /* io.c:31-60 */
 void ccm_write_raw(FILE *out, const float *x, int ncol);
+/* io.c:105-196 (write_raw_msgpack, the -b output): a MessagePack map
+ * {format: "ccm-1", ncol, x_single: [20L float32, i outer], x_pair:
+ * {"i/j": {i, j, x: [441 float32 W(b,j,a,i), a outer]} for i < j}}, without
+ * the optional JSON metadata key.  Returns 0, or -1 on a write error. */
+int ccm_write_raw_msgpack(FILE *out, const float *x, int ncol);
 /* io.c:62-103; fills both mirrored couplings.  Returns 0, or -1 on a
  * malformed file (the reference asserts). */
 int ccm_read_raw(const char *filename, int ncol, float *x);

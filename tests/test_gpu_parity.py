@@ -266,17 +266,53 @@ def test_cg_50_iterations_contacts(prec):
 
 
 def test_cli_1atz(tmp_path):
-    """The reference's only test: the contact matrix of example/1atzA.aln has 75 lines."""
+    """The reference's only test: the contact matrix of example/1atzA.aln has
+    75 lines.  Also: the matrix is the APC map of the couplings the -r raw
+    file holds (bit for bit: both print floats with %.20e), the -b
+    MessagePack file holds the same couplings, and a warm start from the raw
+    file (-i) continues from the same objective."""
+    import ctypes as C
+    msgpack = pytest.importorskip("msgpack")
     out = tmp_path / "1atzA.mat"
     raw = tmp_path / "1atzA.raw"
+    braw = tmp_path / "1atzA.braw"
+    aln = os.path.join(ROOT, "tests", "golden", "1atzA.aln")
     exe = os.path.join(ROOT, "ccmpred_amd", "bin", "ccmpred")
     say("cli 1atzA")
-    r = subprocess.run([exe, "-n", "20", "-r", str(raw), os.path.join(ROOT, "tests", "golden", "1atzA.aln"),
-                        str(out)], capture_output=True, text=True, timeout=300)
+    r = subprocess.run([exe, "-n", "20", "-r", str(raw), "-b", str(braw), aln, str(out)],
+                       capture_output=True, text=True, timeout=300)
     sys.stdout.write(r.stdout[-2000:])
     assert r.returncode == 0, r.stderr
     lines = out.read_text().splitlines()
     assert len(lines) == 75
-    mat = np.array([[float(v) for v in ln.split()] for ln in lines])
+    mat = np.array([[float(v) for v in ln.split()] for ln in lines], dtype=np.float32)
     assert mat.shape == (75, 75) and np.all(np.diag(mat) == 0)
     assert np.allclose(mat, mat.T, atol=1e-5)
+
+    L = 75
+    from ccmpred_amd import _lib
+    lib = _lib.lib()
+    x = np.zeros(P.nvar_padded(L), np.float32)
+    assert lib.ccm_read_raw(str(raw).encode(), L, C.c_void_p(x.ctypes.data)) == 0
+    assert np.array_equal(P.contact_map(x, L, apc=True), mat)
+    doc = msgpack.unpackb(braw.read_bytes(), strict_map_key=False)
+    assert doc["format"] == "ccm-1" and doc["ncol"] == L and len(doc["x_pair"]) == L * (L - 1) // 2
+    X2 = x[P.nsingle_padded(L):].reshape(NA, L, NAP, L)
+    for e in doc["x_pair"].values():
+        blk = np.asarray(e["x"], np.float32).reshape(NA, NA)  # [a][b] = W(b,j,a,i)
+        assert np.array_equal(blk.T, X2[:, e["j"], :NA, e["i"]])
+    singles = np.asarray(doc["x_single"], np.float32).reshape(L, 20).T.ravel()
+    assert np.allclose(singles, x[:20 * L], rtol=1e-5, atol=1e-6)  # the raw text prints singles with %g
+
+    def final_fx(text):
+        return float([ln for ln in text.splitlines() if ln.startswith("Final fx")][0].split("=")[1])
+
+    say("cli 1atzA warm start")
+    out2 = tmp_path / "warm.mat"
+    r2 = subprocess.run([exe, "-n", "2", "-i", str(raw), aln, str(out2)], capture_output=True, text=True,
+                        timeout=300)
+    sys.stdout.write(r2.stdout[-1500:])
+    assert r2.returncode == 0, r2.stderr
+    f1, f2 = final_fx(r.stdout), final_fx(r2.stdout)
+    say(f"  final fx {f1} -> warm start {f2}")
+    assert f2 <= f1 * (1 + 1e-4)
