@@ -118,7 +118,7 @@ def run_batched(args):
     the solve (set-up and autotuning reported beside it)."""
     import torch
     from ccmpred_amd.batch import Batch
-    from ccmpred_amd.pll import PLL, init_bias
+    from ccmpred_amd.pll import PLL, init_bias, contact_map
     msas = batched_workload(args.batch_size)
     t0 = time.perf_counter()
     b = Batch(msas, precision=args.batch_precision)
@@ -135,6 +135,16 @@ def run_batched(args):
     t4 = time.perf_counter()
     print(f"batched: solved in {t4 - t3:.2f} s ({rounds} rounds)", file=sys.stderr, flush=True)
     res = [b.result(i) for i in range(b.n)]
+    # contact maps: formed on the device (only L*L floats copied back) and,
+    # for comparison, from x read back to the host
+    t5 = time.perf_counter()
+    maps = [b.contact_map(i) for i in range(b.n)]
+    t6 = time.perf_counter()
+    maps_host = [contact_map(b.get_x(i), b.ncol[i]) for i in range(b.n)]
+    t7 = time.perf_counter()
+    maps_same = sum(int(np.array_equal(u, v)) for u, v in zip(maps, maps_host))
+    print(f"batched: contact maps {t6 - t5:.2f} s on the device, {t7 - t6:.2f} s via host x", file=sys.stderr,
+          flush=True)
     fwd, grad = b.variants()
     dev_gb = b.device_bytes / 1e9
     b.close()
@@ -158,6 +168,8 @@ def run_batched(args):
         "iterations_done": [r["iterations"] for r in res],
         "status": {str(k): int(sum(r["ret"] == k for r in res)) for k in sorted({r["ret"] for r in res})},
         "effective_tflops": round(flops / (t4 - t3) / 1e12, 1),
+        "contact_maps_device_s": round(t6 - t5, 3), "contact_maps_host_s": round(t7 - t6, 3),
+        "contact_maps_equal": maps_same,
         "device_gb": round(dev_gb, 2),
         "variants_fwd": fwd.tolist(), "variants_grad": grad.tolist(),
     }

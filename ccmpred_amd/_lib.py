@@ -67,9 +67,9 @@ EXPORTED = [
     "ccm_launch_gemm_grad", "ccm_gemm_variants", "ccm_gemm_default_variant", "ccm_pair_epilogue_blocks", "ccm_singles_blocks",
     "ccm_launch_pair_epilogue", "ccm_launch_hsym", "ccm_launch_singles_finalize",
     "ccm_launch_sum3", "ccm_dots_partial_len", "ccm_launch_dots", "ccm_launch_axpy",
-    "ccm_launch_direction", "ccm_launch_negate",
+    "ccm_launch_direction", "ccm_launch_negate", "ccm_contact_map_work_len", "ccm_launch_contact_map",
     "ccm_batch_create", "ccm_batch_destroy", "ccm_batch_size", "ccm_batch_x", "ccm_batch_g",
-    "ccm_batch_write_x", "ccm_batch_read", "ccm_batch_stream", "ccm_batch_device_bytes",
+    "ccm_batch_write_x", "ccm_batch_read", "ccm_batch_contact_map", "ccm_batch_stream", "ccm_batch_device_bytes",
     "ccm_batch_set_variants", "ccm_batch_get_variants", "ccm_batch_autotune", "ccm_batch_evaluate",
     "ccm_batch_solve", "ccm_batch_result",
     "ccm_convert_wt_blocks", "ccm_gemm_tile_count", "ccm_gemm_variant_ok",
@@ -160,6 +160,12 @@ def _declare(L):
     L.ccm_batch_write_x.restype = ip
     L.ccm_batch_read.argtypes = [vp, ip, vp, vp]
     L.ccm_batch_read.restype = ip
+    L.ccm_batch_contact_map.argtypes = [vp, ip, ip, ip, vp]
+    L.ccm_batch_contact_map.restype = ip
+    L.ccm_contact_map_work_len.argtypes = [ip]
+    L.ccm_contact_map_work_len.restype = ip
+    L.ccm_launch_contact_map.argtypes = [vp, ip, ip, ip, vp, vp, vp]
+    L.ccm_launch_contact_map.restype = ip
     L.ccm_batch_stream.argtypes = [vp]
     L.ccm_batch_stream.restype = vp
     L.ccm_batch_device_bytes.argtypes = [vp]

@@ -82,6 +82,15 @@ class Batch:
             raise RuntimeError("ccm_batch_read failed")
         return out
 
+    def contact_map(self, p, apc=True, normalize=False):
+        """The L x L contact map of alignment p's current x, computed on the
+        device (util.c:16-139; equal bit for bit to pll.contact_map on get_x(p))."""
+        L = self.ncol[self._p(p)]
+        out = np.empty((L, L), dtype=np.float32)
+        if _lib.lib().ccm_batch_contact_map(self._h, p, int(apc), int(normalize), out.ctypes.data) != 0:
+            raise RuntimeError("ccm_batch_contact_map failed")
+        return out
+
     def get_g(self, p):
         out = np.empty(self.nvar[self._p(p)], dtype=np.float32)
         if _lib.lib().ccm_batch_read(self._h, p, None, out.ctypes.data) != 0:

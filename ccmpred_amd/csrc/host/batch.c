@@ -63,6 +63,8 @@ struct ccm_batch {
 	bgroup g_fwd[NVAR_MAX], g_grad[NVAR_MAX];
 	int n_fwd, n_grad;
 	int32_t *d_nactive, *h_nactive;
+	float *d_cmap; /* contact map + column means, allocated on first use */
+	size_t cmap_cap;
 	size_t bytes;
 };
 
@@ -408,6 +410,33 @@ int ccm_batch_read(ccm_batch *b, int p, float *host_x, float *host_g)
 	return 0;
 }
 
+int ccm_batch_contact_map(ccm_batch *b, int p, int apc, int normalize, float *host_out)
+{
+	if (!b || p < 0 || p >= b->n || !host_out)
+		return -1;
+	const int L = b->uds[p].ncol;
+	const size_t need = (size_t)L * L + (size_t)ccm_contact_map_work_len(L);
+	if (need > b->cmap_cap) {
+		if (hipStreamSynchronize(b->s) != hipSuccess)
+			return -1;
+		if (b->d_cmap)
+			(void)hipFree(b->d_cmap);
+		b->d_cmap = NULL;
+		b->cmap_cap = 0;
+		if (hipMalloc((void **)&b->d_cmap, need * sizeof(float)) != hipSuccess)
+			return -1;
+		b->cmap_cap = need;
+	}
+	float *d_out = b->d_cmap, *d_work = b->d_cmap + (size_t)L * L;
+	if (ccm_launch_contact_map(b->h_probs[p].x, L, apc, normalize, d_out, d_work, b->s) != 0)
+		return -1;
+	if (hipMemcpyAsync(host_out, d_out, sizeof(float) * (size_t)L * L, hipMemcpyDeviceToHost,
+	                   b->s) != hipSuccess ||
+	    hipStreamSynchronize(b->s) != hipSuccess)
+		return -1;
+	return 0;
+}
+
 void ccm_batch_destroy(ccm_batch *b)
 {
 	if (!b)
@@ -435,6 +464,8 @@ void ccm_batch_destroy(ccm_batch *b)
 		(void)hipFree(b->d_tab);
 	if (b->d_nactive)
 		(void)hipFree(b->d_nactive);
+	if (b->d_cmap)
+		(void)hipFree(b->d_cmap);
 	if (b->h_nactive)
 		(void)hipHostFree(b->h_nactive);
 	if (b->s)

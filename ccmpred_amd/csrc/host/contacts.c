@@ -161,6 +161,8 @@ void ccm_apc(float *mat, int ncol)
 void ccm_normalize(float *mat, int ncol)
 {
 	const long L = ncol;
+	if (L < 2)
+		return; /* no off-diagonal entry (the reference reads past the 1x1 map) */
 	float lo = mat[1], hi = mat[1];
 	for (long i = 0; i < L; i++)
 		for (long j = 0; j < L; j++) {

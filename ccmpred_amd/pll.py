@@ -50,6 +50,26 @@ def contact_map(x: np.ndarray, ncol: int, apc: bool = True, normalize: bool = Fa
     return out
 
 
+def contact_map_device(x, ncol: int, apc: bool = True, normalize: bool = False):
+    """contact_map on the device: x is a CUDA float32 tensor in the padded
+    layout; returns the L x L map as a CUDA tensor (equal bit for bit to
+    contact_map on the same values)."""
+    import torch
+    if not (isinstance(x, torch.Tensor) and x.is_cuda and x.dtype == torch.float32):
+        raise TypeError("x must be a CUDA float32 tensor")
+    if x.numel() != nvar_padded(ncol) or not x.is_contiguous():
+        raise ValueError("x must be a contiguous tensor of nvar_padded(ncol) floats")
+    L = _lib.lib()
+    out = torch.empty((ncol, ncol), dtype=torch.float32, device=x.device)
+    work = torch.empty(max(1, L.ccm_contact_map_work_len(int(ncol))), dtype=torch.float32, device=x.device)
+    stream = torch.cuda.current_stream(x.device).cuda_stream
+    rc = L.ccm_launch_contact_map(x.data_ptr(), int(ncol), int(apc), int(normalize), out.data_ptr(),
+                                  work.data_ptr(), stream)
+    if rc != 0:
+        raise RuntimeError(f"ccm_launch_contact_map failed ({rc})")
+    return out
+
+
 def top_pairs(mat: np.ndarray, num: int, min_sep: int = 6):
     """Highest-scoring (i, j), j - i >= min_sep, ties by (i, j) ascending
     (top_couplings.py:28-39 selection rule)."""

@@ -182,6 +182,10 @@ float *ccm_batch_g(ccm_batch *b, int p);
  * either pointer may be NULL) */
 int ccm_batch_write_x(ccm_batch *b, int p, const float *host_x);
 int ccm_batch_read(ccm_batch *b, int p, float *host_x, float *host_g);
+/* util.c:16-139 on the device for alignment p: the L x L contact map of its
+ * current x (sum_submatrices, then apc and normalize if asked) into host_out,
+ * so that only L*L floats leave the device instead of x. */
+int ccm_batch_contact_map(ccm_batch *b, int p, int apc, int normalize, float *host_out);
 void *ccm_batch_stream(ccm_batch *b);
 size_t ccm_batch_device_bytes(const ccm_batch *b);
 

@@ -95,6 +95,15 @@ int ccm_launch_axpy(float *x, const float *sdir, float alpha, long n, ccm_stream
 int ccm_launch_direction(float *sdir, const float *g, float beta, long n, ccm_stream_t s);
 int ccm_launch_negate(float *sdir, const float *g, long n, ccm_stream_t s);
 
+/* contact scores on the device (util.c:16-139): the L x L map of
+ * sum_submatrices from the couplings of x, then apc (if apc) and the [0,1]
+ * rescaling (if normalize), equal bit for bit to ccm_sum_submatrices /
+ * ccm_apc / ccm_normalize on the same x.  d_work holds
+ * ccm_contact_map_work_len(L) floats (read only when apc). */
+int ccm_contact_map_work_len(int L);
+int ccm_launch_contact_map(const float *d_x, int L, int apc, int normalize, float *d_out,
+                           float *d_work, ccm_stream_t s);
+
 /* Grouped forms for a batch of alignments (descriptors in
  * ccmpred_amd_batch.h).  No counterpart in the reference, which runs one
  * alignment per process.  Each step is one launch over every alignment of

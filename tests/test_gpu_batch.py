@@ -96,6 +96,12 @@ def test_batch_solve_matches_single():
         rounds = b.solve(max_iterations=its, epsilon=0.0, rounds_per_graph=3)
         res = [b.result(i) for i in range(b.n)]
         xb = [b.get_x(i) for i in range(b.n)]
+        # contact maps formed on the device equal the host maps of the read-back x
+        for i in range(b.n):
+            L = msas[i].shape[1]
+            for apc, norm in ((True, False), (False, False), (True, True)):
+                assert np.array_equal(b.contact_map(i, apc=apc, normalize=norm),
+                                      P.contact_map(xb[i], L, apc=apc, normalize=norm)), (i, apc, norm)
     say(f"  rounds={rounds}")
     for i, (m, x) in enumerate(zip(msas, xs)):
         with P.PLL(m, precision="bf16") as p:
