@@ -628,10 +628,28 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
   // nested loop over K blocks made the compiler lose track of which stage
   // the loads in flight at the inner loop's head write, and wait for all of
   // them before the first step's LDS reads
-  for (int st = 0; st < nsteps; st += 3) {
-    step(s0, s1, st);
-    if (st + 1 < nsteps) step(s1, s2, st + 1);
-    if (st + 2 < nsteps) step(s2, s0, st + 2);
+  if constexpr (MODE == 1) {
+    // complete triples without conditions, so that each iteration is one
+    // scheduling region and a step's one-hot build for the next step
+    // interleaves with its last MFMAs; the last one or two steps after it
+    int st = 0;
+    for (; st + 3 <= nsteps; st += 3) {
+      step(s0, s1, st);
+      step(s1, s2, st + 1);
+      step(s2, s0, st + 2);
+    }
+    if (st < nsteps) {
+      step(s0, s1, st);
+      if (st + 1 < nsteps) step(s1, s2, st + 1);
+    }
+  } else {
+    // (the same form for MODE 0, whose K blocks open at every 21st step,
+    // measured slower)
+    for (int st = 0; st < nsteps; st += 3) {
+      step(s0, s1, st);
+      if (st + 1 < nsteps) step(s1, s2, st + 1);
+      if (st + 2 < nsteps) step(s2, s0, st + 2);
+    }
   }
   // the refills past the last step complete before the workgroup ends
   ccm_wait_vmcnt<0>();
