@@ -54,7 +54,7 @@ PROGRESS_T = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
 EXPORTED = [
     "ccm_nsingle_padded", "ccm_nvar_padded", "ccm_hip_default_config", "init_hip", "init_hip_ex",
     "init_hip_bf16", "evaluate_hip", "destroy_hip", "ccm_hip_evaluate", "ccm_hip_stream",
-    "ccm_hip_set_timing", "ccm_hip_get_timing", "ccm_hip_objective", "ccm_hip_forward",
+    "ccm_hip_set_timing", "ccm_hip_get_timing", "ccm_hip_step_names", "ccm_hip_objective", "ccm_hip_forward",
     "ccm_hip_backward", "ccm_hip_device_bytes", "ccm_reg_only_blocks", "ccm_launch_reg_only",
     "ccm_conjugrad_init", "ccm_conjugrad_gpu", "ccm_conjugrad_gpu_ex", "ccm_read_msa",
     "ccm_init_bias", "ccm_sum_submatrices", "ccm_apc", "ccm_normalize", "ccm_write_matrix",
@@ -64,7 +64,9 @@ EXPORTED = [
     "ccm_launch_neg_weighted_onehot", "ccm_seg_meta_bytes", "ccm_launch_gather_grad_seg_f32", "ccm_softmax_rows_blocks", "ccm_softmax_rows_nblk",
     "ccm_softmax_cols_blocks", "ccm_softmax_cols_nblk", "ccm_softmax_cols_granule", "ccm_launch_softmax_rows",
     "ccm_launch_softmax_cols", "ccm_launch_convert_wt", "ccm_launch_gemm_fwd",
-    "ccm_launch_gemm_grad", "ccm_gemm_variants", "ccm_gemm_default_variant", "ccm_pair_epilogue_blocks", "ccm_singles_blocks",
+    "ccm_launch_gemm_grad", "ccm_fwd_softmax_blocks", "ccm_fwd_softmax_default_variant",
+    "ccm_launch_gemm_fwd_softmax", "ccm_fwd_softmax_fits", "ccm_fwd_softmax_tile_count",
+    "ccm_fwd_softmax_tile_rows", "ccm_launch_gemm_fwd_softmax_grouped", "ccm_gemm_variants", "ccm_gemm_default_variant", "ccm_pair_epilogue_blocks", "ccm_singles_blocks",
     "ccm_launch_pair_epilogue", "ccm_launch_hsym", "ccm_launch_singles_finalize",
     "ccm_launch_sum3", "ccm_dots_partial_len", "ccm_launch_dots", "ccm_launch_axpy",
     "ccm_launch_direction", "ccm_launch_negate", "ccm_contact_map_work_len", "ccm_launch_contact_map",
@@ -134,6 +136,8 @@ def _declare(L):
     L.ccm_hip_set_timing.restype = ip
     L.ccm_hip_get_timing.argtypes = [vp, C.POINTER(C.c_double), ip]
     L.ccm_hip_get_timing.restype = ip
+    L.ccm_hip_step_names.argtypes = [vp, C.POINTER(C.c_char_p), ip]
+    L.ccm_hip_step_names.restype = ip
     L.ccm_conjugrad_init.argtypes = [C.POINTER(CGParam)]
     L.ccm_conjugrad_gpu_ex.argtypes = [ip, vp, C.POINTER(C.c_double), EVALUATE_T, vp, vp,
                                        C.POINTER(CGParam), C.POINTER(C.c_int), C.POINTER(C.c_int),

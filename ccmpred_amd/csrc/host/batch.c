@@ -47,6 +47,7 @@ typedef struct {
 
 struct ccm_batch {
 	int n;
+	int fused; /* forward contractions with the softmax fused (every alignment fits) */
 	ccm_hip_config cfg;
 	hipStream_t s;
 	userdata *uds;
@@ -162,9 +163,12 @@ static uint32_t push_tile_map(tabbuf *t, const ccm_bprob *probs, const int *list
 	int tm, tn;
 	if (ccm_gemm_tile_shape(variant, &tm, &tn))
 		return 0;
+	/* mode 2: the fused forward, whose row tiles cover tm / 21 alignment
+	 * columns each */
 	size_t npan = 0;
 	for (int i = 0; i < np; i++)
-		npan += (size_t)((probs[list[i]].P + tm - 1) / tm);
+		npan += (size_t)(mode == 2 ? ccm_fwd_softmax_tile_rows(variant, probs[list[i]].L)
+		                           : (probs[list[i]].P + tm - 1) / tm);
 	const size_t ntiles = offs[np];
 	panel *pan = (panel *)malloc(sizeof(panel) * (npan ? npan : 1));
 	uint32_t *xl = (uint32_t *)malloc(sizeof(uint32_t) * 8 * (ntiles ? ntiles : 1));
@@ -176,9 +180,10 @@ static uint32_t push_tile_map(tabbuf *t, const ccm_bprob *probs, const int *list
 	npan = 0;
 	for (int i = 0; i < np; i++) {
 		const ccm_bprob *p = &probs[list[i]];
-		const int ncol = mode == 0 ? p->ncols : p->P;
-		const int cols = (ncol + tn - 1) / tn, rows = (p->P + tm - 1) / tm;
-		const double steps = mode == 0 ? (double)p->passes_fwd * p->P / 64.0 : (double)p->passes * p->ncols / 64.0;
+		const int ncol = mode != 1 ? p->ncols : p->P;
+		const int cols = (ncol + tn - 1) / tn;
+		const int rows = mode == 2 ? ccm_fwd_softmax_tile_rows(variant, p->L) : (p->P + tm - 1) / tm;
+		const double steps = mode != 1 ? (double)p->passes_fwd * p->P / 64.0 : (double)p->passes * p->ncols / 64.0;
 		for (int r = 0; r < rows; r++) {
 			pan[npan].work = steps * cols;
 			pan[npan].prob = i;
@@ -193,7 +198,7 @@ static uint32_t push_tile_map(tabbuf *t, const ccm_bprob *probs, const int *list
 			if (load[y] < load[x])
 				x = y;
 		const ccm_bprob *p = &probs[list[pan[k].prob]];
-		const int ncol = mode == 0 ? p->ncols : p->P;
+		const int ncol = mode != 1 ? p->ncols : p->P;
 		const int cols = (ncol + tn - 1) / tn;
 		for (int c = 0; c < cols; c++)
 			xl[(size_t)x * ntiles + cnt[x]++] = offs[pan[k].prob] + (uint32_t)(pan[k].row * cols + c);
@@ -226,8 +231,9 @@ static int add_group(tabbuf *t, bgroup *bg, const ccm_bprob *probs, const int *l
 		const ccm_bprob *p = &probs[list[i]];
 		uint32_t nb;
 		if (kind < 0)
-			nb = mode == 0 ? ccm_gemm_tile_count(variant, p->P, p->ncols)
-			               : ccm_gemm_tile_count(variant, p->P, p->P);
+			nb = mode == 2   ? ccm_fwd_softmax_tile_count(variant, p->L, p->ncols)
+			     : mode == 0 ? ccm_gemm_tile_count(variant, p->P, p->ncols)
+			                 : ccm_gemm_tile_count(variant, p->P, p->P);
 		else
 			nb = blocks_of(p, kind);
 		acc += nb;
@@ -300,7 +306,7 @@ static int build_tables(ccm_batch *b)
 			if ((mode == 0 ? b->n_fwd : b->n_grad) >= NVAR_MAX)
 				goto out;
 			bgroup *bg = mode == 0 ? &b->g_fwd[b->n_fwd++] : &b->g_grad[b->n_grad++];
-			if (add_group(&t, bg, b->h_probs, list, np, -1, mode, v))
+			if (add_group(&t, bg, b->h_probs, list, np, -1, mode == 0 && b->fused ? 2 : mode, v))
 				goto out;
 		}
 	}
@@ -498,6 +504,13 @@ int ccm_batch_create(userdata *uds, int nprob, const ccm_hip_config *cfg_in, ccm
 	b->n = nprob;
 	b->cfg = cfg;
 	b->uds = uds;
+	{
+		const int fp = cfg.bf16_fwd_passes ? cfg.bf16_fwd_passes : cfg.bf16_passes;
+		b->fused = 1;
+		for (int i = 0; i < nprob; i++)
+			b->fused &= ccm_fwd_softmax_fits((uds[i].ncol + 63) / 64 * 64, fp, cfg.bf16_passes,
+			                                 (uds[i].nrow + 127) / 128 * 128);
+	}
 	b->h_probs = (ccm_bprob *)calloc((size_t)nprob, sizeof(ccm_bprob));
 	b->var_fwd = (int *)malloc(sizeof(int) * (size_t)nprob);
 	b->var_grad = (int *)malloc(sizeof(int) * (size_t)nprob);
@@ -510,7 +523,7 @@ int ccm_batch_create(userdata *uds, int nprob, const ccm_hip_config *cfg_in, ccm
 	HIPCHK(balloc(b, (void **)&b->d_ctl, sizeof(ccm_cg_ctl) * (size_t)nprob));
 	HIPCHK(hipMemset(b->d_ctl, 0, sizeof(ccm_cg_ctl) * (size_t)nprob));
 	for (int i = 0; i < nprob; i++) {
-		if (init_hip_ex(&uds[i], &cfg) != 0) {
+		if (ccm_init_hip_state(&uds[i], &cfg, b->fused) != 0) {
 			fprintf(stderr, "ccmpred_amd batch: alignment %d failed to initialise\n", i);
 			goto fail;
 		}
@@ -584,8 +597,9 @@ static int enqueue_gemms(ccm_batch *b, int mode)
 	const int ng = mode == 0 ? b->n_fwd : b->n_grad;
 	const bgroup *gs = mode == 0 ? b->g_fwd : b->g_grad;
 	for (int i = 0; i < ng; i++) {
-		int r = mode == 0 ? ccm_launch_gemm_fwd_grouped(b->d_probs, gs[i].g, gs[i].variant, b->s)
-		                  : ccm_launch_gemm_grad_grouped(b->d_probs, gs[i].g, gs[i].variant, b->s);
+		int r = mode == 1  ? ccm_launch_gemm_grad_grouped(b->d_probs, gs[i].g, gs[i].variant, b->s)
+		        : b->fused ? ccm_launch_gemm_fwd_softmax_grouped(b->d_probs, gs[i].g, gs[i].variant, b->s)
+		                   : ccm_launch_gemm_fwd_grouped(b->d_probs, gs[i].g, gs[i].variant, b->s);
 		if (r)
 			return r;
 	}
@@ -596,7 +610,8 @@ static int enqueue_eval(ccm_batch *b)
 {
 	KCHK(ccm_launch_convert_wt_grouped(b->d_probs, b->g_conv.g, b->s));
 	KCHK(enqueue_gemms(b, 0));
-	KCHK(ccm_launch_softmax_cols_grouped(b->d_probs, b->g_soft.g, b->s));
+	if (!b->fused)
+		KCHK(ccm_launch_softmax_cols_grouped(b->d_probs, b->g_soft.g, b->s));
 	KCHK(enqueue_gemms(b, 1));
 	KCHK(ccm_launch_assemble_grouped(b->d_probs, b->g_asm.g, b->s));
 	KCHK(ccm_launch_sum3_grouped(b->d_probs, b->g_all.g, b->s));

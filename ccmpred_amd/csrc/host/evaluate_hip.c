@@ -148,6 +148,11 @@ static const unsigned char *seg_chunk(const ccm_hip_state *st, int n0)
 
 int init_hip_ex(void *instance, const ccm_hip_config *cfg_in)
 {
+	return ccm_init_hip_state(instance, cfg_in, 1);
+}
+
+int ccm_init_hip_state(void *instance, const ccm_hip_config *cfg_in, int allow_fused)
+{
 	userdata *ud = (userdata *)instance;
 	ccm_hip_config cfg;
 	if (cfg_in)
@@ -286,15 +291,24 @@ int init_hip_ex(void *instance, const ccm_hip_config *cfg_in)
 		const int passes = cfg.bf16_passes;
 		st->ldwt = (long)st->cfg.bf16_fwd_passes * P;
 		st->lddt = (long)passes * st->ccols;
+		/* the fused forward addresses the gathered rows of Wt with 32-bit
+		 * offsets */
+		st->fused = allow_fused && ccm_fwd_softmax_fits(Lp, st->cfg.bf16_fwd_passes, passes, st->ccols);
 		HIPCHK(dalloc(st, (void **)&st->d_wt, sizeof(uint16_t) * (size_t)P * st->ldwt));
-		HIPCHK(dalloc(st, (void **)&st->d_zt, sizeof(float) * (size_t)P * st->ccols));
+		if (!st->fused)
+			HIPCHK(dalloc(st, (void **)&st->d_zt, sizeof(float) * (size_t)P * st->ccols));
 		HIPCHK(dalloc(st, (void **)&st->d_dt, sizeof(uint16_t) * (size_t)P * st->lddt));
 		HIPCHK(hipMemsetAsync(st->d_dt, 0, sizeof(uint16_t) * (size_t)P * st->lddt, s));
 		HIPCHK(dalloc(st, (void **)&st->d_gt, sizeof(float) * (size_t)P * P));
 		HIPCHK(dalloc(st, (void **)&st->d_hsym,
 		              sizeof(float) * (size_t)CCM_N_ALPHA * L * CCM_N_ALPHA_PAD * L));
-		st->n_fpart = ccm_softmax_cols_blocks(L, N);
-		st->n_g1blk = ccm_softmax_cols_nblk(N);
+		if (st->fused) {
+			st->n_g1blk = ccm_fwd_softmax_blocks(N);
+			st->n_fpart = st->n_g1blk * L;
+		} else {
+			st->n_fpart = ccm_softmax_cols_blocks(L, N);
+			st->n_g1blk = ccm_softmax_cols_nblk(N);
+		}
 
 		/* exact data term: hsym = -(H2 + H2^T), H2 = A^T diag(w) A, computed
 		 * once with the FP32 scatter kernel over sequence chunks (the
@@ -423,14 +437,23 @@ static int run_chunks(ccm_hip_state *st, const float *d_x, int what, int *step)
 			const int ncols = (int)round_up(nc, 128);
 			if (first)
 				MARK((*step)++);
-			KCHK(ccm_launch_gemm_fwd(st->d_wt, st->ldwt, Lp, st->cfg.bf16_fwd_passes, msap, ncols,
-			                         st->d_zt, (long)ncols, st->cfg.gemm_variant, s));
-			if (first)
-				MARK((*step)++);
-			KCHK(ccm_launch_softmax_cols(st->d_zt, (long)ncols, d_x, st->d_msapt + n0, (long)Np,
-			                             st->d_w + n0, L, Lp, nc, ncols, n0, st->d_dt,
-			                             (long)passes * ncols, (long)ncols, passes, mode,
-			                             st->d_fpart, st->d_g1part, s));
+			if (st->fused) {
+				KCHK(ccm_launch_gemm_fwd_softmax(st->d_wt, st->ldwt, Lp, st->cfg.bf16_fwd_passes,
+				                                 msap, ncols, d_x, st->d_msapt + n0, (long)Np,
+				                                 st->d_w + n0, L, nc, st->d_dt,
+				                                 (long)passes * ncols, (long)ncols, passes, mode,
+				                                 n0 / 128, st->d_fpart, st->d_g1part,
+				                                 st->cfg.gemm_variant, s));
+			} else {
+				KCHK(ccm_launch_gemm_fwd(st->d_wt, st->ldwt, Lp, st->cfg.bf16_fwd_passes, msap,
+				                         ncols, st->d_zt, (long)ncols, st->cfg.gemm_variant, s));
+				if (first)
+					MARK((*step)++);
+				KCHK(ccm_launch_softmax_cols(st->d_zt, (long)ncols, d_x, st->d_msapt + n0,
+				                             (long)Np, st->d_w + n0, L, Lp, nc, ncols, n0,
+				                             st->d_dt, (long)passes * ncols, (long)ncols, passes,
+				                             mode, st->d_fpart, st->d_g1part, s));
+			}
 			if (first)
 				MARK((*step)++);
 			if (what == RUN_FULL)
@@ -612,6 +635,22 @@ int ccm_hip_set_timing(void *instance, int on)
 	return 0;
 fail:
 	return -1;
+}
+
+int ccm_hip_step_names(void *instance, const char **names, int n)
+{
+	static const char *fp32[] = {"gather_fwd", "softmax_rows", "gather_grad", "assemble"};
+	static const char *bf16[] = {"convert_wt", "gemm_fwd", "softmax_cols", "gemm_grad", "assemble"};
+	static const char *fused[] = {"convert_wt", "gemm_fwd_softmax", "gemm_grad", "assemble"};
+	userdata *ud = (userdata *)instance;
+	ccm_hip_state *st = (ccm_hip_state *)ud->extra;
+	if (!st)
+		return -1;
+	const char **v = st->cfg.precision == CCM_PRECISION_FP32 ? fp32 : st->fused ? fused : bf16;
+	const int cnt = st->cfg.precision == CCM_PRECISION_FP32 ? 4 : st->fused ? 4 : 5;
+	for (int i = 0; i < n && i < cnt; i++)
+		names[i] = v[i];
+	return cnt;
 }
 
 int ccm_hip_get_timing(void *instance, double *avg_ms, int n)

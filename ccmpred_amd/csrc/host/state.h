@@ -36,7 +36,8 @@ typedef struct ccm_hip_state {
 	/* BF16 path */
 	uint16_t *d_wt;
 	long ldwt;
-	float *d_zt;    /* P x ccols */
+	int fused;      /* forward contraction with the softmax as its epilogue (no d_zt) */
+	float *d_zt;    /* P x ccols, unfused only */
 	uint16_t *d_dt; /* P x passes*ccols */
 	long lddt;
 	float *d_gt;
@@ -60,5 +61,9 @@ typedef struct ccm_hip_state {
 	double step_ms[CCM_TIMING_STEPS];
 	int timed_calls;
 } ccm_hip_state;
+
+/* init_hip_ex with the choice of the fused forward (the batched form runs
+ * the grouped unfused kernels on the instance's buffers) */
+int ccm_init_hip_state(void *instance, const ccm_hip_config *cfg, int allow_fused);
 
 #endif

@@ -37,6 +37,44 @@ __device__ __forceinline__ float ccm_wave_sum_f32(float v) {
   return v;
 }
 
+// Wave-wide sums through DPP lane moves (VALU only, no LDS round trips as in
+// the shuffles above); the total is valid in lane 63.  Fixed order: the two
+// quad swaps, half-row and row mirrors (every lane then holds its row's
+// sum), row 0 into row 1 and row 2 into row 3 (row_bcast:15), rows 0-1 into
+// rows 2-3 (row_bcast:31).
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ float ccm_dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+
+__device__ __forceinline__ float ccm_wave_sum_dpp_f32(float v) {
+  v += ccm_dpp_f32<0xB1>(v);       // quad_perm [1,0,3,2]
+  v += ccm_dpp_f32<0x4E>(v);       // quad_perm [2,3,0,1]
+  v += ccm_dpp_f32<0x141>(v);      // row_half_mirror
+  v += ccm_dpp_f32<0x140>(v);      // row_mirror
+  v += ccm_dpp_f32<0x142, 0xA>(v); // row_bcast:15 into rows 1 and 3
+  v += ccm_dpp_f32<0x143, 0xC>(v); // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ double ccm_dpp_f64(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWS, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+__device__ __forceinline__ double ccm_wave_sum_dpp_f64(double v) {
+  v += ccm_dpp_f64<0xB1>(v);
+  v += ccm_dpp_f64<0x4E>(v);
+  v += ccm_dpp_f64<0x141>(v);
+  v += ccm_dpp_f64<0x140>(v);
+  v += ccm_dpp_f64<0x142, 0xA>(v);
+  v += ccm_dpp_f64<0x143, 0xC>(v);
+  return v;
+}
+
 // Block-wide FP64 sum with a fixed reduction order; result valid in thread 0.
 template <int NT>
 __device__ __forceinline__ double ccm_block_sum_f64(double v, double* lds) {

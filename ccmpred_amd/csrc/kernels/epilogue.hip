@@ -208,8 +208,17 @@ __device__ __forceinline__ void singles_block(const double* __restrict__ g1part,
   const long ns = 20L * L;
   double reg = 0.0;
   if (v < ns) {
+    // blocks summed in order; eight loads in flight at a time
     double s = 0.0;
-    for (int b = 0; b < nblk; b++) s += g1part[(long)b * ns + v];
+    int b = 0;
+    for (; b + 8 <= nblk; b += 8) {
+      double t[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) t[u] = g1part[(long)(b + u) * ns + v];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += t[u];
+    }
+    for (; b < nblk; b++) s += g1part[(long)b * ns + v];
     const float xv = x1[v];
     g1[v] = (float)s + (2.0f * lam1) * xv;
     reg = (double)(lam1 * xv * xv);
@@ -357,11 +366,13 @@ __device__ __forceinline__ void assemble_block(const float* __restrict__ M, long
                                                const double* __restrict__ g1part, int n_g1blk,
                                                double* __restrict__ regpair,
                                                double* __restrict__ regsingle, uint32_t blk) {
-  const uint32_t npair = (uint32_t)ccm_pair_sym_blocks(L);
-  if (blk < npair)
-    pair_sym_block(M, ldm, Lm, hsym, x + nsp, g + nsp, L, lam2, regpair, blk);
+  // the few singles blocks first, so that their partial sums overlap the
+  // pair blocks instead of trailing them
+  const uint32_t nsing = (uint32_t)((nsp + EP_NTH - 1) / EP_NTH);  // ccm_singles_blocks(L)
+  if (blk < nsing)
+    singles_block(g1part, n_g1blk, x, g, L, nsp, lam1, regsingle, blk);
   else
-    singles_block(g1part, n_g1blk, x, g, L, nsp, lam1, regsingle, blk - npair);
+    pair_sym_block(M, ldm, Lm, hsym, x + nsp, g + nsp, L, lam2, regpair, blk - nsing);
 }
 
 __global__ __launch_bounds__(EP_NTH) void k_assemble(const float* __restrict__ M, long ldm, int Lm,

@@ -368,10 +368,16 @@ __device__ __forceinline__ ccm_rsrc_t ccm_make_rsrc(const void* base) {
 __device__ __forceinline__ void ccm_buffer_load_lds16(ccm_rsrc_t r, void* lds, uint32_t voff, uint32_t soff) {
   __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)lds, 16, voff, soff, 0, 0);
 }
+// one 16-bit store at base + voff + soff (32-bit offsets, no 64-bit address
+// arithmetic per store)
+__device__ __forceinline__ void ccm_buffer_store16(ccm_rsrc_t r, uint16_t v, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b16(v, r, (int)voff, (int)soff, 0);
+}
 #else
 typedef const void* ccm_rsrc_t;
 __device__ __forceinline__ ccm_rsrc_t ccm_make_rsrc(const void* base) { return base; }
 __device__ __forceinline__ void ccm_buffer_load_lds16(ccm_rsrc_t, void*, uint32_t, uint32_t) {}
+__device__ __forceinline__ void ccm_buffer_store16(ccm_rsrc_t, uint16_t, uint32_t, uint32_t) {}
 #endif
 
 // LDS stages of the direct-to-LDS kernel: one __shared__ object per stage
@@ -384,12 +390,32 @@ __device__ __forceinline__ uint8_t* dtl_stage() {
   return s;
 }
 
+// What the fused forward (MODE 2) does with a tile of logits instead of
+// storing it: the column softmax of softmax_cols_block for the tile's
+// sequences, its probability operand Dt and its objective and node-gradient
+// partials, per block of 128 sequences.
+struct ccm_fwd_epi {
+  const float* x1;       // singles V(j,a) = x1[a*L + j]
+  const uint8_t* msapt;  // transposed padded MSA at the chunk's first sequence
+  long ldmt;
+  const float* w;        // weights from the chunk's first sequence
+  int L, Nc;             // alignment columns; real sequences in the chunk
+  uint16_t* dt;          // rows a*Lp + j, lddt apart; lo part at lo_off
+  long lddt, lo_off;
+  int passes, mode;      // Dt passes (1 | 2); mode 0: objective only
+  int blk0;              // global index of the chunk's first 128-sequence block
+  double* fpart;         // [block][L]
+  double* g1part;        // [block][20][L]
+};
+
 template <int MODE, int WM, int WN, int RB, int CB, int TK, int S>
 __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict__ A, long lda, int M,
                                                      int Kb, int passes,
                                                      const uint8_t* __restrict__ bytes, long ldbytes,
                                                      int Lp, int Ncol, float* __restrict__ C, long ldc,
-                                                     int accumulate, int mt, int nt) {
+                                                     int accumulate, int mt, int nt,
+                                                     const ccm_fwd_epi& epi) {
+  constexpr bool FWD = MODE != 1;  // forward contraction (MODE 0: logits out; 2: softmax fused)
   constexpr int NW = WM * WN;
   constexpr int TM = WM * RB * 32, TN = WN * CB * 32;
   constexpr int G = TK / 8;                   // 16-byte granules per dense row
@@ -400,8 +426,8 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
   constexpr int HALF = TK / 2, NB16 = HALF / 16, SUBS = TK / 16;
   constexpr int PER_STEP = A_INS + B_INS;     // vector-memory instructions per step
   static_assert((TM * G) % (64 * NW) == 0, "dense loader mapping");
-  static_assert(MODE == 0 || (TN * GB) % (64 * NW) == 0, "byte loader mapping");
-  static_assert(MODE == 0 || TK == 64, "gradient K step");
+  static_assert(FWD || (TN * GB) % (64 * NW) == 0, "byte loader mapping");
+  static_assert(FWD || TK == 64, "gradient K step");
   static_assert(S == 3, "stages");
   static_assert((S - 2) * PER_STEP < 64, "vmcnt range");
 
@@ -418,14 +444,26 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
   // count would do.  The dense operand's resource starts at the tile's first
   // row, so the per-lane byte offsets (row, granule) fit 32 bits and stay
   // fixed; a step's column offset goes in the scalar offset.
-  const ccm_rsrc_t arsrc = ccm_make_rsrc(A + (long)m0 * lda);
+  // MODE 2 gathers its rows: tile mt holds alignment columns j = mt*JG ..
+  // mt*JG + JG-1 for all 21 residues, local row a*JG + jj <- row a*Lp + j of
+  // the dense operand (rows past 21*JG, and columns j >= L, repeat a row
+  // whose results are dropped), so that every softmax of the tile is local.
+  constexpr int JG = TM / CCM_NA;
+  const ccm_rsrc_t arsrc = ccm_make_rsrc(MODE == 2 ? A : A + (long)m0 * lda);
   uint32_t aoff[A_INS];
 #pragma unroll
   for (int i = 0; i < A_INS; i++) {
     const int p = (wave * A_INS + i) * 64 + lane;
     const int r = p / G, g = swz_a<G>(r, p % G);
-    const int row = m0 + r < M ? r : M - 1 - m0;  // rows past M are read but never stored
-    aoff[i] = (uint32_t)(((long)row * lda + g * 8) * 2);
+    long row;
+    if constexpr (MODE == 2) {
+      const int a = r / JG < CCM_NA ? r / JG : 0;
+      const int j = mt * JG + (r - (r / JG) * JG);
+      row = (long)a * Lp + (j < epi.L ? j : epi.L - 1);
+    } else {
+      row = m0 + r < M ? r : M - 1 - m0;  // rows past M are read but never stored
+    }
+    aoff[i] = (uint32_t)((row * lda + g * 8) * 2);
   }
   const ccm_rsrc_t brsrc = ccm_make_rsrc(bytes);
   uint32_t boff[B_INS > 0 ? B_INS : 1];
@@ -445,7 +483,7 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
   for (int cb = 0; cb < CB; cb++) {
     const int col = n0 + (wn * CB + cb) * 32 + l32;
     colv[cb] = col;
-    if (MODE == 0) {
+    if (FWD) {
       bval[cb] = 0;
       bptr[cb] = bytes + (long)(col < Ncol ? col : Ncol - 1) * ldbytes + q * HALF;
     } else {
@@ -460,11 +498,11 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
 #pragma unroll
     for (int j = 0; j < CB; j++) acc[i][j] = (floatx16)0.0f;
 
-  const int kblocks = (MODE == 0 ? Lp : Kb) / TK;
-  const int inner = MODE == 0 ? passes * CCM_NA : 1;
-  const int nsteps = MODE == 0 ? kblocks * inner : passes * kblocks;
+  const int kblocks = (FWD ? Lp : Kb) / TK;
+  const int inner = FWD ? passes * CCM_NA : 1;
+  const int nsteps = FWD ? kblocks * inner : passes * kblocks;
   auto acol = [&](int st) -> long {
-    if (MODE == 0) {
+    if (FWD) {
       const int kc = st / inner, r = st - kc * inner, pass = r / CCM_NA, bb = r - pass * CCM_NA;
       return (long)pass * Kb + (long)bb * Lp + (long)kc * TK;
     } else {
@@ -493,15 +531,15 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
       for (int i = 0; i < NB16; i++)
         bv[cb][i] = *reinterpret_cast<const uint4*>(bptr[cb] + (long)kc * TK + 16 * i);
   };
-  uint4 bcur[CB][MODE == 0 ? NB16 : 1];
-  // MODE 0: the block's bytes spread for onehot8_spread, once per K block
+  uint4 bcur[CB][FWD ? NB16 : 1];
+  // forward: the block's bytes spread for onehot8_spread, once per K block
   // (at its first step, so that the byte loads overlap the previous steps)
-  uint32_t bsp[CB][MODE == 0 ? NB16 * 8 : 1];
+  uint32_t bsp[CB][FWD ? NB16 * 8 : 1];
   auto spread_bytes = [&]() {
 #pragma unroll
     for (int cb = 0; cb < CB; cb++)
 #pragma unroll
-      for (int i = 0; i < (MODE == 0 ? NB16 : 0); i++) {
+      for (int i = 0; i < (FWD ? NB16 : 0); i++) {
         const uint4 w = bcur[cb][i];
         bsp[cb][i * 8 + 0] = spread_lo(w.x);
         bsp[cb][i * 8 + 1] = spread_hi(w.x);
@@ -527,7 +565,7 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
     }
   };
   // MODE 1: the 16 staged bytes that serve sub-steps 2*pair and 2*pair + 1
-  // (MODE 0 reads the spread bytes held in registers)
+  // (the forward reads the spread bytes held in registers)
   auto ld_bytes = [&](const uint8_t* __restrict__ sp, uint4(&w)[CB], int pair) {
     if constexpr (MODE == 1) {
       const uint8_t* bs = sp + A_BYTES;
@@ -549,7 +587,7 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
   // then leaves the matrix pipe without queued work.
   auto step = [&](uint8_t* cur, const uint8_t* nxt, int st) {
     uint32_t bsel = 0;
-    if constexpr (MODE == 0) {
+    if constexpr (FWD) {
       const int r = st % inner;
       bsel = (uint32_t)(r - (r / CCM_NA) * CCM_NA);
     }
@@ -562,7 +600,7 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
       bf16x8_t bfrag[CB];
 #pragma unroll
       for (int cb = 0; cb < CB; cb++) {
-        if constexpr (MODE == 0) {
+        if constexpr (FWD) {
           const int d = (sub >> 1) * 8 + (sub & 1) * 4;
           bfrag[cb] = onehot8_spread(bsp[cb][d], bsp[cb][d + 1], bsp[cb][d + 2], bsp[cb][d + 3],
                                      bsel * 0x00010001u);
@@ -573,7 +611,7 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
         }
       }
       if (sub == SUBS - 1) {
-        // loads in flight: those of steps st+1 and st+2 (and, in MODE 0, a
+        // loads in flight: those of steps st+1 and st+2 (and, in the forward, a
         // block's bytes issued before the latter).  The refill and the LDS
         // reads below are unconditional: past the last step the refill
         // repeats the last step's loads into a stage nobody reads and the
@@ -581,7 +619,7 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
         // wait for all loads before the LDS reads or this sub-step's MFMAs.
         ccm_wait_vmcnt<PER_STEP>();
         ccm_lds_barrier();
-        if constexpr (MODE == 0) {
+        if constexpr (FWD) {
           // the next step opens a K block: spread its bytes (loaded a block
           // earlier) and request the following block's
           const int kn = (st + 1) / inner;
@@ -615,10 +653,10 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
   issue(s0, 0);
   if (nsteps > 1) issue(s1, 1);
   if (nsteps > 2) issue(s2, 2);
-  if constexpr (MODE == 0) loadb(bcur, 0);
+  if constexpr (FWD) loadb(bcur, 0);
   ccm_wait_vmcnt<0>();
   ccm_lds_barrier();
-  if constexpr (MODE == 0) {
+  if constexpr (FWD) {
     spread_bytes();
     if (kblocks > 1) loadb(bcur, 1);
   }
@@ -643,7 +681,7 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
       if (st + 1 < nsteps) step(s1, s2, st + 1);
     }
   } else {
-    // (the same form for MODE 0, whose K blocks open at every 21st step,
+    // (the same form for the forward, whose K blocks open at every 21st step,
     // measured slower)
     for (int st = 0; st < nsteps; st += 3) {
       step(s0, s1, st);
@@ -654,18 +692,130 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
   // the refills past the last step complete before the workgroup ends
   ccm_wait_vmcnt<0>();
 
+  if constexpr (MODE == 2) {
+    static_assert(CB == 2 && WN % 2 == 0, "fused epilogue: 64 sequences per wave column");
+    // Softmax of the tile's logits in rounds of 64 sequences (the wave
+    // column wn == k): those waves put their two 32-column blocks into two
+    // stages, TM rows of 32 floats each, then wave w takes alignment columns
+    // jj = w, w + NW, .. for the 64 sequences, one per lane.  (Both the
+    // writes and the reads meet two-way bank conflicts; an address swizzle
+    // that avoids them made the compiler keep every write address in a
+    // register.)
+    ccm_lds_barrier();  // no wave reads a stage any more
+    float* const buf0 = reinterpret_cast<float*>(s0);
+    float* const buf1 = reinterpret_cast<float*>(s1);
+    // per round and column jj: the 20 node-gradient sums and the objective;
+    // then the singles V(j, a) of the tile's columns
+    float* const gsum = reinterpret_cast<float*>(s2);
+    double* const fsum = reinterpret_cast<double*>(s2 + WN * JG * 20 * 4);
+    float* const xs = reinterpret_cast<float*>(s2 + WN * JG * 20 * 4 + WN * JG * 8);
+    static_assert(WN * JG * 20 * 4 + WN * JG * 8 + 20 * JG * 4 <= A_BYTES, "epilogue sums fit a stage");
+    for (int e = tid; e < 20 * JG; e += NW * 64) {
+      const int a = e / JG, j = mt * JG + (e - a * JG);
+      xs[e] = j < epi.L ? epi.x1[(long)a * epi.L + j] : 0.f;
+    }
+    const ccm_rsrc_t drsrc = ccm_make_rsrc(epi.dt);
+    const uint32_t dstep = (uint32_t)((long)Lp * epi.lddt * 2), dlo = (uint32_t)(epi.lo_off * 2);
+    auto slot = [](int R, int c) { return (R << 5) + c; };
+    for (int k = 0; k < WN; k++) {
+      if (wn == k) {
 #pragma unroll
-  for (int rb = 0; rb < RB; rb++) {
+        for (int rb = 0; rb < RB; rb++)
 #pragma unroll
-    for (int cb = 0; cb < CB; cb++) {
-      const int col = colv[cb];
-      if (col >= Ncol) continue;
+          for (int cb = 0; cb < CB; cb++)
 #pragma unroll
-      for (int i = 0; i < 16; i++) {
-        const int row = m0 + (wm * RB + rb) * 32 + (i >> 2) * 8 + 4 * q + (i & 3);
-        if (row < M) {
-          float* dst = C + (long)row * ldc + col;
-          *dst = accumulate ? *dst + acc[rb][cb][i] : acc[rb][cb][i];
+            for (int i = 0; i < 16; i++) {
+              const int R = (wm * RB + rb) * 32 + (i >> 2) * 8 + 4 * q + (i & 3);
+              (cb ? buf1 : buf0)[slot(R, l32)] = acc[rb][cb][i];
+            }
+      }
+      __syncthreads();
+      // Ncol is a multiple of 128: a round's 64 sequences are all inside the
+      // chunk or all past it
+      const bool in = n0 + k * 64 < Ncol;
+      for (int jj = wave_u; in && jj < JG; jj += NW) {
+        const int j = mt * JG + jj;
+        if (j >= epi.L) break;
+        const int n = n0 + k * 64 + lane;  // sequence within the chunk
+        const bool ok = n < epi.Nc;
+        const int b = lane >> 5, c = lane & 31;
+        const float* bp = (b ? buf1 : buf0) + jj * 32 + c;
+        float z[CCM_NA];
+#pragma unroll
+        for (int a = 0; a < CCM_NA; a++) {
+          float v = bp[a * JG * 32];
+          if (a < 20) v += xs[a * JG + jj];
+          z[a] = ok ? v : 0.f;  // padding sequences: finite placeholder logits, weight 0
+        }
+        const int xb = ok ? epi.msapt[(long)j * epi.ldmt + n] : 255;
+        const float wv = ok ? epi.w[n] : 0.f;
+        float mx = -INFINITY, zx = 0.f;
+#pragma unroll
+        for (int a = 0; a < CCM_NA; a++) {
+          mx = fmaxf(mx, z[a]);
+          zx = (a == xb) ? z[a] : zx;
+        }
+        float ssum = 0.f;
+#pragma unroll
+        for (int a = 0; a < CCM_NA; a++) {
+          z[a] = __expf(z[a] - mx);
+          ssum += z[a];
+        }
+        double fl = ok ? (double)(wv * (mx + __logf(ssum) - zx)) : 0.0;
+        if (epi.mode) {
+          const float ws = wv / ssum;
+          float g1[20];
+          const uint32_t voff = (uint32_t)(((long)j * epi.lddt + n) * 2);  // row j (residue 0)
+#pragma unroll
+          for (int a = 0; a < CCM_NA; a++) {
+            const float qv = z[a] * ws;
+            const uint16_t h = ccm_f2bf(qv);
+            ccm_buffer_store16(drsrc, h, voff, a * dstep);
+            if (epi.passes > 1) ccm_buffer_store16(drsrc, ccm_f2bf(qv - ccm_bf2f(h)), voff, a * dstep + dlo);
+            if (a < 20) g1[a] = qv - ((a == xb) ? wv : 0.f);
+          }
+          // sums over the round's 64 sequences (lane 63)
+#pragma unroll
+          for (int a = 0; a < 20; a++) g1[a] = ccm_wave_sum_dpp_f32(g1[a]);
+          if (lane == 63) {
+#pragma unroll
+            for (int a = 0; a < 20; a++) gsum[(k * JG + jj) * 20 + a] = g1[a];
+          }
+        }
+        fl = ccm_wave_sum_dpp_f64(fl);
+        if (lane == 63) fsum[k * JG + jj] = fl;
+      }
+      __syncthreads();
+    }
+    // partials per block of 128 sequences: rounds 2h and 2h + 1
+    for (int e = tid; e < (WN / 2) * JG * CCM_NA; e += NW * 64) {
+      const int h = e / (JG * CCM_NA), rem = e - h * (JG * CCM_NA);
+      const int jj = rem / CCM_NA, a = rem - jj * CCM_NA;
+      const int j = mt * JG + jj, nb = n0 + h * 128;
+      if (j >= epi.L || nb >= Ncol) continue;
+      const long gb = (long)epi.blk0 + nb / 128;
+      const int r0 = (2 * h) * JG + jj, r1 = (2 * h + 1) * JG + jj;
+      if (a < 20) {
+        if (epi.mode)
+          epi.g1part[(gb * 20 + a) * epi.L + j] = (double)gsum[r0 * 20 + a] + (double)gsum[r1 * 20 + a];
+      } else {
+        epi.fpart[gb * epi.L + j] = fsum[r0] + fsum[r1];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+#pragma unroll
+      for (int cb = 0; cb < CB; cb++) {
+        const int col = colv[cb];
+        if (col >= Ncol) continue;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+          const int row = m0 + (wm * RB + rb) * 32 + (i >> 2) * 8 + 4 * q + (i & 3);
+          if (row < M) {
+            float* dst = C + (long)row * ldc + col;
+            *dst = accumulate ? *dst + acc[rb][cb][i] : acc[rb][cb][i];
+          }
         }
       }
     }
@@ -676,11 +826,11 @@ template <int MODE, int WM, int WN, int RB, int CB, int TK, int S, int OCC>
 __global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot_dtl(
     const uint16_t* __restrict__ A, long lda, int M, int Kb, int passes,
     const uint8_t* __restrict__ bytes, long ldbytes, int Lp, int Ncol, float* __restrict__ C,
-    long ldc, int accumulate, uint32_t tiles_n, uint32_t tiles, uint32_t total) {
+    long ldc, int accumulate, uint32_t tiles_n, uint32_t tiles, uint32_t total, ccm_fwd_epi epi) {
   const uint32_t qid = ccm_xcd_remap(blockIdx.x, total);
   if (qid >= tiles) return;
   gemm_onehot_tile_dtl<MODE, WM, WN, RB, CB, TK, S>(A, lda, M, Kb, passes, bytes, ldbytes, Lp, Ncol, C,
-                                                    ldc, accumulate, qid / tiles_n, qid % tiles_n);
+                                                    ldc, accumulate, qid / tiles_n, qid % tiles_n, epi);
 }
 
 template <int MODE, int WM, int WN, int RB, int CB, int TK, int OCC>
@@ -741,11 +891,21 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot_dtl_grouped(
   if constexpr (MODE == 0) {
     const uint32_t tiles_n = (uint32_t)((B.ncols + TN - 1) / TN);
     gemm_onehot_tile_dtl<0, WM, WN, RB, CB, TK, S>(B.wt, B.ldwt, B.P, B.P, B.passes_fwd, B.msap, B.Lp, B.Lp,
-                                                   B.ncols, B.zt, B.ncols, 0, t / tiles_n, t % tiles_n);
+                                                   B.ncols, B.zt, B.ncols, 0, t / tiles_n, t % tiles_n,
+                                                   ccm_fwd_epi{});
+  } else if constexpr (MODE == 2) {
+    // the softmax of softmax_cols_grouped as the epilogue (whole alignment
+    // as one chunk)
+    const uint32_t tiles_n = (uint32_t)((B.ncols + TN - 1) / TN);
+    const ccm_fwd_epi epi{B.x,   B.msapt, B.ldmt,          B.w,      B.L, B.N, B.dt, B.lddt,
+                          (long)B.ncols, B.passes, 1, 0, B.fpart, B.g1part};
+    gemm_onehot_tile_dtl<2, WM, WN, RB, CB, TK, S>(B.wt, B.ldwt, B.P, B.P, B.passes_fwd, B.msap, B.Lp, B.Lp,
+                                                   B.ncols, nullptr, 0, 0, t / tiles_n, t % tiles_n, epi);
   } else {
     const uint32_t tiles_n = (uint32_t)((B.P + TN - 1) / TN);
     gemm_onehot_tile_dtl<1, WM, WN, RB, CB, TK, S>(B.dt, B.lddt, B.P, B.ncols, B.passes, B.msapt, B.ldmt,
-                                                   B.Lp, B.P, B.gt, B.P, 0, t / tiles_n, t % tiles_n);
+                                                   B.Lp, B.P, B.gt, B.P, 0, t / tiles_n, t % tiles_n,
+                                                   ccm_fwd_epi{});
   }
 }
 
@@ -798,7 +958,9 @@ static const float gm_eff[GM_NVAR] = {0.66f, 0.70f, 0.70f, 1.0f, 0.65f, 0.88f};
 // kernel of a variant (only the staging form in use is instantiated)
 template <int MODE, int WM, int WN, int RB, int CB, int OCC, int DTL, bool GROUPED>
 static const void* gm_kernel() {
-  if constexpr (GROUPED) {
+  if constexpr (MODE == 2 && !DTL) {
+    return nullptr;  // the fused forward exists in the direct-to-LDS form only
+  } else if constexpr (GROUPED) {
     if constexpr (DTL)
       return (const void*)k_gemm_onehot_dtl_grouped<MODE, WM, WN, RB, CB, GM_TK, 3, OCC>;
     else
@@ -829,9 +991,11 @@ static int launch_gemm(int variant, const uint16_t* A, long lda, int M, int Kb, 
     break;
   switch (variant) { GM_VARIANTS(GM_CASE) }
 #undef GM_CASE
+  ccm_fwd_epi epi{};  // (the register-staged kernels take no epilogue argument)
   void* args[] = {(void*)&A,     (void*)&lda,   (void*)&M,  (void*)&Kb,   (void*)&passes,
                   (void*)&bytes, (void*)&ldbytes, (void*)&Lp, (void*)&Ncol, (void*)&C,
-                  (void*)&ldc,   (void*)&accumulate, (void*)&tiles_n, (void*)&tiles, (void*)&total};
+                  (void*)&ldc,   (void*)&accumulate, (void*)&tiles_n, (void*)&tiles, (void*)&total,
+                  (void*)&epi};
   if (hipLaunchKernel(fn, dim3(total), dim3(nthreads), args, 0, s) != hipSuccess) return CCM_ERR_LAUNCH;
   CCM_LAUNCH_CHECK();
   return CCM_OK;
@@ -851,6 +1015,7 @@ static int launch_gemm_grouped(int variant, const ccm_bprob* probs, const ccm_gr
     break;
   switch (variant) { GM_VARIANTS(GG_CASE) }
 #undef GG_CASE
+  if (!fn) return CCM_ERR_ARG;
   const uint32_t* offs = g.offs;
   const int32_t* pidx = g.pidx;
   int np = g.np;
@@ -961,6 +1126,104 @@ int ccm_launch_gemm_grad_grouped(const ccm_bprob* d_probs, ccm_group g, int vari
 }
 
 int ccm_gemm_default_variant(int M, int Ncol) { return gemm_default_variant(M, Ncol); }
+
+// ---- forward contraction with the column softmax fused (MODE 2) --------
+// Tiles of the direct-to-LDS variants only; a register-staged variant maps
+// to the direct one of the same shape.  Tile rows cover TM / 21 alignment
+// columns, so the row tiles number ceil(L / (TM / 21)).
+static int fused_variant(int variant) { return variant >= 0 && variant < 3 ? variant + 3 : variant; }
+
+static uint32_t fused_tiles(int v, int L, int ncols) {
+  const int jg = gm_tm[v] / CCM_NA;
+  return (uint32_t)(((L + jg - 1) / jg) * ((ncols + gm_tn[v] - 1) / gm_tn[v]));
+}
+
+int ccm_fwd_softmax_blocks(int N) { return N <= 0 ? 0 : (N + 127) / 128; }
+
+uint32_t ccm_fwd_softmax_tile_count(int variant, int L, int ncols) {
+  variant = fused_variant(variant);
+  if (variant < 3 || variant >= GM_NVAR || L < 1 || ncols < 1) return 0;
+  return fused_tiles(variant, L, ncols);
+}
+
+int ccm_fwd_softmax_tile_rows(int variant, int L) {
+  variant = fused_variant(variant);
+  if (variant < 3 || variant >= GM_NVAR || L < 1) return 0;
+  const int jg = gm_tm[variant] / CCM_NA;
+  return (L + jg - 1) / jg;
+}
+
+int ccm_fwd_softmax_fits(int Lp, int passes_fwd, int passes, int ncols) {
+  const long P = (long)CCM_NA * Lp;
+  return (P + 1) * (passes_fwd * P) * 2 <= 0x7FFFFFFFL && P * passes * (long)ncols * 2 <= 0x7FFFFFFFL;
+}
+
+int ccm_launch_gemm_fwd_softmax_grouped(const ccm_bprob* d_probs, ccm_group g, int variant, hipStream_t s) {
+  variant = fused_variant(variant);
+  if (variant < 3 || variant >= GM_NVAR) return CCM_ERR_ARG;
+  return launch_gemm_grouped<2>(variant, d_probs, g, s);
+}
+
+int ccm_fwd_softmax_default_variant(int L, int ncols) {
+  const int cus = device_cus();
+  int best = 3;
+  float best_t = 0.f;
+  const int cand[2] = {3, 5};
+  for (int i = 0; i < 2; i++) {
+    const int v = cand[i];
+    const uint32_t tiles = fused_tiles(v, L, ncols);
+    const uint32_t slots = (uint32_t)(cus * gm_wgpc[v]);
+    const float waves = (float)((tiles + slots - 1) / slots);
+    const float t = waves * (float)(gm_tm[v] * gm_tn[v] * gm_wgpc[v]) / gm_eff[v];
+    if (i == 0 || t < best_t) {
+      best_t = t;
+      best = v;
+    }
+  }
+  return best;
+}
+
+// The logits of ccm_launch_gemm_fwd for the chunk's ncols sequences (never
+// stored) with softmax_cols as the epilogue: Dt (mode 1), the objective
+// partials fpart[blk][j] and the node-gradient partials g1part[blk][a][j]
+// (mode 1) per block blk = blk0 + n / 128 of 128 sequences.
+int ccm_launch_gemm_fwd_softmax(const uint16_t* d_wt, long lda, int Lp, int passes_fwd,
+                                const uint8_t* d_msap, int ncols, const float* d_x1,
+                                const uint8_t* d_msapt, long ldmt, const float* d_w, int L, int Nc,
+                                uint16_t* d_dt, long lddt, long lo_off, int passes, int mode, int blk0,
+                                double* d_fpart, double* d_g1part, int variant, hipStream_t s) {
+  const long P = (long)CCM_NA * Lp;
+  if (Lp % 64 || Lp < L || L < 1 || ncols % 128 || ncols <= 0 || Nc < 1 || Nc > ncols ||
+      lda < passes_fwd * P || lda % 8 || !d_fpart || (mode && (!d_dt || !d_g1part)) || blk0 < 0)
+    return CCM_ERR_ARG;
+  if (mode && (passes < 1 || passes > 2 || lddt < (long)passes * ncols || (passes > 1 && lo_off < ncols)))
+    return CCM_ERR_ARG;
+  // the gathered rows address the whole dense operand, and the stores the
+  // whole of Dt, through 32-bit offsets
+  if ((P + 1) * lda * 2 > 0x7FFFFFFFL || (mode && P * lddt * 2 > 0x7FFFFFFFL)) return CCM_ERR_ARG;
+  variant = variant < 0 ? ccm_fwd_softmax_default_variant(L, ncols) : fused_variant(variant);
+  if (variant < 3 || variant >= GM_NVAR) return CCM_ERR_ARG;
+  ccm_fwd_epi epi{d_x1, d_msapt, ldmt, d_w, L, Nc, d_dt, lddt, lo_off, passes, mode, blk0, d_fpart, d_g1part};
+  const uint32_t tiles_n = (uint32_t)((ncols + gm_tn[variant] - 1) / gm_tn[variant]);
+  const uint32_t tiles = fused_tiles(variant, L, ncols), total = round8(tiles);
+  const int M = (int)P, Kb = (int)P, accumulate = 0;
+  const long ldbytes = Lp, ldc = 0;
+  float* C = nullptr;
+  const void* fn = nullptr;
+  int nthreads = 0;
+  switch (variant) {
+    case 3: fn = (const void*)k_gemm_onehot_dtl<2, 2, 4, 4, 2, GM_TK, 3, 1>; nthreads = 512; break;
+    case 4: fn = (const void*)k_gemm_onehot_dtl<2, 2, 2, 4, 2, GM_TK, 3, 1>; nthreads = 256; break;
+    default: fn = (const void*)k_gemm_onehot_dtl<2, 2, 2, 2, 2, GM_TK, 3, 2>; nthreads = 256; break;
+  }
+  void* args[] = {(void*)&d_wt,   (void*)&lda,     (void*)&M,  (void*)&Kb,    (void*)&passes_fwd,
+                  (void*)&d_msap, (void*)&ldbytes, (void*)&Lp, (void*)&ncols, (void*)&C,
+                  (void*)&ldc,    (void*)&accumulate, (void*)&tiles_n, (void*)&tiles, (void*)&total,
+                  (void*)&epi};
+  if (hipLaunchKernel(fn, dim3(total), dim3(nthreads), args, 0, s) != hipSuccess) return CCM_ERR_LAUNCH;
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
 
 // Zt[c][n] (P x ldc floats, ldc >= ncols) = sum_r Wt[c][r] [x_nk = b] for the
 // ncols (multiple of 128) sequences whose padded MSA rows start at d_msap;

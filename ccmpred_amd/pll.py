@@ -16,8 +16,6 @@ from . import _lib
 
 # precision form -> (precision enum, gradient-operand passes, forward-operand passes)
 PRECISIONS = {"fp32": (0, 1, 1), "bf16": (1, 1, 1), "bf16x2": (1, 2, 2), "bf16f": (1, 1, 2)}
-FP32_STEPS = ["gather_fwd", "softmax_rows", "gather_grad", "assemble"]
-BF16_STEPS = ["convert_wt", "gemm_fwd", "softmax_cols", "gemm_grad", "assemble"]
 
 
 def nsingle_padded(ncol: int) -> int:
@@ -196,10 +194,12 @@ class PLL:
         _lib.lib().ccm_hip_set_timing(self._uptr, 1 if on else 0)
 
     def timing(self):
+        lib = _lib.lib()
+        names = (C.c_char_p * 5)()
+        cnt = lib.ccm_hip_step_names(self._uptr, names, 5)
         out = (C.c_double * 5)()
-        n = _lib.lib().ccm_hip_get_timing(self._uptr, out, 5)
-        names = FP32_STEPS if self.precision == "fp32" else BF16_STEPS
-        return n, {k: out[i] for i, k in enumerate(names)}
+        n = lib.ccm_hip_get_timing(self._uptr, out, 5)
+        return n, {names[i].decode(): out[i] for i in range(cnt)}
 
     def conjugrad(self, x, max_iterations=50, epsilon=0.0, k=5, max_linesearch=5, alpha_mul=0.5,
                   ftol=1e-4, wolfe=0.2):

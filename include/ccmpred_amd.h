@@ -115,9 +115,14 @@ void *ccm_hip_stream(void *instance);
  * clears the accumulators; get_timing fills up to n averages in ms and
  * returns the number of timed evaluations.  Steps, FP32 path: gather_fwd,
  * softmax, gather_grad, assembly (pair epilogue, singles and the objective's
- * sum); BF16 path: convert_wt, gemm_fwd, softmax, gemm_grad, assembly. */
+ * sum); BF16 path: convert_wt, gemm_fwd (with the softmax when fused),
+ * softmax (unfused only), gemm_grad, assembly. */
 int ccm_hip_set_timing(void *instance, int on);
 int ccm_hip_get_timing(void *instance, double *avg_ms, int n);
+/* Names of those steps (static strings; up to n filled); returns their
+ * number.  BF16 with the softmax fused into the forward contraction:
+ * convert_wt, gemm_fwd_softmax, gemm_grad, assemble. */
+int ccm_hip_step_names(void *instance, const char **names, int n);
 
 /* ---- device conjugate gradient (libconjugrad's conjugrad_gpu shape) --- */
 

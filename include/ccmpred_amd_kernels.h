@@ -61,6 +61,23 @@ int ccm_gemm_variants(void);
 int ccm_gemm_default_variant(int M, int Ncol);
 int ccm_launch_gemm_fwd(const uint16_t *d_wt, long lda, int Lp, int passes, const uint8_t *d_msap,
                         int ncols, float *d_zt, long ldc, int variant, ccm_stream_t s);
+/* The forward contraction with the column softmax as its epilogue
+ * (cu:30-107 and cu:114-141 in one launch; no counterpart kernel): the
+ * logits are never stored.  Writes Dt (mode 1, as ccm_launch_softmax_cols),
+ * the objective partials fpart[blk][j] and the node-gradient partials
+ * g1part[blk][a][j] (mode 1) per block blk = blk0 + n / 128 of 128
+ * sequences; ccm_fwd_softmax_blocks(N) blocks cover N sequences. */
+int ccm_fwd_softmax_blocks(int N);
+int ccm_fwd_softmax_default_variant(int L, int ncols);
+/* whether Wt (lda = passes_fwd * 21 * Lp) and Dt (21 * Lp rows of
+ * passes * ncols) fit the 32-bit offsets of the fused forward */
+int ccm_fwd_softmax_fits(int Lp, int passes_fwd, int passes, int ncols);
+int ccm_launch_gemm_fwd_softmax(const uint16_t *d_wt, long lda, int Lp, int passes_fwd,
+                                const uint8_t *d_msap, int ncols, const float *d_x1,
+                                const uint8_t *d_msapt, long ldmt, const float *d_w, int L, int Nc,
+                                uint16_t *d_dt, long lddt, long lo_off, int passes, int mode,
+                                int blk0, double *d_fpart, double *d_g1part, int variant,
+                                ccm_stream_t s);
 int ccm_launch_gemm_grad(const uint16_t *d_dt, long lda, int Lp, int passes,
                          const uint8_t *d_msapt, long ldmt, int kcols, float *d_gt, long ldc,
                          int accumulate, int variant, ccm_stream_t s);
@@ -121,6 +138,14 @@ int ccm_dots_blocks(long n);
 int ccm_cg_vec_blocks(long n);
 int ccm_launch_convert_wt_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
 int ccm_launch_gemm_fwd_grouped(const ccm_bprob *d_probs, ccm_group g, int variant, ccm_stream_t s);
+/* the fused forward over a group (alignments whole, x1 = x): tiles per
+ * alignment ccm_fwd_softmax_tile_count, ccm_fwd_softmax_tile_rows row
+ * panels of them; register-staged variants map to the direct-to-LDS
+ * variant of the same shape */
+uint32_t ccm_fwd_softmax_tile_count(int variant, int L, int ncols);
+int ccm_fwd_softmax_tile_rows(int variant, int L);
+int ccm_launch_gemm_fwd_softmax_grouped(const ccm_bprob *d_probs, ccm_group g, int variant,
+                                        ccm_stream_t s);
 int ccm_launch_gemm_grad_grouped(const ccm_bprob *d_probs, ccm_group g, int variant, ccm_stream_t s);
 int ccm_launch_softmax_cols_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
 int ccm_launch_pair_epilogue_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);

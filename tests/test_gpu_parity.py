@@ -218,9 +218,13 @@ def test_bf16_evaluate_error(prec, tol, L, N, seed):
         f, g = p.evaluate(torch.from_numpy(x).cuda())
         g = g.cpu().numpy()
     e = rel_l2(g, g64)
-    say(f"  f rel={abs(f - f64) / abs(f64):.2e} g rel_l2={e:.2e}")
+    e1 = rel_l2(g[:20 * L], g64[:20 * L])
+    say(f"  f rel={abs(f - f64) / abs(f64):.2e} g rel_l2={e:.2e} singles rel_l2={e1:.2e}")
     assert abs(f - f64) / abs(f64) <= 1e-4
     assert e <= tol
+    # the node gradient on its own (a small part of g): the single-pass BF16
+    # logits leave ~1.5e-3 there; a broken reduction would give O(1) errors
+    assert e1 <= 5 * tol
 
 
 # ---- device CG end to end ------------------------------------------------
