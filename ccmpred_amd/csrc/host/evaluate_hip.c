@@ -170,7 +170,8 @@ int ccm_init_hip_state(void *instance, const ccm_hip_config *cfg_in, int allow_f
 	unsigned char *h_msap = NULL, *h_msapt = NULL, *d_msa_rows = NULL;
 	uint32_t *d_planes = NULL;
 	int32_t *d_counts = NULL;
-	float *d_tmp_d = NULL, *d_tmp_g = NULL;
+	float *d_tmp_d = NULL, *d_tmp_g = NULL, *h_w = NULL;
+	double *h_c = NULL;
 	if (!st)
 		return 1;
 	st->cfg = cfg;
@@ -303,8 +304,9 @@ int ccm_init_hip_state(void *instance, const ccm_hip_config *cfg_in, int allow_f
 		HIPCHK(dalloc(st, (void **)&st->d_hsym,
 		              sizeof(float) * (size_t)CCM_N_ALPHA * L * CCM_N_ALPHA_PAD * L));
 		if (st->fused) {
-			st->n_g1blk = ccm_fwd_softmax_blocks(N);
-			st->n_fpart = st->n_g1blk * L;
+			/* one block per 128 sequences, then the constant data term */
+			st->n_fpart = ccm_fwd_softmax_blocks(N) * L;
+			st->n_g1blk = ccm_fwd_softmax_blocks(N) + 1;
 		} else {
 			st->n_fpart = ccm_softmax_cols_blocks(L, N);
 			st->n_g1blk = ccm_softmax_cols_nblk(N);
@@ -328,6 +330,24 @@ int ccm_init_hip_state(void *instance, const ccm_hip_config *cfg_in, int allow_f
 	HIPCHK(dalloc(st, (void **)&st->d_fpart, sizeof(double) * (size_t)st->n_fpart));
 	HIPCHK(dalloc(st, (void **)&st->d_g1part,
 	              sizeof(double) * (size_t)st->n_g1blk * 20 * (size_t)L));
+	if (st->fused) {
+		/* The fused forward sums only sum_n w_n P_n(a, j) per block; the data
+		 * term -sum_n w_n [x_nj = a] of the node gradient is constant and
+		 * sits in the last block of the partials (sequences in order, FP64) */
+		h_w = (float *)malloc(sizeof(float) * (size_t)N);
+		h_c = (double *)calloc((size_t)20 * L, sizeof(double));
+		if (!h_w || !h_c)
+			goto fail;
+		HIPCHK(hipMemcpy(h_w, st->d_w, sizeof(float) * (size_t)N, hipMemcpyDeviceToHost));
+		for (long n = 0; n < N; n++)
+			for (long j = 0; j < L; j++) {
+				const int a = ud->msa[n * L + j];
+				if (a < 20)
+					h_c[(long)a * L + j] -= (double)h_w[n];
+			}
+		HIPCHK(hipMemcpy(st->d_g1part + (size_t)(st->n_g1blk - 1) * 20 * L, h_c,
+		                 sizeof(double) * (size_t)20 * L, hipMemcpyHostToDevice));
+	}
 	HIPCHK(hipStreamSynchronize(s));
 
 	(void)hipFree(d_msa_rows);
@@ -337,6 +357,8 @@ int ccm_init_hip_state(void *instance, const ccm_hip_config *cfg_in, int allow_f
 	(void)hipFree(d_tmp_g);
 	free(h_msap);
 	free(h_msapt);
+	free(h_w);
+	free(h_c);
 	ud->extra = st;
 	return 0;
 
@@ -350,6 +372,8 @@ fail:
 	(void)hipFree(d_tmp_g);
 	free(h_msap);
 	free(h_msapt);
+	free(h_w);
+	free(h_c);
 	free_state(st);
 	ud->extra = NULL;
 	return 1;

@@ -57,6 +57,15 @@ __device__ __forceinline__ float ccm_wave_sum_dpp_f32(float v) {
   return v;
 }
 
+// sum over each row of 16 lanes, held by every lane of the row
+__device__ __forceinline__ float ccm_row_sum_dpp_f32(float v) {
+  v += ccm_dpp_f32<0xB1>(v);
+  v += ccm_dpp_f32<0x4E>(v);
+  v += ccm_dpp_f32<0x141>(v);
+  v += ccm_dpp_f32<0x140>(v);
+  return v;
+}
+
 template <int CTRL, int ROWS = 0xF>
 __device__ __forceinline__ double ccm_dpp_f64(double v) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
@@ -72,6 +81,14 @@ __device__ __forceinline__ double ccm_wave_sum_dpp_f64(double v) {
   v += ccm_dpp_f64<0x140>(v);
   v += ccm_dpp_f64<0x142, 0xA>(v);
   v += ccm_dpp_f64<0x143, 0xC>(v);
+  return v;
+}
+
+__device__ __forceinline__ double ccm_row_sum_dpp_f64(double v) {
+  v += ccm_dpp_f64<0xB1>(v);
+  v += ccm_dpp_f64<0x4E>(v);
+  v += ccm_dpp_f64<0x141>(v);
+  v += ccm_dpp_f64<0x140>(v);
   return v;
 }
 

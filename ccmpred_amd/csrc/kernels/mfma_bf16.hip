@@ -373,11 +373,15 @@ __device__ __forceinline__ void ccm_buffer_load_lds16(ccm_rsrc_t r, void* lds, u
 __device__ __forceinline__ void ccm_buffer_store16(ccm_rsrc_t r, uint16_t v, uint32_t voff, uint32_t soff) {
   __builtin_amdgcn_raw_buffer_store_b16(v, r, (int)voff, (int)soff, 0);
 }
+__device__ __forceinline__ void ccm_buffer_store32(ccm_rsrc_t r, uint32_t v, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)voff, (int)soff, 0);
+}
 #else
 typedef const void* ccm_rsrc_t;
 __device__ __forceinline__ ccm_rsrc_t ccm_make_rsrc(const void* base) { return base; }
 __device__ __forceinline__ void ccm_buffer_load_lds16(ccm_rsrc_t, void*, uint32_t, uint32_t) {}
 __device__ __forceinline__ void ccm_buffer_store16(ccm_rsrc_t, uint16_t, uint32_t, uint32_t) {}
+__device__ __forceinline__ void ccm_buffer_store32(ccm_rsrc_t, uint32_t, uint32_t, uint32_t) {}
 #endif
 
 // LDS stages of the direct-to-LDS kernel: one __shared__ object per stage
@@ -694,31 +698,38 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
 
   if constexpr (MODE == 2) {
     static_assert(CB == 2 && WN % 2 == 0, "fused epilogue: 64 sequences per wave column");
-    // Softmax of the tile's logits in rounds of 64 sequences (the wave
-    // column wn == k): those waves put their two 32-column blocks into two
-    // stages, TM rows of 32 floats each, then wave w takes alignment columns
-    // jj = w, w + NW, .. for the 64 sequences, one per lane.  (Both the
-    // writes and the reads meet two-way bank conflicts; an address swizzle
-    // that avoids them made the compiler keep every write address in a
-    // register.)
+    // Softmax of the tile's logits in rounds of 128 sequences (the wave
+    // columns wn = 2r, 2r + 1): those waves put their 32-column blocks into
+    // four LDS buffers of TM rows x 32 floats (the three stages and one more;
+    // the writes meet two-way bank conflicts, a swizzle that avoids them made
+    // the compiler keep every write address in a register).  Item it of the
+    // round is alignment column jj = it / 2 for the 64 sequences of wave
+    // column 2r + it % 2, one sequence per lane; waves take items it = w,
+    // w + NW, .. (three each with eight waves).  The objective and
+    // node-gradient terms are summed over each row of 16 lanes and the eight
+    // row sums of a 128-sequence block combined in a fixed order, the same
+    // for every tile shape.
     ccm_lds_barrier();  // no wave reads a stage any more
-    float* const buf0 = reinterpret_cast<float*>(s0);
-    float* const buf1 = reinterpret_cast<float*>(s1);
-    // per round and column jj: the 20 node-gradient sums and the objective;
-    // then the singles V(j, a) of the tile's columns
-    float* const gsum = reinterpret_cast<float*>(s2);
-    double* const fsum = reinterpret_cast<double*>(s2 + WN * JG * 20 * 4);
-    float* const xs = reinterpret_cast<float*>(s2 + WN * JG * 20 * 4 + WN * JG * 8);
-    static_assert(WN * JG * 20 * 4 + WN * JG * 8 + 20 * JG * 4 <= A_BYTES, "epilogue sums fit a stage");
+    float* const b0 = reinterpret_cast<float*>(s0);
+    float* const b1 = reinterpret_cast<float*>(s1);
+    float* const b2 = reinterpret_cast<float*>(s2);
+    float* const b3 = reinterpret_cast<float*>(dtl_stage<A_BYTES, 3>());
+    // per round, wave column, row of lanes and column jj: the 20
+    // node-gradient sums and the objective; the singles V(j, a) of the tile
+    constexpr int NSLOT = (WN / 2) * 2 * 4 * JG;
+    __shared__ float gsum[NSLOT * 20];
+    __shared__ double fsum[NSLOT];
+    __shared__ float xs[20 * JG];
     for (int e = tid; e < 20 * JG; e += NW * 64) {
       const int a = e / JG, j = mt * JG + (e - a * JG);
       xs[e] = j < epi.L ? epi.x1[(long)a * epi.L + j] : 0.f;
     }
     const ccm_rsrc_t drsrc = ccm_make_rsrc(epi.dt);
     const uint32_t dstep = (uint32_t)((long)Lp * epi.lddt * 2), dlo = (uint32_t)(epi.lo_off * 2);
-    auto slot = [](int R, int c) { return (R << 5) + c; };
-    for (int k = 0; k < WN; k++) {
-      if (wn == k) {
+    for (int r = 0; r < WN / 2; r++) {
+      if ((wn >> 1) == r) {
+        float* const w0 = (wn & 1) ? b2 : b0;
+        float* const w1 = (wn & 1) ? b3 : b1;
 #pragma unroll
         for (int rb = 0; rb < RB; rb++)
 #pragma unroll
@@ -726,35 +737,35 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
 #pragma unroll
             for (int i = 0; i < 16; i++) {
               const int R = (wm * RB + rb) * 32 + (i >> 2) * 8 + 4 * q + (i & 3);
-              (cb ? buf1 : buf0)[slot(R, l32)] = acc[rb][cb][i];
+              (cb ? w1 : w0)[R * 32 + l32] = acc[rb][cb][i];
             }
       }
       __syncthreads();
-      // Ncol is a multiple of 128: a round's 64 sequences are all inside the
+      // Ncol is a multiple of 128: a round's sequences are all inside the
       // chunk or all past it
-      const bool in = n0 + k * 64 < Ncol;
-      for (int jj = wave_u; in && jj < JG; jj += NW) {
-        const int j = mt * JG + jj;
+      const bool in = n0 + r * 128 < Ncol;
+      for (int it = wave_u; in && it < 2 * JG; it += NW) {
+        const int jj = it >> 1, hc = it & 1, j = mt * JG + jj;
         if (j >= epi.L) break;
-        const int n = n0 + k * 64 + lane;  // sequence within the chunk
+        const int n = n0 + r * 128 + hc * 64 + lane;  // sequence within the chunk
         const bool ok = n < epi.Nc;
-        const int b = lane >> 5, c = lane & 31;
-        const float* bp = (b ? buf1 : buf0) + jj * 32 + c;
+        const float* bp = (hc ? ((lane >> 5) ? b3 : b2) : ((lane >> 5) ? b1 : b0)) + jj * 32 + (lane & 31);
+        // padding sequences (past Nc) carry the finite logits of an all-255
+        // MSA row and weight 0
         float z[CCM_NA];
 #pragma unroll
         for (int a = 0; a < CCM_NA; a++) {
           float v = bp[a * JG * 32];
           if (a < 20) v += xs[a * JG + jj];
-          z[a] = ok ? v : 0.f;  // padding sequences: finite placeholder logits, weight 0
+          z[a] = v;
         }
-        const int xb = ok ? epi.msapt[(long)j * epi.ldmt + n] : 255;
+        const int xb = ok ? epi.msapt[(long)j * epi.ldmt + n] : 0;
         const float wv = ok ? epi.w[n] : 0.f;
-        float mx = -INFINITY, zx = 0.f;
+        // the observed residue's logit, read once more instead of selected
+        const float zx = bp[xb * JG * 32] + (xb < 20 ? xs[xb * JG + jj] : 0.f);
+        float mx = -INFINITY;
 #pragma unroll
-        for (int a = 0; a < CCM_NA; a++) {
-          mx = fmaxf(mx, z[a]);
-          zx = (a == xb) ? z[a] : zx;
-        }
+        for (int a = 0; a < CCM_NA; a++) mx = fmaxf(mx, z[a]);
         float ssum = 0.f;
 #pragma unroll
         for (int a = 0; a < CCM_NA; a++) {
@@ -762,6 +773,7 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
           ssum += z[a];
         }
         double fl = ok ? (double)(wv * (mx + __logf(ssum) - zx)) : 0.0;
+        const int slot = ((r * 2 + hc) * 4 + (lane >> 4)) * JG + jj;
         if (epi.mode) {
           const float ws = wv / ssum;
           float g1[20];
@@ -772,34 +784,40 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
             const uint16_t h = ccm_f2bf(qv);
             ccm_buffer_store16(drsrc, h, voff, a * dstep);
             if (epi.passes > 1) ccm_buffer_store16(drsrc, ccm_f2bf(qv - ccm_bf2f(h)), voff, a * dstep + dlo);
-            if (a < 20) g1[a] = qv - ((a == xb) ? wv : 0.f);
+            if (a < 20) g1[a] = qv;  // the data term -w [x = a] is added once, at assembly
           }
-          // sums over the round's 64 sequences (lane 63)
 #pragma unroll
-          for (int a = 0; a < 20; a++) g1[a] = ccm_wave_sum_dpp_f32(g1[a]);
-          if (lane == 63) {
+          for (int a = 0; a < 20; a++) g1[a] = ccm_row_sum_dpp_f32(g1[a]);
+          if ((lane & 15) == 15) {
 #pragma unroll
-            for (int a = 0; a < 20; a++) gsum[(k * JG + jj) * 20 + a] = g1[a];
+            for (int a = 0; a < 20; a++) gsum[slot * 20 + a] = g1[a];
           }
         }
-        fl = ccm_wave_sum_dpp_f64(fl);
-        if (lane == 63) fsum[k * JG + jj] = fl;
+        fl = ccm_row_sum_dpp_f64(fl);
+        if ((lane & 15) == 15) fsum[slot] = fl;
       }
       __syncthreads();
     }
-    // partials per block of 128 sequences: rounds 2h and 2h + 1
+    // partials per block of 128 sequences (one round): its eight row sums
     for (int e = tid; e < (WN / 2) * JG * CCM_NA; e += NW * 64) {
-      const int h = e / (JG * CCM_NA), rem = e - h * (JG * CCM_NA);
+      const int r = e / (JG * CCM_NA), rem = e - r * (JG * CCM_NA);
       const int jj = rem / CCM_NA, a = rem - jj * CCM_NA;
-      const int j = mt * JG + jj, nb = n0 + h * 128;
+      const int j = mt * JG + jj, nb = n0 + r * 128;
       if (j >= epi.L || nb >= Ncol) continue;
       const long gb = (long)epi.blk0 + nb / 128;
-      const int r0 = (2 * h) * JG + jj, r1 = (2 * h + 1) * JG + jj;
+      const int s0i = r * 8 * JG + jj;
       if (a < 20) {
-        if (epi.mode)
-          epi.g1part[(gb * 20 + a) * epi.L + j] = (double)gsum[r0 * 20 + a] + (double)gsum[r1 * 20 + a];
+        if (epi.mode) {
+          double t = 0.0;
+#pragma unroll
+          for (int u = 0; u < 8; u++) t += (double)gsum[(s0i + u * JG) * 20 + a];
+          epi.g1part[(gb * 20 + a) * epi.L + j] = t;
+        }
       } else {
-        epi.fpart[gb * epi.L + j] = fsum[r0] + fsum[r1];
+        double t = 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; u++) t += fsum[s0i + u * JG];
+        epi.fpart[gb * epi.L + j] = t;
       }
     }
   } else {
@@ -1185,7 +1203,8 @@ int ccm_fwd_softmax_default_variant(int L, int ncols) {
 
 // The logits of ccm_launch_gemm_fwd for the chunk's ncols sequences (never
 // stored) with softmax_cols as the epilogue: Dt (mode 1), the objective
-// partials fpart[blk][j] and the node-gradient partials g1part[blk][a][j]
+// partials fpart[blk][j] and the partials g1part[blk][a][j] = sum w P (the
+// node gradient without its constant data term)
 // (mode 1) per block blk = blk0 + n / 128 of 128 sequences.
 int ccm_launch_gemm_fwd_softmax(const uint16_t* d_wt, long lda, int Lp, int passes_fwd,
                                 const uint8_t* d_msap, int ncols, const float* d_x1,

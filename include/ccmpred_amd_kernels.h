@@ -64,9 +64,11 @@ int ccm_launch_gemm_fwd(const uint16_t *d_wt, long lda, int Lp, int passes, cons
 /* The forward contraction with the column softmax as its epilogue
  * (cu:30-107 and cu:114-141 in one launch; no counterpart kernel): the
  * logits are never stored.  Writes Dt (mode 1, as ccm_launch_softmax_cols),
- * the objective partials fpart[blk][j] and the node-gradient partials
- * g1part[blk][a][j] (mode 1) per block blk = blk0 + n / 128 of 128
- * sequences; ccm_fwd_softmax_blocks(N) blocks cover N sequences. */
+ * the objective partials fpart[blk][j] and the partials
+ * g1part[blk][a][j] = sum_n w_n P_n(a, j) (mode 1; unlike softmax_cols
+ * without the data term -sum_n w_n [x_nj = a], which the caller adds as one
+ * more block) per block blk = blk0 + n / 128 of 128 sequences;
+ * ccm_fwd_softmax_blocks(N) blocks cover N sequences. */
 int ccm_fwd_softmax_blocks(int N);
 int ccm_fwd_softmax_default_variant(int L, int ncols);
 /* whether Wt (lda = passes_fwd * 21 * Lp) and Dt (21 * Lp rows of
