@@ -149,6 +149,26 @@ def test_bf16_deterministic_and_structured(prec):
         assert torch.all(g1[20 * L: nsp] == 0)
 
 
+@pytest.mark.parametrize("prec", ["bf16", "bf16x2"])
+@pytest.mark.parametrize("L,N", [(75, 1000), (37, 300), (130, 700)])
+def test_tile_variants_bit_identical(prec, L, N):
+    """Every MFMA tile variant (register-staged or direct-to-LDS, 256x256,
+    256x128, 128x128; the forward's softmax epilogue at 12 or 6 alignment
+    columns per tile) sums in the same order: f and g equal the default
+    variant's bit for bit.  L = 75, 37, 130 leave the last tile rows partly
+    past L; N = 300, 700 leave partly filled column tiles."""
+    say(f"tile variants {prec} L={L} N={N}")
+    msa, _ = synthetic_msa(L, N, 41, 6)
+    xd = torch.from_numpy(make_x(msa, 42, 0.02)).cuda()
+    with P.PLL(msa, precision=prec) as p:
+        f0, g0 = p.evaluate(xd)
+    for v in range(6):
+        with P.PLL(msa, precision=prec, gemm_variant=v) as p:
+            f, g = p.evaluate(xd)
+        say(f"  variant {v}: f={f:.17g} (default {f0:.17g}) g equal={torch.equal(g, g0)}")
+        assert f == f0 and torch.equal(g, g0)
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "bf16f"])
 @pytest.mark.parametrize("chunk", [512, 1000, 3000])
 def test_chunked_evaluate_matches_unchunked(prec, chunk):
