@@ -214,7 +214,7 @@ def main():
                     help="with --batched: also time the same solves one MSA at a time")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: there is no multi-GPU path "
               "(the evaluate callback works on one device); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.batched:

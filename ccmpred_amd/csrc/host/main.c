@@ -4,9 +4,10 @@
  * Flags follow the reference's CLI (ccmpred.c:183-339): same options, same
  * defaults (-n 250, -e 0.01, -k 5, -w 0.8, -l 0.2), same progress table and
  * output formats, same exit codes (1 usage/device, 2 MSA open, 3/4 output
- * open, 100 FASTA/a3m input).  Additions: -P selects the precision form
- * (fp32 | bf16 | bf16x2).  The reference's CPU evaluators (-t) are not
- * part of this GPU build.
+ * open, 100 FASTA/a3m input), -b for the MessagePack raw file.  Addition:
+ * -P selects the precision form (fp32 | bf16 | bf16f | bf16x2).  The
+ * reference's CPU evaluators (-t) are not
+ * offered: this driver evaluates on the GPU only.
  */
 #define __HIP_PLATFORM_AMD__
 #include <hip/hip_runtime_api.h>
@@ -80,7 +81,7 @@ int main(int argc, char **argv)
 			switch (o) {
 			case 'd': device = atoi(val); break;
 			case 't':
-				fprintf(stderr, "CPU evaluation (-t) is not part of this build\n");
+				fprintf(stderr, "CPU evaluation (-t) is not available: this ccmpred evaluates on the GPU only\n");
 				return 1;
 			case 'n': numiter = atoi(val); break;
 			case 'e': cg_eps = atof(val); break;
