@@ -783,7 +783,10 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
             const float qv = z[a] * ws;
             const uint16_t h = ccm_f2bf(qv);
             ccm_buffer_store16(drsrc, h, voff, a * dstep);
-            if (epi.passes > 1) ccm_buffer_store16(drsrc, ccm_f2bf(qv - ccm_bf2f(h)), voff, a * dstep + dlo);
+            // residual of the exact product (explicit fused multiply-add, as
+            // in softmax_cols)
+            if (epi.passes > 1)
+              ccm_buffer_store16(drsrc, ccm_f2bf(__builtin_fmaf(z[a], ws, -ccm_bf2f(h))), voff, a * dstep + dlo);
             if (a < 20) g1[a] = qv;  // the data term -w [x = a] is added once, at assembly
           }
 #pragma unroll

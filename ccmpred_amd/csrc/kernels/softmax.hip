@@ -165,7 +165,11 @@ __device__ __forceinline__ void softmax_cols_block(const float* __restrict__ Zt,
         const uint32_t h = ccm_f2bf2(q0, q1);
         dcol[a * ds] = h;
         if (passes > 1)
-          dcol[a * ds + lo_off / 2] = ccm_f2bf2(q0 - ccm_bf2f((uint16_t)h), q1 - ccm_bf2f((uint16_t)(h >> 16)));
+          // residual of the exact product z * (w / s): an explicit fused
+          // multiply-add, so that no contraction choice of the compiler
+          // changes it (the fused forward's epilogue forms it the same way)
+          dcol[a * ds + lo_off / 2] = ccm_f2bf2(__builtin_fmaf(z[a].x, ws0, -ccm_bf2f((uint16_t)h)),
+                                                __builtin_fmaf(z[a].y, ws1, -ccm_bf2f((uint16_t)(h >> 16))));
         if (a < 20) g1[a] += (q0 - ((a == x0) ? w0 : 0.f)) + (q1 - ((a == x1v) ? w1 : 0.f));
       }
     }
