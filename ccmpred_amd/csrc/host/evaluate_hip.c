@@ -652,6 +652,11 @@ int ccm_hip_set_timing(void *instance, int on)
 		for (int i = 0; i <= CCM_TIMING_STEPS; i++)
 			HIPCHK(hipEventCreate(&st->ev[i]));
 		st->timing = 1;
+	} else if (!on && st->timing) {
+		HIPCHK(hipStreamSynchronize(st->stream));
+		for (int i = 0; i <= CCM_TIMING_STEPS; i++)
+			(void)hipEventDestroy(st->ev[i]);
+		st->timing = 0;
 	}
 	for (int i = 0; i < CCM_TIMING_STEPS; i++)
 		st->step_ms[i] = 0.0;

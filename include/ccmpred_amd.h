@@ -112,7 +112,8 @@ void *ccm_hip_stream(void *instance);
 
 /* Per-step device timing with HIP events on the instance stream (no
  * counterpart in the reference).  set_timing(on=1) creates the events and
- * clears the accumulators; get_timing fills up to n averages in ms and
+ * clears the accumulators, on=0 releases them (the events cost host time
+ * per evaluation); get_timing fills up to n averages in ms and
  * returns the number of timed evaluations.  Steps, FP32 path: gather_fwd,
  * softmax, gather_grad, assembly (pair epilogue, singles and the objective's
  * sum); BF16 path: convert_wt, gemm_fwd (with the softmax when fused),
