@@ -104,7 +104,7 @@ enum { K_CONV, K_SOFT, K_ASM, K_ALL, K_DOTS, K_VEC };
 static uint32_t blocks_of(const ccm_bprob *p, int kind)
 {
 	switch (kind) {
-	case K_CONV: return (uint32_t)ccm_convert_wt_blocks(p->Lp);
+	case K_CONV: return (uint32_t)ccm_convert_wt_blocks(p->L, p->Lp);
 	case K_SOFT: return (uint32_t)ccm_softmax_cols_blocks(p->L, p->N);
 	case K_ASM: return (uint32_t)ccm_assemble_blocks(p->L);
 	case K_ALL: return 1;

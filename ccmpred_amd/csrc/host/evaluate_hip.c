@@ -296,6 +296,8 @@ int ccm_init_hip_state(void *instance, const ccm_hip_config *cfg_in, int allow_f
 		 * offsets */
 		st->fused = allow_fused && ccm_fwd_softmax_fits(Lp, st->cfg.bf16_fwd_passes, passes, st->ccols);
 		HIPCHK(dalloc(st, (void **)&st->d_wt, sizeof(uint16_t) * (size_t)P * st->ldwt));
+		/* the rows of padding columns j >= L stay zero: convert_wt skips them */
+		HIPCHK(hipMemsetAsync(st->d_wt, 0, sizeof(uint16_t) * (size_t)P * st->ldwt, s));
 		if (!st->fused)
 			HIPCHK(dalloc(st, (void **)&st->d_zt, sizeof(float) * (size_t)P * st->ccols));
 		HIPCHK(dalloc(st, (void **)&st->d_dt, sizeof(uint16_t) * (size_t)P * st->lddt));

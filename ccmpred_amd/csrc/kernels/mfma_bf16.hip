@@ -18,24 +18,28 @@
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-// coupling matrix -> Wt (bf16, P x lda), zero outside the real 21L x 21L block
+// coupling matrix -> Wt (bf16, P x lda).  A block takes 64 rows r = b*Lp + k
+// (one residue block b: Lp is a multiple of 64) and 64 columns C = a*L + j
+// of the x layout starting at a multiple of 64, so that every row segment it
+// reads is 256 contiguous, aligned bytes; column C goes to Wt row
+// c = a*Lp + j.  Columns k >= L of Wt are written as zeros; rows j >= L
+// (a*Lp + j) are never written: the caller zeroes Wt once.
 __device__ __forceinline__ void convert_wt_block(const float* __restrict__ x2, int L, int Lp,
                                                  uint16_t* __restrict__ wt, long lda, int passes,
                                                  uint32_t tiles_r, uint32_t blk) {
   __shared__ float T[64][65];
-  const int P = CCM_NA * Lp;  // a multiple of 64: the tiles cover it exactly
+  const int P = CCM_NA * Lp;  // a multiple of 64: the row tiles cover it exactly
+  const int C21 = CCM_NA * L;
   const int tc = blk / tiles_r, tr = blk % tiles_r;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const long ld2 = (long)CCM_NAP * L;
   {
-    // T[rr][cc] = W at coupling row r = tr*64 + rr, output row c = tc*64 + cc;
-    // Lp being a multiple of 64, a tile lies in one residue block b (rows)
-    // and a (columns), so the divisions are per tile; the 16 loads of a
-    // thread are issued before they are stored
-    const int a = (tc * 64) / Lp, j = tc * 64 - a * Lp + lane;
+    // T[rr][cc] = W at coupling row r = tr*64 + rr, x column C = tc*64 + cc;
+    // the 16 loads of a thread are issued before they are stored
     const int b = (tr * 64) / Lp, k0 = tr * 64 - b * Lp;
-    const bool cok = a < CCM_NA && j < L;
-    const float* src = x2 + (long)(b * L + k0) * ld2 + a * L + j;
+    const int C = tc * 64 + lane;
+    const bool cok = C < C21;
+    const float* src = x2 + (long)(b * L + k0) * ld2 + C;
     float v[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) {
@@ -49,7 +53,10 @@ __device__ __forceinline__ void convert_wt_block(const float* __restrict__ x2, i
   // each thread converts 16 consecutive r of one output row c and stores
   // them as two 16-byte vectors (and the residual half likewise)
   const int cl = threadIdx.x >> 2, qr = threadIdx.x & 3;
-  const int c = tc * 64 + cl, r0 = tr * 64 + qr * 16;
+  const int C = tc * 64 + cl;
+  if (C >= C21) return;
+  const int a = C / L;
+  const int c = a * Lp + (C - a * L), r0 = tr * 64 + qr * 16;
   uint32_t hi[8], lo[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) {
@@ -938,9 +945,9 @@ int ccm_launch_convert_wt(const float* d_x, int L, int Lp, uint16_t* d_wt, long 
                           hipStream_t s) {
   const int P = CCM_NA * Lp;
   if (Lp % 64 || Lp < L || lda < (long)passes * P || lda % 8) return CCM_ERR_ARG;
-  const uint32_t tiles = (uint32_t)((P + 63) / 64);
-  hipLaunchKernelGGL(k_convert_wt, dim3(tiles * tiles), dim3(256), 0, s, d_x + ccm_nsp(L), L, Lp,
-                     d_wt, lda, passes, tiles);
+  const uint32_t tiles_r = (uint32_t)((P + 63) / 64), tiles_c = (uint32_t)((CCM_NA * L + 63) / 64);
+  hipLaunchKernelGGL(k_convert_wt, dim3(tiles_c * tiles_r), dim3(256), 0, s, d_x + ccm_nsp(L), L, Lp,
+                     d_wt, lda, passes, tiles_r);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }
@@ -1124,9 +1131,8 @@ int ccm_gemm_default_variant_for(int mode, int Lp, int ncols) {
   return gemm_default_variant(P, mode == 0 ? ncols : P);
 }
 
-int ccm_convert_wt_blocks(int Lp) {
-  const int t = (CCM_NA * Lp + 63) / 64;
-  return t * t;
+int ccm_convert_wt_blocks(int L, int Lp) {
+  return ((CCM_NA * Lp + 63) / 64) * ((CCM_NA * L + 63) / 64);
 }
 
 // grouped forms: g.offs holds each alignment's block range; every alignment

@@ -83,7 +83,7 @@ def test_fused_matches_unfused(lib, L, N, pf, pd, variant):
     d_x = torch.from_numpy(x).to(dev)
     d_w = torch.from_numpy(w).to(dev)
     lda = pf * P
-    d_wt = torch.empty(P * lda, dtype=torch.int16, device=dev)
+    d_wt = torch.zeros(P * lda, dtype=torch.int16, device=dev)  # convert_wt leaves padding rows alone
     s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     assert lib.ccm_launch_convert_wt(ptr(d_x), L, Lp, ptr(d_wt), lda, pf, s) == 0
 
