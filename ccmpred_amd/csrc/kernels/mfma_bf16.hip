@@ -13,6 +13,10 @@
 //              the exact init-time histogram)
 // With two passes, A carries a hi and a lo BF16 half along K (K = 2*Kb), the
 // one-hot operand repeats, and the FP32 accumulator sums both.
+// The forward also exists with the column softmax as its epilogue (MODE 2):
+// its tiles gather whole softmax groups (all 21 residues of 12 or 6
+// alignment columns), and the logits stay in LDS; Dt and the objective and
+// node-gradient partials are its outputs (evaluate_cpu.c:76-109).
 #include "common.h"
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
