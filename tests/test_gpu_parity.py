@@ -247,6 +247,30 @@ def test_bf16_evaluate_error(prec, tol, L, N, seed):
     assert e1 <= 5 * tol
 
 
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-4), ("bf16x2", 1e-4), ("bf16f", 1e-3), ("bf16", 1e-3)])
+@pytest.mark.parametrize("L,N", [(1, 1), (2, 3), (3, 130), (12, 128), (13, 129), (7, 513)])
+def test_tiny_shapes(prec, tol, L, N):
+    """Degenerate sizes against the FP64 oracle: one column (only self-edges),
+    one sequence, a single tile row holding 1..13 columns, N at and just past
+    the 128-sequence blocks and the 512-sequence chunk granule.  Random MSA
+    (gaps included) and random symmetric couplings; the same tolerances as
+    at full size."""
+    say(f"tiny {prec} L={L} N={N}")
+    rng = np.random.default_rng(L * 1000 + N)
+    msa = rng.integers(0, 21, (N, L), dtype=np.uint8)
+    x = make_x(msa, L + N, 0.05)
+    with P.PLL(msa, precision=prec) as p:
+        f64, g64 = ora.evaluate_f64(msa, p.weights.astype(np.float64), x.astype(np.float64),
+                                    p.lambda_single, p.lambda_pair)
+        f, g = p.evaluate(torch.from_numpy(x).cuda())
+        g = g.cpu().numpy()
+    e = rel_l2(g, g64)
+    say(f"  f={f:.12g} f64={f64:.12g} rel={abs(f - f64) / abs(f64):.2e} g rel_l2={e:.2e}")
+    assert np.all(np.isfinite(g))
+    assert abs(f - f64) / abs(f64) <= (1e-5 if prec in ("fp32", "bf16x2") else 1e-4)
+    assert e <= tol
+
+
 # ---- device CG end to end ------------------------------------------------
 
 def _spearman(a, b):
