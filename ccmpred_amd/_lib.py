@@ -79,7 +79,7 @@ EXPORTED = [
     "ccm_launch_convert_wt_grouped", "ccm_launch_gemm_fwd_grouped", "ccm_launch_gemm_grad_grouped",
     "ccm_launch_softmax_cols_grouped", "ccm_launch_pair_epilogue_grouped",
     "ccm_launch_singles_finalize_grouped", "ccm_launch_sum3_grouped", "ccm_launch_dots_grouped",
-    "ccm_launch_cg_vec_grouped", "ccm_launch_cg_ctrl",
+    "ccm_launch_cg_vec_grouped", "ccm_launch_cg_ctrl", "ccm_launch_pad_scan",
     "ccm_assemble_blocks", "ccm_launch_assemble", "ccm_launch_assemble_grouped",
 ]
 

@@ -857,6 +857,9 @@ int ccm_batch_solve(ccm_batch *b, const ccm_conjugrad_parameter *param, int roun
 		HIPCHK(hipMemsetAsync(b->h_probs[i].s, 0, vb, b->s));
 		HIPCHK(hipMemsetAsync(b->h_probs[i].gprev, 0, vb, b->s));
 	}
+	/* the vector step and dots skip the a >= 21 padding of every x that
+	 * starts with zeros there */
+	KCHK(ccm_launch_pad_scan(b->d_probs, b->n, b->s));
 	HIPCHK(hipStreamSynchronize(b->s));
 
 	HIPCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));

@@ -6,13 +6,56 @@
 
 #define VO_BLOCKS 1024
 
+// The padded parameter layout (ccmpred.c:144-151) in float4 units: ns4
+// float4s of singles, then one row of row4 = 8 L float4s per (b, k) whose
+// first act4 = ceil(21 L / 4) hold the pair entries a < 21 (the last one
+// may also hold a few a = 21 entries); the rest of the row is a >= 21
+// padding.  There the gradient is 2 lambda_pair x (evaluate_cpu.c:158-161),
+// so where x starts at zero in that padding (ccmpred.c:497 clears it) the
+// gradient, the CG direction and the previous gradient stay exactly zero
+// for the whole solve: a dot product whose first operand is the gradient
+// only adds exact zeros there, and a vector step leaves x unchanged, so
+// skipping the padding changes no bit of either.  row4 == 0 means no
+// layout (every float4 is read): so it is when ccm_launch_pad_scan found a
+// nonzero value in the skipped float4s.  Indices past the singles fit 32
+// bits whenever row4 is set.
+struct ccm_pad_rows {
+  long ns4 = 0;
+  uint32_t row4 = 0, act4 = 0, nrows = 0;
+  __device__ __forceinline__ bool skip(long i) const {
+    return row4 != 0 && i >= ns4 && (uint32_t)(i - ns4) % row4 >= act4;
+  }
+  // float4 index of the q-th float4 that is not skipped
+  __device__ __forceinline__ long active(long q) const {
+    if (q < ns4) return q;
+    const uint32_t t = (uint32_t)(q - ns4), r = t / act4;
+    return ns4 + (long)r * row4 + (t - r * act4);
+  }
+  __device__ __forceinline__ long n_active() const { return ns4 + (long)nrows * act4; }
+};
+
+// the layout of alignment B, checking its padding flag when scan is false
+template <bool scan = false>
+__device__ __forceinline__ ccm_pad_rows ccm_pad_rows_of(const ccm_bprob& B) {
+  ccm_pad_rows p;
+  const long n4 = B.nvp >> 2;
+  if ((scan || !B.ctl->x_pad_nz) && (B.nsp & 3) == 0 && (B.nvp & 3) == 0 && B.L > 0 &&
+      B.nvp == B.nsp + (long)B.L * B.L * CCM_NA * CCM_NAP && n4 - (B.nsp >> 2) < (1L << 32)) {
+    p.ns4 = B.nsp >> 2;
+    p.row4 = (uint32_t)B.L * (CCM_NAP / 4);
+    p.act4 = ((uint32_t)B.L * CCM_NA + 3) / 4;
+    p.nrows = (uint32_t)B.L * CCM_NA;
+  }
+  return p;
+}
+
 // up to three dot products sharing the first operand: out[i] = <a, b_i>
 __device__ __forceinline__ void dots_partial_block(const float* __restrict__ a,
                                                    const float* __restrict__ b0,
                                                    const float* __restrict__ b1,
                                                    const float* __restrict__ b2, long n, int nd,
                                                    double* __restrict__ part, uint32_t blk,
-                                                   uint32_t nblk) {
+                                                   uint32_t nblk, const ccm_pad_rows pad = {}) {
   __shared__ double red[3][4];
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
   // 16-byte loads over the first 4*(n/4) elements when every vector is
@@ -27,6 +70,7 @@ __device__ __forceinline__ void dots_partial_block(const float* __restrict__ a,
   const float4* b14 = reinterpret_cast<const float4*>(b1);
   const float4* b24 = reinterpret_cast<const float4*>(b2);
   for (long i = (long)blk * 256 + threadIdx.x; i < n4; i += stride) {
+    if (pad.skip(i)) continue;
     const float4 av = a4[i], v0 = b04[i];
     s0 += (double)av.x * (double)v0.x + (double)av.y * (double)v0.y + (double)av.z * (double)v0.z +
           (double)av.w * (double)v0.w;
@@ -110,16 +154,16 @@ __global__ __launch_bounds__(256) void k_dots_final(const double* __restrict__ p
   dots_final_block(part, nblk, nd, out);
 }
 
-// x += alpha * s
+// x += alpha * s (one rounding)
 __global__ void k_axpy(float* __restrict__ x, const float* __restrict__ s, float alpha, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] += alpha * s[i];
+  if (i < n) x[i] = __builtin_fmaf(alpha, s[i], x[i]);
 }
 
-// s = -g + beta * s
+// s = -g + beta * s (one rounding)
 __global__ void k_direction(float* __restrict__ s, const float* __restrict__ g, float beta, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) s[i] = -g[i] + beta * s[i];
+  if (i < n) s[i] = __builtin_fmaf(beta, s[i], -g[i]);
 }
 
 // s = -g
@@ -131,14 +175,16 @@ __global__ void k_negate(float* __restrict__ s, const float* __restrict__ g, lon
 // ---- batched conjugate gradient: every alignment of a group per launch ----
 
 // <g,s>, <g,g>, <g,gprev> of each evaluated alignment: n_dots blocks each,
-// the same partition and order as ccm_launch_dots over nvp elements
+// the same partition and order as ccm_launch_dots over nvp elements (the
+// skipped padding adds only zeros)
 __global__ __launch_bounds__(256) void k_dots_partial_grouped(const ccm_bprob* __restrict__ probs,
                                                               const uint32_t* __restrict__ offs,
                                                               const int32_t* __restrict__ pidx, int np) {
   const int gi = ccm_group_find(offs, np, blockIdx.x);
   const ccm_bprob& B = probs[pidx[gi]];
   if (!B.ctl->eval_on) return;
-  dots_partial_block(B.g, B.s, B.g, B.gprev, B.nvp, 3, B.dpart, blockIdx.x - offs[gi], (uint32_t)B.n_dots);
+  dots_partial_block(B.g, B.s, B.g, B.gprev, B.nvp, 3, B.dpart, blockIdx.x - offs[gi], (uint32_t)B.n_dots,
+                     ccm_pad_rows_of(B));
 }
 
 __global__ __launch_bounds__(256) void k_dots_final_grouped(const ccm_bprob* __restrict__ probs,
@@ -148,10 +194,27 @@ __global__ __launch_bounds__(256) void k_dots_final_grouped(const ccm_bprob* __r
   dots_final_block(B.dpart, B.n_dots, 3, B.ctl->dots);
 }
 
+// One element of the vector step (one rounding per multiply-add, as in
+// k_axpy / k_direction / k_negate); gpi is read only by CCM_VEC_RESTART.
+__device__ __forceinline__ void cg_vec_elem(float& xi, float& si, float gi, float gpi, int mode, float cpre,
+                                            float ctr, float beta) {
+  if (cpre != 0.0f) xi = __builtin_fmaf(cpre, si, xi);
+  if (mode == CCM_VEC_NEGATE)
+    si = -gi;
+  else if (mode == CCM_VEC_DIRECTION)
+    si = __builtin_fmaf(beta, si, -gi);
+  else if (mode == CCM_VEC_RESTART)
+    si = -gpi;
+  if (ctr != 0.0f) xi = __builtin_fmaf(ctr, si, xi);
+}
+
 // The vector step of a round, as its control block says:
 //   x += c_pre s;  s = per vec_mode (gprev = g where the mode moves on);
 //   x += c_trial s
-// with the same FP32 arithmetic as k_axpy / k_direction / k_negate.
+// with the same FP32 arithmetic as k_axpy / k_direction / k_negate.  With
+// the padded layout, only the float4s that hold singles and pair entries
+// a < 21 are visited (the padding of x is left as it is, which is what the
+// step computes there from a zero direction).
 __global__ __launch_bounds__(256) void k_cg_vec_grouped(const ccm_bprob* __restrict__ probs,
                                                         const uint32_t* __restrict__ offs,
                                                         const int32_t* __restrict__ pidx, int np) {
@@ -166,25 +229,57 @@ __global__ __launch_bounds__(256) void k_cg_vec_grouped(const ccm_bprob* __restr
   float* __restrict__ sd = B.s;
   float* __restrict__ gp = B.gprev;
   const float* __restrict__ g = B.g;
+  const bool moves = mode == CCM_VEC_NEGATE || mode == CCM_VEC_DIRECTION;
+  const ccm_pad_rows pad = ccm_pad_rows_of(B);
+  if (pad.row4 != 0 && (((uintptr_t)x | (uintptr_t)sd | (uintptr_t)gp | (uintptr_t)g) & 15u) == 0) {
+    float4* __restrict__ x4 = reinterpret_cast<float4*>(x);
+    float4* __restrict__ s4 = reinterpret_cast<float4*>(sd);
+    float4* __restrict__ gp4 = reinterpret_cast<float4*>(gp);
+    const float4* __restrict__ g4 = reinterpret_cast<const float4*>(g);
+    const long nq = pad.n_active();
+    for (long q = (long)blk * 256 + threadIdx.x; q < nq; q += (long)nblk * 256) {
+      const long i = pad.active(q);
+      float4 xv = x4[i], sv = s4[i];
+      float4 gv = make_float4(0.f, 0.f, 0.f, 0.f), pv = gv;
+      if (moves) gv = g4[i];
+      if (mode == CCM_VEC_RESTART) pv = gp4[i];
+      cg_vec_elem(xv.x, sv.x, gv.x, pv.x, mode, cpre, ctr, beta);
+      cg_vec_elem(xv.y, sv.y, gv.y, pv.y, mode, cpre, ctr, beta);
+      cg_vec_elem(xv.z, sv.z, gv.z, pv.z, mode, cpre, ctr, beta);
+      cg_vec_elem(xv.w, sv.w, gv.w, pv.w, mode, cpre, ctr, beta);
+      x4[i] = xv;
+      if (mode != CCM_VEC_KEEP) s4[i] = sv;
+      if (moves) gp4[i] = gv;
+    }
+    return;
+  }
   const long n = B.nvp;
   for (long i = (long)blk * 256 + threadIdx.x; i < n; i += (long)nblk * 256) {
     float xi = x[i], si = sd[i];
-    if (cpre != 0.0f) xi += cpre * si;
-    if (mode == CCM_VEC_NEGATE) {
-      const float gi2 = g[i];
-      si = -gi2;
-      gp[i] = gi2;
-    } else if (mode == CCM_VEC_DIRECTION) {
-      const float gi2 = g[i];
-      si = -gi2 + beta * si;
-      gp[i] = gi2;
-    } else if (mode == CCM_VEC_RESTART) {
-      si = -gp[i];
-    }
-    if (ctr != 0.0f) xi += ctr * si;
+    const float gi2 = moves ? g[i] : 0.0f;
+    cg_vec_elem(xi, si, gi2, mode == CCM_VEC_RESTART ? gp[i] : 0.0f, mode, cpre, ctr, beta);
     x[i] = xi;
     if (mode != CCM_VEC_KEEP) sd[i] = si;
+    if (moves) gp[i] = gi2;
   }
+}
+
+// ctl->x_pad_nz = 1 where x has a nonzero value in a float4 that the
+// padded layout skips; blockIdx.y is the alignment
+__global__ __launch_bounds__(256) void k_pad_scan(const ccm_bprob* __restrict__ probs) {
+  const ccm_bprob& B = probs[blockIdx.y];
+  const ccm_pad_rows pad = ccm_pad_rows_of<true>(B);
+  if (pad.row4 == 0 || pad.act4 >= pad.row4 || ((uintptr_t)B.x & 15u) != 0) return;
+  const uint32_t skip4 = pad.row4 - pad.act4;
+  const uint32_t n = pad.nrows * skip4;  // below the 2^32 the layout guarantees
+  const float4* __restrict__ x4 = reinterpret_cast<const float4*>(B.x);
+  bool nz = false;
+  for (uint32_t t = blockIdx.x * 256 + threadIdx.x; t < n; t += gridDim.x * 256) {
+    const uint32_t r = t / skip4, c = t - r * skip4;
+    const float4 v = x4[pad.ns4 + (long)r * pad.row4 + pad.act4 + c];
+    nz |= v.x != 0.0f || v.y != 0.0f || v.z != 0.0f || v.w != 0.0f;
+  }
+  if (nz) B.ctl->x_pad_nz = 1;
 }
 
 // Line search and direction control of one alignment after an evaluation
@@ -369,6 +464,13 @@ int ccm_launch_dots_grouped(const ccm_bprob* d_probs, ccm_group g, ccm_group gfi
 int ccm_launch_cg_vec_grouped(const ccm_bprob* d_probs, ccm_group g, hipStream_t s) {
   if (g.np <= 0 || g.total == 0) return CCM_ERR_ARG;
   hipLaunchKernelGGL(k_cg_vec_grouped, dim3(g.total), dim3(256), 0, s, d_probs, g.offs, g.pidx, g.np);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_pad_scan(const ccm_bprob* d_probs, int nprob, hipStream_t s) {
+  if (nprob <= 0 || !d_probs) return CCM_ERR_ARG;
+  hipLaunchKernelGGL(k_pad_scan, dim3(256, (unsigned)nprob), dim3(256), 0, s, d_probs);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }

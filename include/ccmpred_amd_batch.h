@@ -35,7 +35,9 @@ typedef struct ccm_cg_ctl {
 	double c_pre, c_trial, beta;
 	double f, f0, alpha, moved, lo, hi, dg0, gg;
 	int32_t phase, it, ls, steepest;
-	int32_t status, nevals, pad0, pad1;
+	int32_t status, nevals;
+	int32_t x_pad_nz; /* set by ccm_launch_pad_scan: x holds nonzero a >= 21 padding */
+	int32_t pad1;
 	double fx;      /* objective of the last evaluation (FP64 sum) */
 	double dots[3]; /* <g,s>, <g,g>, <g,gprev> at the last evaluation */
 } ccm_cg_ctl;

@@ -158,6 +158,11 @@ int ccm_launch_singles_finalize_grouped(const ccm_bprob *d_probs, ccm_group g, c
 int ccm_launch_sum3_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
 int ccm_launch_dots_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_group gfinal, ccm_stream_t s);
 int ccm_launch_cg_vec_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
+/* sets ctl->x_pad_nz of every alignment whose x holds a nonzero value in
+ * the a >= 21 padding of the pair block; with the flag clear, the grouped
+ * vector step and dots skip that padding (a zero padding stays zero: its
+ * gradient is 2 lambda_pair x = 0) */
+int ccm_launch_pad_scan(const ccm_bprob *d_probs, int nprob, ccm_stream_t s);
 int ccm_launch_cg_ctrl(ccm_bprob *d_probs, int nprob, const ccm_cg_prm *prm, int32_t *d_nactive,
                        ccm_stream_t s);
 

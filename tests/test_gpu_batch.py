@@ -5,7 +5,9 @@ and the device-side CG control makes the host loop's decisions in the same
 FP64 arithmetic, so every alignment of a batch must reproduce the
 single-alignment BF16 results bit for bit: objective and gradient of one
 evaluation, and x, f and the counters after a CG run (also after the
-autotuner has changed the tile variants, whose K order is the same).
+autotuner has changed the tile variants, whose K order is the same, and
+with values in the padding of x, which the batched vector step and dot
+products skip).
 """
 import numpy as np
 import pytest
@@ -88,6 +90,11 @@ def test_batch_evaluate_bitwise(prec):
 def test_batch_solve_matches_single():
     shapes = SHAPES[:4]
     msas, xs = make_set(shapes)
+    # values in the a >= 21 padding of one start point: the batched vector
+    # step skips the padding, the single-alignment one adds a zero step there
+    L0 = msas[0].shape[1]
+    pad = xs[0][nsingle_padded(L0):].reshape(NA, L0, NAP, L0)
+    pad[:, :, NA:, :] = np.random.default_rng(7).normal(0, 1, (NA, L0, NAP - NA, L0)).astype(np.float32)
     its = 12
     say(f"batch CG {its} iterations: shapes {[(m.shape[1], m.shape[0]) for m in msas]}")
     with Batch(msas, precision="bf16") as b:
