@@ -6,9 +6,10 @@ device-resident x and g (f read back to the host), for the seeded
 synthetic MSA with L=250 columns, N=5000 sequences and 60 planted contacts
 (seed 2), reweighting threshold 0.8, lambda_single 0.01,
 lambda_pair 0.2*(L-1), and x = init_bias plus symmetric N(0, 0.01^2)
-couplings.  `value` is the BF16-MFMA path; the plain FP32 path and the
-OpenMP CPU evaluator (the oracle's restatement of evaluate_cpu_omp.c) are
-timed in the same run on the same MSA and x.
+couplings.  `value` is the BF16-MFMA path; the plain FP32 path, the
+OpenMP CPU evaluator (the oracle's restatement of evaluate_cpu_omp.c) and
+the single-thread one (of evaluate_cpu.c) are timed in the same run on the
+same MSA and x.
 
 Prints one JSON line.
 """
@@ -104,6 +105,30 @@ def time_cpu(msa, x_host, evals):
     ev.close()
     med = float(np.median(ts))
     return dict(evals_per_s=1.0 / med, seconds=ts, threads=threads)
+
+
+def time_cpu_serial(msa, x_host, evals):
+    """The oracle's single-thread restatement of evaluate_cpu.c (seconds per call)."""
+    from oracle import oracle as ora
+    w = ora.calculate_weights(msa, 0.8)
+    L = msa.shape[1]
+    ts = []
+    for _ in range(evals):
+        t0 = time.perf_counter()
+        ora.evaluate_f32(msa, w, x_host, 0.01, np.float32(0.2 * (L - 1)))
+        ts.append(time.perf_counter() - t0)
+    return ts
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def batched_workload(nmsa, seed=999):
@@ -204,6 +229,8 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-evals", type=int, default=3)
+    ap.add_argument("--cpu-serial-evals", type=int, default=1,
+                    help="single-thread evaluate_cpu restatement calls to time (0: none)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a device copy of this many MiB (counter calibration)")
@@ -243,6 +270,7 @@ def main():
     bf16 = time_gpu(msa, x, "bf16", args.steps, args.warmup)
     fp32 = time_gpu(msa, x, "fp32", max(10, args.steps // 4), max(2, args.warmup // 2))
     cpu = None if args.no_cpu else time_cpu(msa, x, args.cpu_evals)
+    serial = None if args.no_cpu or args.cpu_serial_evals <= 0 else time_cpu_serial(msa, x, args.cpu_serial_evals)
 
     L, N = L_COLS, N_SEQS
     flops = 2.0 * N * (21 * L) ** 2          # algorithmic FLOPs of one contraction
@@ -290,6 +318,13 @@ def main():
                                 "sample": f"{len(cpu['seconds'])} evaluations of the full L=250,N=5000 "
                                           "workload (median), OpenMP restatement of evaluate_cpu_omp.c"}
         line["bf16_over_cpu"] = round(bf16["evals_per_s"] / cpu["evals_per_s"], 1)
+        line["cpu_model"] = cpu_model()
+        line["cpu_threads_available"] = os.cpu_count()
+    if serial is not None:
+        line["cpu_single_thread"] = {"value": round(1.0 / float(np.median(serial)), 4), "unit": "evals/s",
+                                     "cores": 1, "kind": "port",
+                                     "sample": f"{len(serial)} evaluation(s) of the full L=250,N=5000 "
+                                               "workload, single-thread restatement of evaluate_cpu.c"}
     print(json.dumps(line), flush=True)
 
 

@@ -80,6 +80,8 @@ EXPORTED = [
     "ccm_launch_softmax_cols_grouped", "ccm_launch_pair_epilogue_grouped",
     "ccm_launch_singles_finalize_grouped", "ccm_launch_sum3_grouped", "ccm_launch_dots_grouped",
     "ccm_launch_cg_vec_grouped", "ccm_launch_cg_ctrl", "ccm_launch_pad_scan",
+    "ccm_launch_pad_scan_x", "ccm_launch_dots_padded", "ccm_launch_axpy_padded",
+    "ccm_launch_direction_padded", "ccm_launch_negate_padded",
     "ccm_assemble_blocks", "ccm_launch_assemble", "ccm_launch_assemble_grouped",
 ]
 

@@ -51,12 +51,16 @@ typedef struct {
 	double *d_out;
 	double *h_out;
 	long n;
+	int ncol; /* > 0: skip the a >= 21 padding of this many columns' layout */
 } cg_ctx;
 
 static int dots(cg_ctx *c, const float *a, const float *b0, const float *b1, const float *b2,
                 int nd, double *out)
 {
-	if (ccm_launch_dots(a, b0, b1, b2, c->n, nd, c->d_part, c->d_out, c->s) != 0)
+	const int r = c->ncol ? ccm_launch_dots_padded(a, b0, b1, b2, c->n, nd, c->ncol, c->d_part,
+	                                               c->d_out, c->s)
+	                      : ccm_launch_dots(a, b0, b1, b2, c->n, nd, c->d_part, c->d_out, c->s);
+	if (r != 0)
 		return -1;
 	if (hipMemcpyAsync(c->h_out, c->d_out, sizeof(double) * (size_t)nd, hipMemcpyDeviceToHost,
 	                   c->s) != hipSuccess)
@@ -69,6 +73,45 @@ static int dots(cg_ctx *c, const float *a, const float *b0, const float *b1, con
 }
 
 static int sync_stream(cg_ctx *c) { return hipStreamSynchronize(c->s) == hipSuccess ? 0 : -1; }
+
+static int axpy(cg_ctx *c, float *x, const float *sdir, float alpha)
+{
+	return c->ncol ? ccm_launch_axpy_padded(x, sdir, alpha, c->ncol, c->n, c->s)
+	               : ccm_launch_axpy(x, sdir, alpha, c->n, c->s);
+}
+
+static int negate(cg_ctx *c, float *sdir, const float *g)
+{
+	return c->ncol ? ccm_launch_negate_padded(sdir, g, c->ncol, c->n, c->s)
+	               : ccm_launch_negate(sdir, g, c->n, c->s);
+}
+
+static int direction(cg_ctx *c, float *sdir, const float *g, float beta)
+{
+	return c->ncol ? ccm_launch_direction_padded(sdir, g, beta, c->ncol, c->n, c->s)
+	               : ccm_launch_direction(sdir, g, beta, c->n, c->s);
+}
+
+/* The padded layout of evaluate_hip's alignment, when x starts with zeros
+ * in its a >= 21 padding: the gradient there is 2 lambda_pair x
+ * (evaluate_cpu.c:158-161), so x, g and s stay zero there for the whole
+ * run and the vector steps and dots may skip it without changing a bit.
+ * 0 (visit every entry) for any other callback, size or x. */
+static int padded_columns(cg_ctx *c, const float *d_x, conjugrad_evaluate_t evaluate, void *instance)
+{
+	if (evaluate != evaluate_hip || !instance || ((uintptr_t)d_x & 15u) != 0)
+		return 0;
+	const userdata *ud = (const userdata *)instance;
+	if (ud->ncol <= 0 || (long)ccm_nvar_padded(ud->ncol) != c->n)
+		return 0;
+	int32_t *flag = (int32_t *)c->d_out;
+	if (hipMemsetAsync(flag, 0, sizeof(int32_t), c->s) != hipSuccess ||
+	    ccm_launch_pad_scan_x(d_x, ud->ncol, c->n, flag, c->s) != 0 ||
+	    hipMemcpyAsync(c->h_out, flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->s) != hipSuccess ||
+	    hipStreamSynchronize(c->s) != hipSuccess)
+		return -1;
+	return *(const int32_t *)c->h_out ? 0 : ud->ncol;
+}
 
 int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_evaluate_t evaluate,
                          conjugrad_progress_t progress, void *instance,
@@ -94,6 +137,15 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 	hist = (double *)malloc(sizeof(double) * (size_t)(p->max_iterations + 1));
 	if (!hist)
 		goto done;
+	/* zeroed, so that the padding of every vector is zero whichever
+	 * launchers run */
+	if (hipMemsetAsync(g, 0, sizeof(float) * (size_t)n, c.s) != hipSuccess ||
+	    hipMemsetAsync(gprev, 0, sizeof(float) * (size_t)n, c.s) != hipSuccess ||
+	    hipMemsetAsync(sdir, 0, sizeof(float) * (size_t)n, c.s) != hipSuccess)
+		goto done;
+	c.ncol = padded_columns(&c, d_x, evaluate, instance);
+	if (c.ncol < 0)
+		goto done;
 
 	f = evaluate(instance, d_x, g, n);
 	nevals++;
@@ -106,7 +158,7 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 		ret = CONJUGRAD_ALREADY_MINIMIZED;
 		goto done;
 	}
-	if (ccm_launch_negate(sdir, g, n, c.s))
+	if (negate(&c, sdir, g))
 		goto done;
 	double dg0 = -gg;
 	double alpha = 1.0 / sqrt(gg);
@@ -122,7 +174,7 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 		double moved = 0.0, ft = 0.0, lo = 0.0, hi = INFINITY;
 		int accepted = 0, ls;
 		for (ls = 1; ls <= p->max_linesearch; ls++) {
-			if (ccm_launch_axpy(d_x, sdir, (float)(alpha - moved), n, c.s) || sync_stream(&c)) {
+			if (axpy(&c, d_x, sdir, (float)(alpha - moved)) || sync_stream(&c)) {
 				ret = CONJUGRADERR_DEVICE;
 				goto done;
 			}
@@ -157,7 +209,7 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 		if (!accepted && lo > 0.0) {
 			/* last trial failed the decrease test: return to the longest
 			 * step that passed it */
-			if (ccm_launch_axpy(d_x, sdir, (float)(lo - moved), n, c.s) || sync_stream(&c)) {
+			if (axpy(&c, d_x, sdir, (float)(lo - moved)) || sync_stream(&c)) {
 				ret = CONJUGRADERR_DEVICE;
 				goto done;
 			}
@@ -171,7 +223,7 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 			accepted = 1;
 		}
 		if (!accepted) {
-			if (ccm_launch_axpy(d_x, sdir, (float)(-moved), n, c.s) || sync_stream(&c)) {
+			if (axpy(&c, d_x, sdir, (float)(-moved)) || sync_stream(&c)) {
 				ret = CONJUGRADERR_DEVICE;
 				goto done;
 			}
@@ -183,7 +235,7 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 				ret = CONJUGRADERR_MAXIMUMLINESEARCH;
 				goto done;
 			}
-			if (ccm_launch_negate(sdir, g, n, c.s)) {
+			if (negate(&c, sdir, g)) {
 				ret = CONJUGRADERR_DEVICE;
 				goto done;
 			}
@@ -226,14 +278,14 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 		/* slope of the new direction -g + beta s, from the dots above */
 		double dgn = -ggn + beta * dgs;
 		if (dgn >= 0) {
-			if (ccm_launch_negate(sdir, g, n, c.s)) {
+			if (negate(&c, sdir, g)) {
 				ret = CONJUGRADERR_DEVICE;
 				goto done;
 			}
 			dgn = -ggn;
 			steepest = 1;
 		} else {
-			if (ccm_launch_direction(sdir, g, (float)beta, n, c.s)) {
+			if (direction(&c, sdir, g, (float)beta)) {
 				ret = CONJUGRADERR_DEVICE;
 				goto done;
 			}

@@ -31,22 +31,27 @@ struct ccm_pad_rows {
     const uint32_t t = (uint32_t)(q - ns4), r = t / act4;
     return ns4 + (long)r * row4 + (t - r * act4);
   }
-  __device__ __forceinline__ long n_active() const { return ns4 + (long)nrows * act4; }
+  __host__ __device__ __forceinline__ long n_active() const { return ns4 + (long)nrows * act4; }
 };
+
+// the layout of an alignment of L columns (nsp singles, nvp floats in all);
+// none where nvp is not that layout or its indices do not fit
+__host__ __device__ __forceinline__ ccm_pad_rows ccm_pad_rows_make(int L, long nsp, long nvp) {
+  ccm_pad_rows p;
+  if ((nsp & 3) == 0 && (nvp & 3) == 0 && L > 0 && nsp >= (long)L * (CCM_NA - 1) &&
+      nvp == nsp + (long)L * L * CCM_NA * CCM_NAP && (nvp >> 2) - (nsp >> 2) < (1L << 32)) {
+    p.ns4 = nsp >> 2;
+    p.row4 = (uint32_t)L * (CCM_NAP / 4);
+    p.act4 = ((uint32_t)L * CCM_NA + 3) / 4;
+    p.nrows = (uint32_t)L * CCM_NA;
+  }
+  return p;
+}
 
 // the layout of alignment B, checking its padding flag when scan is false
 template <bool scan = false>
 __device__ __forceinline__ ccm_pad_rows ccm_pad_rows_of(const ccm_bprob& B) {
-  ccm_pad_rows p;
-  const long n4 = B.nvp >> 2;
-  if ((scan || !B.ctl->x_pad_nz) && (B.nsp & 3) == 0 && (B.nvp & 3) == 0 && B.L > 0 &&
-      B.nvp == B.nsp + (long)B.L * B.L * CCM_NA * CCM_NAP && n4 - (B.nsp >> 2) < (1L << 32)) {
-    p.ns4 = B.nsp >> 2;
-    p.row4 = (uint32_t)B.L * (CCM_NAP / 4);
-    p.act4 = ((uint32_t)B.L * CCM_NA + 3) / 4;
-    p.nrows = (uint32_t)B.L * CCM_NA;
-  }
-  return p;
+  return (scan || !B.ctl->x_pad_nz) ? ccm_pad_rows_make(B.L, B.nsp, B.nvp) : ccm_pad_rows{};
 }
 
 // up to three dot products sharing the first operand: out[i] = <a, b_i>
@@ -196,16 +201,63 @@ __global__ __launch_bounds__(256) void k_dots_final_grouped(const ccm_bprob* __r
 
 // One element of the vector step (one rounding per multiply-add, as in
 // k_axpy / k_direction / k_negate); gpi is read only by CCM_VEC_RESTART.
-__device__ __forceinline__ void cg_vec_elem(float& xi, float& si, float gi, float gpi, int mode, float cpre,
-                                            float ctr, float beta) {
+template <int MODE>
+__device__ __forceinline__ void cg_vec_elem(float& xi, float& si, float gi, float gpi, float cpre, float ctr,
+                                            float beta) {
   if (cpre != 0.0f) xi = __builtin_fmaf(cpre, si, xi);
-  if (mode == CCM_VEC_NEGATE)
+  if constexpr (MODE == CCM_VEC_NEGATE)
     si = -gi;
-  else if (mode == CCM_VEC_DIRECTION)
+  else if constexpr (MODE == CCM_VEC_DIRECTION)
     si = __builtin_fmaf(beta, si, -gi);
-  else if (mode == CCM_VEC_RESTART)
+  else if constexpr (MODE == CCM_VEC_RESTART)
     si = -gpi;
   if (ctr != 0.0f) xi = __builtin_fmaf(ctr, si, xi);
+}
+
+// the vector step of one alignment with the step kind fixed (a branch on
+// the kind inside the loop was compiled wrongly for one of the kinds)
+template <int MODE>
+__device__ __forceinline__ void cg_vec_block(const ccm_bprob& B, float cpre, float ctr, float beta,
+                                             uint32_t blk, uint32_t nblk) {
+  constexpr bool moves = MODE == CCM_VEC_NEGATE || MODE == CCM_VEC_DIRECTION;
+  float* __restrict__ x = B.x;
+  float* __restrict__ sd = B.s;
+  float* __restrict__ gp = B.gprev;
+  const float* __restrict__ g = B.g;
+  const ccm_pad_rows pad = ccm_pad_rows_of(B);
+  if (pad.row4 != 0 && (((uintptr_t)x | (uintptr_t)sd | (uintptr_t)gp | (uintptr_t)g) & 15u) == 0) {
+    float4* __restrict__ x4 = reinterpret_cast<float4*>(x);
+    float4* __restrict__ s4 = reinterpret_cast<float4*>(sd);
+    float4* __restrict__ gp4 = reinterpret_cast<float4*>(gp);
+    const float4* __restrict__ g4 = reinterpret_cast<const float4*>(g);
+    const long nq = pad.n_active();
+    for (long q = (long)blk * 256 + threadIdx.x; q < nq; q += (long)nblk * 256) {
+      const long i = pad.active(q);
+      float4 xv = x4[i], sv = s4[i];
+      float4 gv = make_float4(0.f, 0.f, 0.f, 0.f), pv = gv;
+      if constexpr (moves) gv = g4[i];
+      if constexpr (MODE == CCM_VEC_RESTART) pv = gp4[i];
+      cg_vec_elem<MODE>(xv.x, sv.x, gv.x, pv.x, cpre, ctr, beta);
+      cg_vec_elem<MODE>(xv.y, sv.y, gv.y, pv.y, cpre, ctr, beta);
+      cg_vec_elem<MODE>(xv.z, sv.z, gv.z, pv.z, cpre, ctr, beta);
+      cg_vec_elem<MODE>(xv.w, sv.w, gv.w, pv.w, cpre, ctr, beta);
+      x4[i] = xv;
+      if constexpr (MODE != CCM_VEC_KEEP) s4[i] = sv;
+      if constexpr (moves) gp4[i] = gv;
+    }
+    return;
+  }
+  const long n = B.nvp;
+  for (long i = (long)blk * 256 + threadIdx.x; i < n; i += (long)nblk * 256) {
+    float xi = x[i], si = sd[i];
+    float gi2 = 0.0f, gpi = 0.0f;
+    if constexpr (moves) gi2 = g[i];
+    if constexpr (MODE == CCM_VEC_RESTART) gpi = gp[i];
+    cg_vec_elem<MODE>(xi, si, gi2, gpi, cpre, ctr, beta);
+    x[i] = xi;
+    if constexpr (MODE != CCM_VEC_KEEP) sd[i] = si;
+    if constexpr (moves) gp[i] = gi2;
+  }
 }
 
 // The vector step of a round, as its control block says:
@@ -222,64 +274,85 @@ __global__ __launch_bounds__(256) void k_cg_vec_grouped(const ccm_bprob* __restr
   const ccm_bprob& B = probs[pidx[gi]];
   const ccm_cg_ctl* c = B.ctl;
   if (!c->vec_on) return;
-  const int mode = c->vec_mode;
   const float cpre = (float)c->c_pre, ctr = (float)c->c_trial, beta = (float)c->beta;
   const uint32_t blk = blockIdx.x - offs[gi], nblk = offs[gi + 1] - offs[gi];
-  float* __restrict__ x = B.x;
-  float* __restrict__ sd = B.s;
-  float* __restrict__ gp = B.gprev;
-  const float* __restrict__ g = B.g;
-  const bool moves = mode == CCM_VEC_NEGATE || mode == CCM_VEC_DIRECTION;
-  const ccm_pad_rows pad = ccm_pad_rows_of(B);
-  if (pad.row4 != 0 && (((uintptr_t)x | (uintptr_t)sd | (uintptr_t)gp | (uintptr_t)g) & 15u) == 0) {
-    float4* __restrict__ x4 = reinterpret_cast<float4*>(x);
-    float4* __restrict__ s4 = reinterpret_cast<float4*>(sd);
-    float4* __restrict__ gp4 = reinterpret_cast<float4*>(gp);
-    const float4* __restrict__ g4 = reinterpret_cast<const float4*>(g);
-    const long nq = pad.n_active();
-    for (long q = (long)blk * 256 + threadIdx.x; q < nq; q += (long)nblk * 256) {
-      const long i = pad.active(q);
-      float4 xv = x4[i], sv = s4[i];
-      float4 gv = make_float4(0.f, 0.f, 0.f, 0.f), pv = gv;
-      if (moves) gv = g4[i];
-      if (mode == CCM_VEC_RESTART) pv = gp4[i];
-      cg_vec_elem(xv.x, sv.x, gv.x, pv.x, mode, cpre, ctr, beta);
-      cg_vec_elem(xv.y, sv.y, gv.y, pv.y, mode, cpre, ctr, beta);
-      cg_vec_elem(xv.z, sv.z, gv.z, pv.z, mode, cpre, ctr, beta);
-      cg_vec_elem(xv.w, sv.w, gv.w, pv.w, mode, cpre, ctr, beta);
-      x4[i] = xv;
-      if (mode != CCM_VEC_KEEP) s4[i] = sv;
-      if (moves) gp4[i] = gv;
-    }
-    return;
-  }
-  const long n = B.nvp;
-  for (long i = (long)blk * 256 + threadIdx.x; i < n; i += (long)nblk * 256) {
-    float xi = x[i], si = sd[i];
-    const float gi2 = moves ? g[i] : 0.0f;
-    cg_vec_elem(xi, si, gi2, mode == CCM_VEC_RESTART ? gp[i] : 0.0f, mode, cpre, ctr, beta);
-    x[i] = xi;
-    if (mode != CCM_VEC_KEEP) sd[i] = si;
-    if (moves) gp[i] = gi2;
+  switch (c->vec_mode) {
+    case CCM_VEC_NEGATE: cg_vec_block<CCM_VEC_NEGATE>(B, cpre, ctr, beta, blk, nblk); break;
+    case CCM_VEC_DIRECTION: cg_vec_block<CCM_VEC_DIRECTION>(B, cpre, ctr, beta, blk, nblk); break;
+    case CCM_VEC_RESTART: cg_vec_block<CCM_VEC_RESTART>(B, cpre, ctr, beta, blk, nblk); break;
+    default: cg_vec_block<CCM_VEC_KEEP>(B, cpre, ctr, beta, blk, nblk); break;
   }
 }
 
 // ctl->x_pad_nz = 1 where x has a nonzero value in a float4 that the
 // padded layout skips; blockIdx.y is the alignment
-__global__ __launch_bounds__(256) void k_pad_scan(const ccm_bprob* __restrict__ probs) {
-  const ccm_bprob& B = probs[blockIdx.y];
-  const ccm_pad_rows pad = ccm_pad_rows_of<true>(B);
-  if (pad.row4 == 0 || pad.act4 >= pad.row4 || ((uintptr_t)B.x & 15u) != 0) return;
+__device__ __forceinline__ void pad_scan_block(const float* __restrict__ x, const ccm_pad_rows& pad,
+                                               int32_t* __restrict__ flag) {
+  if (pad.row4 == 0 || pad.act4 >= pad.row4 || ((uintptr_t)x & 15u) != 0) return;
   const uint32_t skip4 = pad.row4 - pad.act4;
   const uint32_t n = pad.nrows * skip4;  // below the 2^32 the layout guarantees
-  const float4* __restrict__ x4 = reinterpret_cast<const float4*>(B.x);
+  const float4* __restrict__ x4 = reinterpret_cast<const float4*>(x);
   bool nz = false;
   for (uint32_t t = blockIdx.x * 256 + threadIdx.x; t < n; t += gridDim.x * 256) {
     const uint32_t r = t / skip4, c = t - r * skip4;
     const float4 v = x4[pad.ns4 + (long)r * pad.row4 + pad.act4 + c];
     nz |= v.x != 0.0f || v.y != 0.0f || v.z != 0.0f || v.w != 0.0f;
   }
-  if (nz) B.ctl->x_pad_nz = 1;
+  if (nz) *flag = 1;
+}
+
+__global__ __launch_bounds__(256) void k_pad_scan(const ccm_bprob* __restrict__ probs) {
+  const ccm_bprob& B = probs[blockIdx.y];
+  pad_scan_block(B.x, ccm_pad_rows_of<true>(B), &B.ctl->x_pad_nz);
+}
+
+__global__ __launch_bounds__(256) void k_pad_scan_x(const float* __restrict__ x, ccm_pad_rows pad,
+                                                    int32_t* __restrict__ flag) {
+  pad_scan_block(x, pad, flag);
+}
+
+// ---- the single-alignment vector steps over the padded layout: only the
+// float4s that hold singles or pair entries a < 21 (same arithmetic as
+// k_axpy / k_direction / k_negate, which the padding leaves unchanged when
+// its direction and gradient are zero)
+__global__ __launch_bounds__(256) void k_dots_partial_pad(const float* __restrict__ a,
+                                                          const float* __restrict__ b0,
+                                                          const float* __restrict__ b1,
+                                                          const float* __restrict__ b2, long n, int nd,
+                                                          double* __restrict__ part, ccm_pad_rows pad) {
+  dots_partial_block(a, b0, b1, b2, n, nd, part, blockIdx.x, gridDim.x, pad);
+}
+
+// MODE 0: x += alpha s;  1: s = -g + alpha s;  2: s = -g  (y is x or s, v
+// is s or g).  The step is a template argument: with a three-way branch on
+// a kernel argument the compiler stored uncomputed registers for mode 2.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_vec_pad(float* __restrict__ y, const float* __restrict__ v,
+                                                 float alpha, ccm_pad_rows pad) {
+  float4* __restrict__ y4 = reinterpret_cast<float4*>(y);
+  const float4* __restrict__ v4 = reinterpret_cast<const float4*>(v);
+  const long nq = pad.n_active();
+  for (long q = (long)blockIdx.x * 256 + threadIdx.x; q < nq; q += (long)gridDim.x * 256) {
+    const long i = pad.active(q);
+    const float4 vv = v4[i];
+    float4 yv;
+    if constexpr (MODE == 0) {
+      yv = y4[i];
+      yv.x = __builtin_fmaf(alpha, vv.x, yv.x);
+      yv.y = __builtin_fmaf(alpha, vv.y, yv.y);
+      yv.z = __builtin_fmaf(alpha, vv.z, yv.z);
+      yv.w = __builtin_fmaf(alpha, vv.w, yv.w);
+    } else if constexpr (MODE == 1) {
+      yv = y4[i];
+      yv.x = __builtin_fmaf(alpha, yv.x, -vv.x);
+      yv.y = __builtin_fmaf(alpha, yv.y, -vv.y);
+      yv.z = __builtin_fmaf(alpha, yv.z, -vv.z);
+      yv.w = __builtin_fmaf(alpha, yv.w, -vv.w);
+    } else {
+      yv = make_float4(-vv.x, -vv.y, -vv.z, -vv.w);
+    }
+    y4[i] = yv;
+  }
 }
 
 // Line search and direction control of one alignment after an evaluation
@@ -511,6 +584,69 @@ int ccm_launch_direction(float* sdir, const float* g, float beta, long n, hipStr
 
 int ccm_launch_negate(float* sdir, const float* g, long n, hipStream_t s) {
   hipLaunchKernelGGL(k_negate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sdir, g, n);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+// ---- single alignment, padded layout of ncol columns (n = nvar_padded)
+
+static bool pad_layout(int ncol, long n, ccm_pad_rows* p) {
+  if (ncol <= 0) return false;
+  const long ns = (long)ncol * (CCM_NA - 1);
+  const long nsp = ns + CCM_NAP - ns % CCM_NAP;
+  *p = ccm_pad_rows_make(ncol, nsp, n);
+  return p->row4 != 0;
+}
+
+static bool aligned16(const void* a, const void* b) { return ((((uintptr_t)a) | ((uintptr_t)b)) & 15u) == 0; }
+
+static unsigned pad_grid(const ccm_pad_rows& p) {
+  const long want = (p.n_active() + 255) / 256;
+  return (unsigned)(want < 8192 ? want : 8192);
+}
+
+int ccm_launch_pad_scan_x(const float* x, int ncol, long n, int32_t* d_flag, hipStream_t s) {
+  ccm_pad_rows p;
+  if (!x || !d_flag || !pad_layout(ncol, n, &p)) return CCM_ERR_ARG;
+  hipLaunchKernelGGL(k_pad_scan_x, dim3(1024), dim3(256), 0, s, x, p, d_flag);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_dots_padded(const float* a, const float* b0, const float* b1, const float* b2, long n,
+                           int nd, int ncol, double* part, double* out, hipStream_t s) {
+  ccm_pad_rows p;
+  if (nd < 1 || nd > 3 || n <= 0) return CCM_ERR_ARG;
+  if (!pad_layout(ncol, n, &p)) return ccm_launch_dots(a, b0, b1, b2, n, nd, part, out, s);
+  const int nblk = ccm_dots_blocks(n);
+  hipLaunchKernelGGL(k_dots_partial_pad, dim3(nblk), dim3(256), 0, s, a, b0, b1 ? b1 : b0, b2 ? b2 : b0, n,
+                     nd, part, p);
+  CCM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_dots_final, dim3(1), dim3(256), 0, s, part, nblk, nd, out);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_axpy_padded(float* x, const float* sdir, float alpha, int ncol, long n, hipStream_t s) {
+  ccm_pad_rows p;
+  if (!pad_layout(ncol, n, &p) || !aligned16(x, sdir)) return ccm_launch_axpy(x, sdir, alpha, n, s);
+  hipLaunchKernelGGL(k_vec_pad<0>, dim3(pad_grid(p)), dim3(256), 0, s, x, sdir, alpha, p);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_direction_padded(float* sdir, const float* g, float beta, int ncol, long n, hipStream_t s) {
+  ccm_pad_rows p;
+  if (!pad_layout(ncol, n, &p) || !aligned16(sdir, g)) return ccm_launch_direction(sdir, g, beta, n, s);
+  hipLaunchKernelGGL(k_vec_pad<1>, dim3(pad_grid(p)), dim3(256), 0, s, sdir, g, beta, p);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_negate_padded(float* sdir, const float* g, int ncol, long n, hipStream_t s) {
+  ccm_pad_rows p;
+  if (!pad_layout(ncol, n, &p) || !aligned16(sdir, g)) return ccm_launch_negate(sdir, g, n, s);
+  hipLaunchKernelGGL(k_vec_pad<2>, dim3(pad_grid(p)), dim3(256), 0, s, sdir, g, 0.0f, p);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }

@@ -115,6 +115,21 @@ int ccm_launch_dots(const float *a, const float *b0, const float *b1, const floa
 int ccm_launch_axpy(float *x, const float *sdir, float alpha, long n, ccm_stream_t s);
 int ccm_launch_direction(float *sdir, const float *g, float beta, long n, ccm_stream_t s);
 int ccm_launch_negate(float *sdir, const float *g, long n, ccm_stream_t s);
+/* The same over the padded layout of an alignment with ncol columns
+ * (n = nvar_padded(ncol)), visiting only the float4s that hold singles or
+ * pair entries a < 21.  Valid while the a >= 21 padding of the operands is
+ * zero, as it stays in a CG whose gradient is evaluate_hip's (2 lambda_pair
+ * x there) from an x with zero padding: the results then equal the
+ * unpadded launchers' bit for bit (the dots keep their block partition).
+ * Any other n falls back to the unpadded launchers.  pad_scan_x sets
+ * *d_flag to 1 if x holds a nonzero value in the skipped float4s. */
+int ccm_launch_pad_scan_x(const float *x, int ncol, long n, int32_t *d_flag, ccm_stream_t s);
+int ccm_launch_dots_padded(const float *a, const float *b0, const float *b1, const float *b2, long n,
+                           int nd, int ncol, double *part, double *out, ccm_stream_t s);
+int ccm_launch_axpy_padded(float *x, const float *sdir, float alpha, int ncol, long n, ccm_stream_t s);
+int ccm_launch_direction_padded(float *sdir, const float *g, float beta, int ncol, long n,
+                                ccm_stream_t s);
+int ccm_launch_negate_padded(float *sdir, const float *g, int ncol, long n, ccm_stream_t s);
 
 /* contact scores on the device (util.c:16-139): the L x L map of
  * sum_submatrices from the couplings of x, then apc (if apc) and the [0,1]
