@@ -542,7 +542,7 @@ int ccm_launch_cg_vec_grouped(const ccm_bprob* d_probs, ccm_group g, hipStream_t
 }
 
 int ccm_launch_pad_scan(const ccm_bprob* d_probs, int nprob, hipStream_t s) {
-  if (nprob <= 0 || !d_probs) return CCM_ERR_ARG;
+  if (nprob <= 0 || nprob > 65535 || !d_probs) return CCM_ERR_ARG;  // one grid row per alignment
   hipLaunchKernelGGL(k_pad_scan, dim3(256, (unsigned)nprob), dim3(256), 0, s, d_probs);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
