@@ -78,21 +78,86 @@ __device__ __forceinline__ void convert_wt_block(const float* __restrict__ x2, i
   }
 }
 
+// The same conversion over 64 rows and CW_TC = 128 columns with 16-byte
+// loads (each row segment read is 512 contiguous bytes): for x addresses
+// that are 16-byte aligned.  A float4 that straddles column 21 L reads a few
+// padding columns of the same row, which are not written.
+#define CW_TC 128
+__device__ __forceinline__ void convert_wt_block128(const float* __restrict__ x2, int L, int Lp,
+                                                    uint16_t* __restrict__ wt, long lda, int passes,
+                                                    uint32_t tiles_r, uint32_t blk) {
+  __shared__ float T[64][CW_TC + 1];
+  const int P = CCM_NA * Lp;
+  const int C21 = CCM_NA * L;
+  const int tc = blk / tiles_r, tr = blk % tiles_r;
+  const long ld2 = (long)CCM_NAP * L;
+  {
+    const int b = (tr * 64) / Lp, k0 = tr * 64 - b * Lp;
+    const float* src = x2 + (long)(b * L + k0) * ld2 + tc * CW_TC;
+    float4 v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int f = threadIdx.x + 256 * i, rr = f >> 5, c4 = f & 31;
+      v[i] = (tc * CW_TC + c4 * 4 < C21 && k0 + rr < L)
+                 ? *reinterpret_cast<const float4*>(src + (long)rr * ld2 + c4 * 4)
+                 : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int f = threadIdx.x + 256 * i, rr = f >> 5, c4 = f & 31;
+      T[rr][c4 * 4 + 0] = v[i].x;
+      T[rr][c4 * 4 + 1] = v[i].y;
+      T[rr][c4 * 4 + 2] = v[i].z;
+      T[rr][c4 * 4 + 3] = v[i].w;
+    }
+  }
+  __syncthreads();
+  // each thread converts 32 consecutive r of one output row c (four 16-byte
+  // stores, and the residual half likewise)
+  const int cl = threadIdx.x >> 1, half = threadIdx.x & 1;
+  const int C = tc * CW_TC + cl;
+  if (C >= C21) return;
+  const int a = C / L;
+  const int c = a * Lp + (C - a * L), r0 = tr * 64 + half * 32;
+  uint32_t hi[16], lo[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const float v0 = T[half * 32 + 2 * i][cl], v1 = T[half * 32 + 2 * i + 1][cl];
+    hi[i] = ccm_f2bf2(v0, v1);
+    lo[i] = ccm_f2bf2(v0 - ccm_bf2f((uint16_t)hi[i]), v1 - ccm_bf2f((uint16_t)(hi[i] >> 16)));
+  }
+  uint4* dst = reinterpret_cast<uint4*>(wt + (long)c * lda + r0);
+#pragma unroll
+  for (int q = 0; q < 4; q++) dst[q] = make_uint4(hi[4 * q], hi[4 * q + 1], hi[4 * q + 2], hi[4 * q + 3]);
+  if (passes > 1) {
+    uint4* dlo = reinterpret_cast<uint4*>(wt + (long)c * lda + P + r0);
+#pragma unroll
+    for (int q = 0; q < 4; q++) dlo[q] = make_uint4(lo[4 * q], lo[4 * q + 1], lo[4 * q + 2], lo[4 * q + 3]);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_convert_wt128(const float* __restrict__ x2, int L, int Lp,
+                                                       uint16_t* __restrict__ wt, long lda, int passes,
+                                                       uint32_t tiles_r) {
+  convert_wt_block128(x2, L, Lp, wt, lda, passes, tiles_r, blockIdx.x);
+}
+
 __global__ __launch_bounds__(256) void k_convert_wt(const float* __restrict__ x2, int L, int Lp,
                                                     uint16_t* __restrict__ wt, long lda, int passes,
                                                     uint32_t tiles_r) {
   convert_wt_block(x2, L, Lp, wt, lda, passes, tiles_r, blockIdx.x);
 }
 
-// every alignment of a group: (P/64)^2 blocks each
+// every alignment of a group: ccm_convert_wt_blocks(L, Lp) blocks each (x
+// buffers of a batch are allocated, so 16-byte aligned)
 __global__ __launch_bounds__(256) void k_convert_wt_grouped(const ccm_bprob* __restrict__ probs,
                                                             const uint32_t* __restrict__ offs,
                                                             const int32_t* __restrict__ pidx, int np) {
   const int gi = ccm_group_find(offs, np, blockIdx.x);
   const ccm_bprob& B = probs[pidx[gi]];
   if (!B.ctl->eval_on) return;
-  convert_wt_block(B.x + B.nsp, B.L, B.Lp, B.wt, B.ldwt, B.passes_fwd, (uint32_t)((B.P + 63) / 64),
-                   blockIdx.x - offs[gi]);
+  convert_wt_block128(B.x + B.nsp, B.L, B.Lp, B.wt, B.ldwt, B.passes_fwd, (uint32_t)((B.P + 63) / 64),
+                      blockIdx.x - offs[gi]);
 }
 
 // One-hot fragments from MSA bytes with packed 16-bit integer arithmetic.
@@ -949,9 +1014,16 @@ int ccm_launch_convert_wt(const float* d_x, int L, int Lp, uint16_t* d_wt, long 
                           hipStream_t s) {
   const int P = CCM_NA * Lp;
   if (Lp % 64 || Lp < L || lda < (long)passes * P || lda % 8) return CCM_ERR_ARG;
-  const uint32_t tiles_r = (uint32_t)((P + 63) / 64), tiles_c = (uint32_t)((CCM_NA * L + 63) / 64);
-  hipLaunchKernelGGL(k_convert_wt, dim3(tiles_c * tiles_r), dim3(256), 0, s, d_x + ccm_nsp(L), L, Lp,
-                     d_wt, lda, passes, tiles_r);
+  const uint32_t tiles_r = (uint32_t)((P + 63) / 64);
+  if (((uintptr_t)d_x & 15u) == 0) {
+    const uint32_t tiles_c = (uint32_t)((CCM_NA * L + CW_TC - 1) / CW_TC);
+    hipLaunchKernelGGL(k_convert_wt128, dim3(tiles_c * tiles_r), dim3(256), 0, s, d_x + ccm_nsp(L), L, Lp,
+                       d_wt, lda, passes, tiles_r);
+  } else {
+    const uint32_t tiles_c = (uint32_t)((CCM_NA * L + 63) / 64);
+    hipLaunchKernelGGL(k_convert_wt, dim3(tiles_c * tiles_r), dim3(256), 0, s, d_x + ccm_nsp(L), L, Lp,
+                       d_wt, lda, passes, tiles_r);
+  }
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }
@@ -1136,7 +1208,7 @@ int ccm_gemm_default_variant_for(int mode, int Lp, int ncols) {
 }
 
 int ccm_convert_wt_blocks(int L, int Lp) {
-  return ((CCM_NA * Lp + 63) / 64) * ((CCM_NA * L + 63) / 64);
+  return ((CCM_NA * Lp + 63) / 64) * ((CCM_NA * L + CW_TC - 1) / CW_TC);
 }
 
 // grouped forms: g.offs holds each alignment's block range; every alignment

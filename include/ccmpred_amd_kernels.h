@@ -56,7 +56,9 @@ int ccm_launch_softmax_cols(const float *d_zt, long ldzt, const float *d_x1, con
 
 /* BF16 MFMA contractions.  convert_wt: Wt[a*Lp + j][b*Lp + k] = W(b,k,a,j)
  * in BF16 (and the residual half at +21*Lp for passes = 2), zeros for
- * k >= L; rows with j >= L are not written (zero Wt once). */
+ * k >= L; rows with j >= L are not written (zero Wt once).  A 16-byte
+ * aligned d_x takes 16-byte loads over 128-column blocks, any other d_x
+ * 4-byte loads over 64-column blocks; both write the same Wt. */
 int ccm_launch_convert_wt(const float *d_x, int L, int Lp, uint16_t *d_wt, long lda, int passes,
                           ccm_stream_t s);
 int ccm_gemm_variants(void);
@@ -147,7 +149,7 @@ int ccm_launch_contact_map(const float *d_x, int L, int apc, int normalize, floa
  * d_probs is a device array of ccm_bprob and a group's offs/pidx live in
  * device memory.  Alignments whose control block has eval_on == 0
  * (vec_on == 0 for the vector step) return at once. */
-int ccm_convert_wt_blocks(int L, int Lp);
+int ccm_convert_wt_blocks(int L, int Lp); /* (21 Lp / 64) x ceil(21 L / 128) */
 uint32_t ccm_gemm_tile_count(int variant, int M, int Ncol);
 int ccm_gemm_variant_ok(int variant, int mode, int Lp, int kcols);
 int ccm_gemm_variant_direct(int variant);

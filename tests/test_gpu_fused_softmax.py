@@ -128,3 +128,36 @@ def test_fused_matches_unfused(lib, L, N, pf, pd, variant):
     print(f"  Dt equal, f unfused={fa:.15g} fused={fb:.15g}, node gradient rel diff={e:.2e}", flush=True)
     assert abs(fa - fb) <= 1e-12 * abs(fa)
     assert e <= 1e-6
+
+
+@pytest.mark.parametrize("L,pf", [(1, 1), (37, 2), (130, 1), (250, 2)])
+def test_convert_wt_aligned_and_unaligned_x(lib, L, pf):
+    """ccm_launch_convert_wt takes 16-byte loads for a 16-byte aligned x and
+    4-byte loads otherwise: both write the same Wt, bit for bit."""
+    print(f"convert_wt L={L} passes={pf}", flush=True)
+    rng = np.random.default_rng(L + pf)
+    nsp = nsingle_padded(L)
+    n = nsp + NA * NAP * L * L
+    x = rng.normal(0, 0.3, n).astype(np.float32)
+    Lp = (L + 63) // 64 * 64
+    P = NA * Lp
+    lda = pf * P
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    buf = torch.from_numpy(np.concatenate([np.zeros(4, np.float32), x])).cuda()
+    xa = buf[4:]      # 16-byte aligned (torch allocations are)
+    xu = torch.empty(n + 1, dtype=torch.float32, device="cuda")
+    xu[1:] = xa       # x at a 4-byte offset
+    assert xa.data_ptr() % 16 == 0 and xu[1:].data_ptr() % 16 == 4
+    wa = torch.zeros(P * lda, dtype=torch.int16, device="cuda")
+    wu = torch.zeros_like(wa)
+    assert lib.ccm_launch_convert_wt(C.c_void_p(xa.data_ptr()), L, Lp, ptr(wa), lda, pf, s) == 0
+    assert lib.ccm_launch_convert_wt(C.c_void_p(xu[1:].data_ptr()), L, Lp, ptr(wu), lda, pf, s) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(wa, wu)
+    # and the values: hi part of Wt[a*Lp + j][b*Lp + k] is W(b,k,a,j) rounded to BF16
+    W = x[nsp:].reshape(NA, L, NAP, L)[:, :, :NA, :]
+    hi = wa.view(P, lda)[:, :P].cpu().numpy().view(np.uint16).reshape(NA, Lp, NA, Lp)
+    got = (hi[:, :L, :, :L].astype(np.uint32) << 16).view(np.float32)   # [a, j, b, k]
+    ref = W.transpose(2, 3, 0, 1)                                          # [a, j, b, k]
+    assert np.all(np.abs(got - ref) <= np.abs(ref) * 2.0 ** -8)
+    assert not hi[:, :L, :, L:].any()   # columns k >= L are zero
