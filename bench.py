@@ -132,14 +132,8 @@ def cpu_model():
 
 
 def batched_workload(nmsa, seed=999):
-    """nmsa synthetic MSAs, L uniform in [50, 600], N log-uniform in
-    [200, 20000], MSA i generated with seed 1000 + i and L // 4 planted pairs."""
-    from ccmpred_amd.msa import synthetic_msa
-    rng = np.random.Generator(np.random.PCG64(seed))
-    Ls = rng.integers(50, 601, nmsa)
-    Ns = np.round(np.exp(rng.uniform(np.log(200), np.log(20000), nmsa))).astype(int)
-    return [synthetic_msa(int(L), int(N), 1000 + i, max(1, int(L) // 4))[0]
-            for i, (L, N) in enumerate(zip(Ls, Ns))]
+    from ccmpred_amd.msa import batched_workload as workload
+    return workload(nmsa, seed)
 
 
 def run_batched(args):

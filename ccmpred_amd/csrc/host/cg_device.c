@@ -8,20 +8,27 @@
  *   per iteration, line search along s from x (x moves in place), at most
  *   max_linesearch trials:
  *     x += (alpha - moved) s ; f_t, g_t = evaluate(x)
- *     Armijo fails (f_t > f0 + ftol alpha <g0,s>): hi = alpha, shrink
- *       (alpha *= alpha_mul, or bisect [lo, hi] once a lower bound exists)
- *     strong Wolfe (|<g_t,s>| <= wolfe |<g0,s>|), or Armijo on the last
- *       trial: accept
+ *     sufficient decrease fails: hi = alpha, shrink to the minimiser of the
+ *       quadratic through f0, <g0,s> and f_t, kept within
+ *       [0.1 alpha, alpha_mul alpha] (or bisect [lo, hi] once a lower
+ *       bound exists).  Sufficient decrease is the Armijo test
+ *       f_t <= f0 + ftol alpha <g0,s>, or, below the objective's resolution
+ *       (CG_FRES |f0|), the approximate Armijo test of Hager and Zhang:
+ *       f_t <= f0 + CG_FRES |f0| and <g_t,s> <= (2 ftol - 1) <g0,s>
+ *     strong Wolfe (|<g_t,s>| <= wolfe |<g0,s>|), or sufficient decrease on
+ *       the last trial: accept
  *     otherwise <g_t,s> > 0 sets hi, < 0 sets lo; extend (alpha /= alpha_mul)
  *       while unbracketed, else bisect
- *   if the last trial fails Armijo the step returns to the longest one that
- *   passed (one more evaluation); with none, x returns to the start, and a
+ *   if the last trial fails the decrease test the step returns to the longest
+ *   one that passed it (one more evaluation); with none, x returns to the start, and a
  *   failed steepest-descent search ends the run with
  *   CONJUGRADERR_MAXIMUMLINESEARCH, otherwise the direction is reset to -g
  *   beta = max(0, (<g,g> - <g,g_prev>) / <g_prev,g_prev>)   (Polak-Ribiere+)
  *   s = -g + beta s unless its slope -<g,g> + beta <g,s_old> >= 0, then s = -g
  *   next alpha = alpha * <g_prev, s_prev> / <g, s>
- *   stop with CONJUGRAD_SUCCESS once (f_{it-k} - f_it) / |f_it| < epsilon.
+ *   stop with CONJUGRAD_SUCCESS once (f_{it-k} - f_it) / |f_it| < epsilon
+ *   (epsilon > 0; epsilon = 0 runs max_iterations: the approximate decrease
+ *   test lets f rise within its resolution, which must not end the run).
  * All scalars, dot products and tests are FP64; vectors are FP32.
  */
 #define __HIP_PLATFORM_AMD__
@@ -43,6 +50,29 @@ void ccm_conjugrad_init(ccm_conjugrad_parameter *p)
 	p->alpha_mul = 0.5;
 	p->ftol = 1e-4;
 	p->wolfe = 0.2;
+}
+
+/* Sufficient decrease (see above).  CG_FRES is the relative resolution
+ * below which differences of f are not trusted: the BF16 forms quantise
+ * the couplings, so their f moves in steps near the optimum while the
+ * gradient, and with it the slope <g_t,s>, still resolves the search. */
+#define CG_FRES 1e-6
+static int cg_decrease(double ft, double f0, double alpha, double dg0, double dgt, double ftol)
+{
+	if (ft <= f0 + ftol * alpha * dg0)
+		return 1;
+	return ft <= f0 + CG_FRES * fabs(f0) && dgt <= (2.0 * ftol - 1.0) * dg0;
+}
+
+/* Step after a trial failed the decrease test with no lower bracket: the
+ * minimiser of the quadratic through f0, dg0 = <g0,s> and ft (Nocedal and
+ * Wright, Numerical Optimization, eq. 3.57) within [0.1, alpha_mul] alpha */
+static double cg_shrink(double ft, double f0, double alpha, double dg0, double alpha_mul)
+{
+	const double den = 2.0 * ((ft - f0) - dg0 * alpha);
+	const double aq = (-dg0 * alpha) * alpha / den;
+	const double amin = 0.1 * alpha, amax = alpha_mul * alpha;
+	return !(aq >= amin) ? amin : (aq > amax ? amax : aq);
 }
 
 typedef struct {
@@ -146,6 +176,10 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 	c.ncol = padded_columns(&c, d_x, evaluate, instance);
 	if (c.ncol < 0)
 		goto done;
+	/* the callback may write g on a stream of its own: the zeroing above
+	 * must be complete before it runs, whichever path padded_columns took */
+	if (sync_stream(&c))
+		goto done;
 
 	f = evaluate(instance, d_x, g, n);
 	nevals++;
@@ -185,16 +219,16 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 				ret = CONJUGRADERR_DEVICE;
 				goto done;
 			}
-			if (!(ft <= f0 + p->ftol * alpha * dg0)) {
-				/* not enough decrease: shrink, bisecting once bracketed */
-				hi = alpha;
-				alpha = lo > 0.0 ? 0.5 * (lo + hi) : alpha * p->alpha_mul;
-				continue;
-			}
 			double dgt;
 			if (dots(&c, g, sdir, NULL, NULL, 1, &dgt)) {
 				ret = CONJUGRADERR_DEVICE;
 				goto done;
+			}
+			if (!cg_decrease(ft, f0, alpha, dg0, dgt, p->ftol)) {
+				/* not enough decrease: shrink, bisecting once bracketed */
+				hi = alpha;
+				alpha = lo > 0.0 ? 0.5 * (lo + hi) : cg_shrink(ft, f0, alpha, dg0, p->alpha_mul);
+				continue;
 			}
 			if (fabs(dgt) <= -p->wolfe * dg0 || ls == p->max_linesearch) {
 				accepted = 1;
@@ -265,7 +299,7 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 			progress(instance, d_x, g, (conjugrad_float_t)f, (conjugrad_float_t)sqrt(xx),
 			         (conjugrad_float_t)sqrt(ggn), (conjugrad_float_t)alpha, n, it, ls);
 		}
-		if (p->k > 0 && it > p->k) {
+		if (p->epsilon > 0.0 && p->k > 0 && it > p->k) {
 			const double before = hist[it - 1 - p->k];
 			if ((before - f) / fabs(f) < p->epsilon) {
 				ret = CONJUGRAD_SUCCESS;

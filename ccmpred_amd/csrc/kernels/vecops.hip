@@ -355,6 +355,27 @@ __global__ __launch_bounds__(256) void k_vec_pad(float* __restrict__ y, const fl
   }
 }
 
+// The sufficient-decrease test and the shrinking step of the host loop in
+// cg_device.c (cg_decrease, cg_shrink), operation for operation: the Armijo
+// test, or below the objective's relative resolution CG_FRES the approximate
+// Armijo test on the slope (Hager and Zhang 2005); after a failure with no
+// lower bracket, the minimiser of the quadratic through f0, dg0 and ft
+// within [0.1, alpha_mul] alpha.
+#define CG_FRES 1e-6
+__device__ bool cg_decrease(double ft, double f0, double alpha, double dg0, double dgt, double ftol) {
+#pragma clang fp contract(off)
+  if (ft <= f0 + ftol * alpha * dg0) return true;
+  return ft <= f0 + CG_FRES * fabs(f0) && dgt <= (2.0 * ftol - 1.0) * dg0;
+}
+
+__device__ double cg_shrink(double ft, double f0, double alpha, double dg0, double alpha_mul) {
+#pragma clang fp contract(off)
+  const double den = 2.0 * ((ft - f0) - dg0 * alpha);
+  const double aq = (-dg0 * alpha) * alpha / den;
+  const double amin = 0.1 * alpha, amax = alpha_mul * alpha;
+  return !(aq >= amin) ? amin : (aq > amax ? amax : aq);
+}
+
 // Line search and direction control of one alignment after an evaluation
 // at x0 + moved s: the decisions of the host loop in cg_device.c, in the
 // same order and FP64 arithmetic (f rounded to float as evaluate returns
@@ -395,10 +416,10 @@ __device__ void cg_control(ccm_cg_ctl* c, double* log, int log_cap, const ccm_cg
     c->steepest = 1;
   } else if (c->phase == CCM_CG_FORCED) {
     accept = true;
-  } else if (!(ft <= c->f0 + p.ftol * c->alpha * c->dg0)) {
+  } else if (!cg_decrease(ft, c->f0, c->alpha, c->dg0, dgt, p.ftol)) {
     // not enough decrease: shrink, bisecting once bracketed
     c->hi = c->alpha;
-    c->alpha = c->lo > 0.0 ? 0.5 * (c->lo + c->hi) : c->alpha * p.alpha_mul;
+    c->alpha = c->lo > 0.0 ? 0.5 * (c->lo + c->hi) : cg_shrink(ft, c->f0, c->alpha, c->dg0, p.alpha_mul);
     c->ls++;
     if (c->ls > p.max_linesearch) exhausted = true;
   } else if (fabs(dgt) <= -p.wolfe * c->dg0 || c->ls == p.max_linesearch) {
@@ -442,7 +463,7 @@ __device__ void cg_control(ccm_cg_ctl* c, double* log, int log_cap, const ccm_cg
       r[3] = ggn;
     }
     c->it++;
-    if (p.k > 0 && c->it > p.k && c->it - 1 - p.k < log_cap) {
+    if (p.epsilon > 0.0 && p.k > 0 && c->it > p.k && c->it - 1 - p.k < log_cap) {
       const double before = log[4L * (c->it - 1 - p.k)];
       if ((before - ft) / fabs(ft) < p.epsilon) {
         c->status = 0;  // CONJUGRAD_SUCCESS

@@ -131,3 +131,14 @@ WORKLOADS = {
 def workload_msa(name: str):
     ncol, nrow, seed, npairs = WORKLOADS[name]
     return synthetic_msa(ncol, nrow, seed, npairs)
+
+
+def batched_workload(nmsa: int = 64, seed: int = 999):
+    """The batched workload: ``nmsa`` synthetic MSAs with L uniform in
+    [50, 600] and N log-uniform in [200, 20000]; MSA i is generated with seed
+    1000 + i and L // 4 planted pairs."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    Ls = rng.integers(50, 601, nmsa)
+    Ns = np.round(np.exp(rng.uniform(np.log(200), np.log(20000), nmsa))).astype(int)
+    return [synthetic_msa(int(L), int(N), 1000 + i, max(1, int(L) // 4))[0]
+            for i, (L, N) in enumerate(zip(Ls, Ns))]

@@ -472,6 +472,33 @@ static double ora_dot(const float *a, const float *b, long n)
 	return total;
 }
 
+/* Sufficient decrease of a trial at step alpha (f0, dg0: objective and
+ * slope at the start of the search; ft, dgt: at the trial).  The Armijo
+ * test, or, where f differences sink below the objective's resolution
+ * (ORA_CG_FRES relative), the approximate form of Hager and Zhang (SIAM J.
+ * Optim. 16(1), 2005, eq. 4.1): the slope at the trial has not risen above
+ * (1 - 2 ftol) |dg0| -- the Armijo test of a quadratic, written with
+ * derivatives -- and f has not risen by more than that resolution. */
+#define ORA_CG_FRES 1e-6
+static int ora_cg_decrease(double ft, double f0, double alpha, double dg0, double dgt, double ftol)
+{
+	if (ft <= f0 + ftol * alpha * dg0)
+		return 1;
+	return ft <= f0 + ORA_CG_FRES * fabs(f0) && dgt <= (2.0 * ftol - 1.0) * dg0;
+}
+
+/* Next step after a trial that failed the decrease test with no lower
+ * bracket: the minimiser of the quadratic through f0, dg0 and ft
+ * (Nocedal and Wright, Numerical Optimization, 2nd ed., eq. 3.57),
+ * kept within [0.1 alpha, alpha_mul alpha]. */
+static double ora_cg_shrink(double ft, double f0, double alpha, double dg0, double alpha_mul)
+{
+	const double den = 2.0 * ((ft - f0) - dg0 * alpha);
+	const double aq = (-dg0 * alpha) * alpha / den;
+	const double amin = 0.1 * alpha, amax = alpha_mul * alpha;
+	return !(aq >= amin) ? amin : (aq > amax ? amax : aq);
+}
+
 enum { ORA_CG_SUCCESS = 0, ORA_CG_ALREADY_MINIMIZED = 1, ORA_CG_MAXITER = 2,
        ORA_CG_ERR_LINESEARCH = -1 };
 
@@ -512,13 +539,14 @@ int ora_cg(int n, float *x, double *fx, ora_eval_fn eval, void *ctx, const ora_c
 			moved = alpha;
 			ft = eval(ctx, x, g, n);
 			nevals++;
-			if (!(ft <= f0 + p->ftol * alpha * dg0)) {
+			const double dgt = ora_dot(g, s, n);
+			if (!ora_cg_decrease(ft, f0, alpha, dg0, dgt, p->ftol)) {
 				/* not enough decrease: shrink, bisecting once bracketed */
 				hi = alpha;
-				alpha = lo > 0.0 ? 0.5 * (lo + hi) : alpha * p->alpha_mul;
+				alpha = lo > 0.0 ? 0.5 * (lo + hi)
+				                 : ora_cg_shrink(ft, f0, alpha, dg0, p->alpha_mul);
 				continue;
 			}
-			const double dgt = ora_dot(g, s, n);
 			if (fabs(dgt) <= -p->wolfe * dg0 || ls == p->max_linesearch) {
 				accepted = 1;
 				break;
@@ -565,7 +593,7 @@ int ora_cg(int n, float *x, double *fx, ora_eval_fn eval, void *ctx, const ora_c
 		if (fhist)
 			fhist[it] = f;
 		it++;
-		if (p->k > 0 && it > p->k) {
+		if (p->epsilon > 0.0 && p->k > 0 && it > p->k) {
 			double before = hist[it - 1 - p->k];
 			if ((before - f) / fabs(f) < p->epsilon) {
 				ret = ORA_CG_SUCCESS;
@@ -772,22 +800,27 @@ typedef struct {
 	int ncol, nrow;
 	double l1, l2;
 	double *xd, *gd;
+	int par;
 } ora_pll64_ctx;
 
 static double ora_pll_eval64(void *vctx, const float *x, float *g, int n)
 {
 	ora_pll64_ctx *c = (ora_pll64_ctx *)vctx;
+	#pragma omp parallel for if (c->par)
 	for (long i = 0; i < n; i++)
 		c->xd[i] = x[i];
-	double f = ora_evaluate_f64(c->msa, c->w, c->ncol, c->nrow, c->l1, c->l2, c->xd, c->gd);
+	double f = c->par ? ora_evaluate_f64_par(c->msa, c->w, c->ncol, c->nrow, c->l1, c->l2, c->xd,
+	                                         c->gd)
+	                  : ora_evaluate_f64(c->msa, c->w, c->ncol, c->nrow, c->l1, c->l2, c->xd, c->gd);
+	#pragma omp parallel for if (c->par)
 	for (long i = 0; i < n; i++)
 		g[i] = (float)c->gd[i];
 	return f;
 }
 
-int ora_cg_pll_f64(const unsigned char *msa, const float *w, int ncol, int nrow,
-                   float lambda_single, float lambda_pair, float *x, const ora_cg_param *p,
-                   double *fx, int *iterations, int *evaluations, double *fhist)
+static int cg_pll_f64(const unsigned char *msa, const float *w, int ncol, int nrow,
+                      float lambda_single, float lambda_pair, float *x, const ora_cg_param *p,
+                      double *fx, int *iterations, int *evaluations, double *fhist, int par)
 {
 	const long n = ora_nvar_padded(ncol);
 	double *wd = (double *)malloc(sizeof(double) * (size_t)nrow);
@@ -795,10 +828,26 @@ int ora_cg_pll_f64(const unsigned char *msa, const float *w, int ncol, int nrow,
 		wd[i] = w[i];
 	ora_pll64_ctx c = {msa, wd, ncol, nrow, lambda_single, lambda_pair,
 	                   (double *)malloc(sizeof(double) * (size_t)n),
-	                   (double *)malloc(sizeof(double) * (size_t)n)};
+	                   (double *)malloc(sizeof(double) * (size_t)n), par};
 	int r = ora_cg((int)n, x, fx, ora_pll_eval64, &c, p, iterations, evaluations, fhist);
 	free(c.xd);
 	free(c.gd);
 	free(wd);
 	return r;
+}
+
+int ora_cg_pll_f64(const unsigned char *msa, const float *w, int ncol, int nrow,
+                   float lambda_single, float lambda_pair, float *x, const ora_cg_param *p,
+                   double *fx, int *iterations, int *evaluations, double *fhist)
+{
+	return cg_pll_f64(msa, w, ncol, nrow, lambda_single, lambda_pair, x, p, fx, iterations,
+	                  evaluations, fhist, 0);
+}
+
+int ora_cg_pll_f64par(const unsigned char *msa, const float *w, int ncol, int nrow,
+                      float lambda_single, float lambda_pair, float *x, const ora_cg_param *p,
+                      double *fx, int *iterations, int *evaluations, double *fhist)
+{
+	return cg_pll_f64(msa, w, ncol, nrow, lambda_single, lambda_pair, x, p, fx, iterations,
+	                  evaluations, fhist, 1);
 }

@@ -14,6 +14,10 @@
  *                           central differences, invariants  (tests/test_oracle_pll.py)
  *   ora_evaluate_omp_f32    agreement with ora_evaluate_f32  (tests/test_oracle_pll.py)
  *   ora_tied_fd_f64         full re-evaluation central differences (tests/test_oracle_pll.py)
+ *   ora_forward_lse_f32/_f64
+ *                           brute-force FP64 log-partitions, x=0 closed form
+ *                                                            (tests/test_oracle_pll.py)
+ *   ora_cg_pll_f64par       same run as ora_cg_pll_f64       (tests/test_golden_cg_maps.py)
  *   ora_sum_submatrices, ora_apc, ora_normalize
  *                           hand-computed cases              (tests/test_oracle_misc.py)
  *   ora_cg                  parity unpinned: the reference optimizer
@@ -100,8 +104,10 @@ void ora_normalize(float *mat, int ncol);
 /*
  * Host nonlinear conjugate gradient with the same algorithm as the device
  * CG in ccmpred_amd/csrc/host/cg_device.c (Polak-Ribiere+ beta, line search
- * with an Armijo test, a strong-Wolfe curvature test, extension of short
- * steps and bisection once bracketed; FP64 scalars, FP32 vectors).
+ * with an Armijo test -- approximate, on the slope, below a relative
+ * objective resolution of 1e-6 --, quadratic-interpolation backtracking,
+ * a strong-Wolfe curvature test, extension of short steps and bisection
+ * once bracketed; FP64 scalars, FP32 vectors).
  */
 typedef double (*ora_eval_fn)(void *ctx, const float *x, float *g, int n);
 typedef struct ora_cg_param {
@@ -121,6 +127,11 @@ int ora_cg(int n, float *x, double *fx, ora_eval_fn eval, void *ctx, const ora_c
 int ora_cg_pll_f64(const unsigned char *msa, const float *w, int ncol, int nrow,
                    float lambda_single, float lambda_pair, float *x, const ora_cg_param *p,
                    double *fx, int *iterations, int *evaluations, double *fhist);
+
+/* The same with ora_evaluate_f64_par (OpenMP), for the large alignments. */
+int ora_cg_pll_f64par(const unsigned char *msa, const float *w, int ncol, int nrow,
+                      float lambda_single, float lambda_pair, float *x, const ora_cg_param *p,
+                      double *fx, int *iterations, int *evaluations, double *fhist);
 
 /* Drives ora_cg with the FP32 oracle evaluator. */
 int ora_cg_pll_f32(const unsigned char *msa, const float *w, int ncol, int nrow,

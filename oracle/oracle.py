@@ -76,6 +76,8 @@ def _declare(L):
     L.ora_cg_pll_f32.restype = C.c_int
     L.ora_cg_pll_f64.argtypes = L.ora_cg_pll_f32.argtypes
     L.ora_cg_pll_f64.restype = C.c_int
+    L.ora_cg_pll_f64par.argtypes = L.ora_cg_pll_f32.argtypes
+    L.ora_cg_pll_f64par.restype = C.c_int
 
 
 def nsingle_padded(ncol: int) -> int:
@@ -147,6 +149,14 @@ def evaluate_f64_par(msa, w, x, lambda_single, lambda_pair):
     return float(f), g
 
 
+def forward_lse_f32(msa, x):
+    msa = _msa(msa)
+    out = np.zeros(msa.shape, dtype=np.float32)
+    lib().ora_forward_lse_f32(msa, msa.shape[1], msa.shape[0],
+                              np.ascontiguousarray(x, dtype=np.float32), out)
+    return out
+
+
 def forward_lse_f64(msa, x):
     msa = _msa(msa)
     out = np.zeros(msa.shape, dtype=np.float64)
@@ -207,7 +217,8 @@ def cg_pll_f32(msa, w, x0, lambda_single, lambda_pair, max_iterations=50, epsilo
     it = C.c_int()
     ne = C.c_int()
     hist = np.zeros(max_iterations + 1, dtype=np.float64)
-    fn = lib().ora_cg_pll_f32 if evaluator == "f32" else lib().ora_cg_pll_f64
+    fn = {"f32": lib().ora_cg_pll_f32, "f64": lib().ora_cg_pll_f64,
+          "f64par": lib().ora_cg_pll_f64par}[evaluator]
     ret = fn(msa, np.ascontiguousarray(w, dtype=np.float32), msa.shape[1],
                                msa.shape[0], float(lambda_single), float(lambda_pair), x,
                                C.byref(p), C.byref(fx), C.byref(it), C.byref(ne), hist)

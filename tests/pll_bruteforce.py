@@ -21,6 +21,18 @@ def unpack(x, L):
     return V, X2
 
 
+def lse(msa, x):
+    """log sum_a exp(logit(n, j, a)) per sequence and column, [n, j]."""
+    msa = np.asarray(msa)
+    N, L = msa.shape
+    V, X2 = unpack(np.asarray(x, dtype=np.float64), L)
+    onehot = np.eye(NA)[msa]
+    logits = np.einsum("nkb,bkaj->nja", onehot, X2[:, :, :NA, :])
+    logits[:, :, :20] += V.T[None, :, :]
+    m = logits.max(axis=2, keepdims=True)
+    return (np.log(np.exp(logits - m).sum(axis=2, keepdims=True)) + m)[:, :, 0]
+
+
 def pll(msa, w, x, lam1, lam2):
     """Returns (f, tied gradient in the padded layout)."""
     msa = np.asarray(msa)

@@ -1,0 +1,87 @@
+"""Device CG end to end against the FP64 oracle (north_star: after 50 CG
+iterations, APC maps with Spearman >= 0.999 and identical top-L/5 pairs).
+
+Each precision form runs the device CG (ccm_conjugrad_gpu_ex over
+evaluate_hip) for 50 iterations with epsilon 0 from init_bias, the
+reference's other CG parameters (ccmpred.c:526-534), and is compared with the
+FP64-evaluated oracle CG of the same algorithm on the same alignment
+(tests/golden/cg_maps/, written by tests/golden/make_cg_maps.py), under the
+criteria of tests/cg_fixtures.py.  Cases: L=75/N=1000 (seed 1), the bench
+alignment L=250/N=5000 (seed 2), alignment 40 of the batched workload
+(L=418, N=348) and L=600/N=20000 (seed 4).  Every form must run all 50
+iterations.  The final objective is compared with the FP64 run's (F_GAP).
+"""
+import time
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from ccmpred_amd import pll as P  # noqa: E402
+from ccmpred_amd.msa import batched_workload, synthetic_msa  # noqa: E402
+from tests import cg_fixtures as F  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+# final objective against the FP64 CG's, relative.  Single-pass bf16 rounds
+# every coupling to BF16 in the forward, so its f moves in steps near the
+# optimum and its line searches accept coarser steps: its 50-iteration
+# objective lags the FP64 run by up to 5.1e-4 (measured: alignment 40 of the
+# batched workload; 1e-5 at L=250, N=5000), while its contacts meet the
+# criteria.
+F_GAP = {"fp32": 1e-4, "bf16f": 1e-4, "bf16": 1e-3}
+
+CASES = {
+    "L75_N1000": lambda: synthetic_msa(75, 1000, 1, 15)[0],
+    "L250_N5000": lambda: synthetic_msa(250, 5000, 2, 60)[0],
+    "batch64_40": lambda: batched_workload(64)[40],
+    "L600_N20000": lambda: synthetic_msa(600, 20000, 4, 60)[0],
+}
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+_msas = {}
+
+
+def case_msa(name):
+    if name not in _msas:
+        _msas[name] = CASES[name]()
+    return _msas[name]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16f", "bf16"])
+@pytest.mark.parametrize("name", list(CASES))
+def test_cg_against_fp64(name, prec):
+    meta, m64 = F.load(name)
+    msa = case_msa(name)
+    assert F.msa_sha256(msa) == meta["msa_sha256"]
+    L = msa.shape[1]
+    say(f"cg-vs-fp64 {name} {prec} L={L} N={msa.shape[0]}")
+    with P.PLL(msa, precision=prec) as p:
+        assert p.lambda_pair == meta["lambda_pair"]
+        xd = torch.from_numpy(P.init_bias(msa)).cuda()
+        t0 = time.perf_counter()
+        r = p.conjugrad(xd, max_iterations=50, epsilon=0.0)
+        dt = time.perf_counter() - t0
+        m = P.contact_map(xd.cpu().numpy(), L)
+    c = F.compare(m, m64)
+    df = (r["fx"] - meta["f"]) / abs(meta["f"])
+    say(f"  ret={r['ret']} it={r['iterations']} evals={r['evaluations']} f={r['fx']:.10g} "
+        f"(FP64 CG {meta['f']:.10g}, rel {df:+.2e}) {dt:.1f} s")
+    say(f"  spearman={c['rho']:.6f} top-L/5 identical={c['identical']} swapped={c['swapped']} "
+        f"rel to FP64 cut-off={['%.2e' % v for v in c['swap_rel']]} "
+        f"FP64 pairs within 1% of cut-off={c['fp64_pairs_in_band']}")
+    assert r["iterations"] == 50
+    assert c["rho"] >= 0.999
+    assert c["identical"] or c["swaps_are_near_ties"]
+    assert abs(df) <= F_GAP[prec]

@@ -16,7 +16,7 @@ torch = pytest.importorskip("torch")
 
 from oracle import oracle as ora  # noqa: E402
 from ccmpred_amd import pll as P  # noqa: E402
-from ccmpred_amd.msa import synthetic_msa  # noqa: E402
+from ccmpred_amd.msa import read_psicov, synthetic_msa  # noqa: E402
 from tests.pll_bruteforce import nsingle_padded, NA, NAP  # noqa: E402
 
 pytestmark = pytest.mark.gpu
@@ -364,3 +364,33 @@ def test_cli_1atz(tmp_path):
     f1, f2 = final_fx(r.stdout), final_fx(r2.stdout)
     say(f"  final fx {f1} -> warm start {f2}")
     assert f2 <= f1 * (1 + 1e-4)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16f", "bf16"])
+def test_cli_1atz_against_fp64(tmp_path, prec):
+    """The reference's example run (test/test.sh:1-9, ccmpred example/1atzA.aln)
+    through the command-line driver with 50 iterations and no early stop
+    (-n 50 -e 0): its .mat (ccmpred.c:683-715: sum_submatrices, APC,
+    write_matrix) against the APC map of the FP64-oracle CG on the same
+    alignment (tests/golden/cg_maps/1atzA.npy), under the criteria of
+    tests/cg_fixtures.py."""
+    from tests import cg_fixtures as F
+    meta, m64 = F.load("1atzA")
+    aln = os.path.join(ROOT, "tests", "golden", "1atzA.aln")
+    assert F.msa_sha256(read_psicov(aln)) == meta["msa_sha256"]
+    exe = os.path.join(ROOT, "ccmpred_amd", "bin", "ccmpred")
+    out = tmp_path / f"1atzA_{prec}.mat"
+    say(f"cli 1atzA -n 50 -e 0 -P {prec}")
+    r = subprocess.run([exe, "-n", "50", "-e", "0", "-P", prec, aln, str(out)], capture_output=True,
+                       text=True, timeout=300)
+    sys.stdout.write(r.stdout[-800:])
+    assert r.returncode == 0, r.stderr
+    iters = [ln.split()[0] for ln in r.stdout.splitlines() if ln[:1].isdigit()]
+    assert iters and int(iters[-1]) == 50
+    lines = out.read_text().splitlines()
+    assert len(lines) == 75
+    mat = np.array([[float(v) for v in ln.split()] for ln in lines], dtype=np.float32)
+    c = F.compare(mat, m64)
+    say(f"  spearman={c['rho']:.6f} top-L/5 identical={c['identical']} swapped={c['swapped']}")
+    assert c["rho"] >= 0.999
+    assert c["identical"] or c["swaps_are_near_ties"]

@@ -9,6 +9,7 @@ import pytest
 
 from oracle import oracle as ora
 from ccmpred_amd.msa import synthetic_msa
+from tests import pll_bruteforce as bf
 from tests.pll_bruteforce import pll, random_symmetric_x, nsingle_padded, NA, NAP
 
 
@@ -196,3 +197,29 @@ def test_parallel_f64_matches_serial():
     f2, g2 = ora.evaluate_f64_par(msa, w, x, 0.01, 0.2 * (L - 1))
     assert abs(f1 - f2) <= 1e-12 * abs(f1)
     assert np.max(np.abs(g1 - g2)) <= 1e-12 * np.max(np.abs(g1))
+
+
+@pytest.mark.parametrize("L,N,seed", [(1, 3, 31), (3, 7, 32), (5, 12, 33)])
+def test_forward_lse_matches_bruteforce(L, N, seed):
+    """ora_forward_lse_f32/_f64 (the per-(sequence, column) log-partition of
+    evaluate_cpu.c:55-87) against the brute-force model's max-shifted
+    log-sum-exp, including non-symmetric couplings and the gap state."""
+    rng = np.random.default_rng(seed)
+    msa = rng.integers(0, 21, (N, L), dtype=np.uint8)
+    x = bf.random_symmetric_x(L, rng, scale=0.3, single_scale=0.7)
+    X2 = x[nsingle_padded(L):].reshape(NA, L, NAP, L)
+    X2[:, :, :NA, :] += rng.normal(0, 0.05, (NA, L, NA, L))  # not symmetric any more
+    ref = bf.lse(msa, x)
+    got64 = ora.forward_lse_f64(msa, x)
+    got32 = ora.forward_lse_f32(msa, x.astype(np.float32))
+    assert got64.shape == (N, L)
+    np.testing.assert_allclose(got64, ref, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(got32, ref, rtol=2e-6, atol=2e-6)
+
+
+def test_forward_lse_x_zero():
+    """x = 0: every log-partition is ln 21."""
+    msa = np.random.default_rng(5).integers(0, 21, (9, 6), dtype=np.uint8)
+    x = np.zeros(ora.nvar_padded(6))
+    np.testing.assert_allclose(ora.forward_lse_f64(msa, x), np.log(21.0), rtol=1e-15)
+    np.testing.assert_allclose(ora.forward_lse_f32(msa, x.astype(np.float32)), np.log(21.0), rtol=1e-6)
