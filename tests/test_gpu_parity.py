@@ -281,9 +281,9 @@ def _spearman(a, b):
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16f", "bf16"])
 def test_cg_50_iterations_contacts(prec):
-    """Device CG vs the same CG on the host driven by the FP64 oracle.  With
-    single-pass BF16 logits f is quantised by the BF16 couplings, so the line
-    search may stop before 50 iterations; the contact criteria still apply."""
+    """Device CG vs the same CG on the host driven by the FP64 oracle: all 50
+    iterations in every form (single-pass BF16 quantises f near the optimum;
+    the approximate decrease test on the slope keeps its line search going)."""
     L, N = 75, 1000
     say(f"cg50 {prec} L={L} N={N}")
     msa, pairs = synthetic_msa(L, N, 1, 15)
@@ -304,10 +304,7 @@ def test_cg_50_iterations_contacts(prec):
         f"dev ret={dev['ret']} it={dev['iterations']} evals={dev['evaluations']} f={dev['fx']:.6f}")
     say(f"  spearman={rho:.6f} top-L/5 equal={top_dev == top_host} planted recovered={len(top_dev & planted)}")
     assert host["iterations"] == 50
-    if prec == "bf16":
-        assert dev["iterations"] >= 20
-    else:
-        assert dev["iterations"] == 50
+    assert dev["iterations"] == 50
     assert abs(dev["fx"] - host["fx"]) / host["fx"] < 5e-3
     assert rho >= 0.999
     assert top_dev == top_host
