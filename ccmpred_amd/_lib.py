@@ -70,6 +70,7 @@ EXPORTED = [
     "ccm_launch_pair_epilogue", "ccm_launch_hsym", "ccm_launch_singles_finalize",
     "ccm_launch_sum3", "ccm_dots_partial_len", "ccm_launch_dots", "ccm_launch_axpy",
     "ccm_launch_direction", "ccm_launch_negate", "ccm_contact_map_work_len", "ccm_launch_contact_map",
+    "ccm_contact_map_noalloc_max_l", "ccm_launch_contact_map_noalloc",
     "ccm_batch_create", "ccm_batch_destroy", "ccm_batch_size", "ccm_batch_x", "ccm_batch_g",
     "ccm_batch_write_x", "ccm_batch_read", "ccm_batch_contact_map", "ccm_batch_stream", "ccm_batch_device_bytes",
     "ccm_batch_set_variants", "ccm_batch_get_variants", "ccm_batch_autotune", "ccm_batch_evaluate",
@@ -172,6 +173,10 @@ def _declare(L):
     L.ccm_contact_map_work_len.restype = ip
     L.ccm_launch_contact_map.argtypes = [vp, ip, ip, ip, vp, vp, vp]
     L.ccm_launch_contact_map.restype = ip
+    L.ccm_contact_map_noalloc_max_l.argtypes = []
+    L.ccm_contact_map_noalloc_max_l.restype = ip
+    L.ccm_launch_contact_map_noalloc.argtypes = [vp, ip, ip, ip, vp, vp]
+    L.ccm_launch_contact_map_noalloc.restype = ip
     L.ccm_batch_stream.argtypes = [vp]
     L.ccm_batch_stream.restype = vp
     L.ccm_batch_device_bytes.argtypes = [vp]

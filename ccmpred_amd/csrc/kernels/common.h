@@ -11,6 +11,7 @@
 #define CCM_OK 0
 #define CCM_ERR_ARG -2
 #define CCM_ERR_LAUNCH -3
+#define CCM_ERR_SIZE -5  // beyond the fixed capacity of an allocation-free entry point
 
 #define CCM_LAUNCH_CHECK()                                  \
   do {                                                      \

@@ -8,6 +8,9 @@
 // (minimum and maximum) are reduced in parallel.
 #include "common.h"
 
+#include <mutex>
+#include <stdio.h>
+
 #include "ccmpred_amd_kernels.h"
 
 namespace {
@@ -144,4 +147,58 @@ extern "C" int ccm_launch_contact_map(const float* d_x, int L, int apc, int norm
     CCM_LAUNCH_CHECK();
   }
   return CCM_OK;
+}
+
+// ---- allocation-free form --------------------------------------------------
+// The APC's column-mean scratch (ccm_contact_map_work_len(L) = L floats) in a
+// fixed device array instead of a caller's or the library's allocation.  Each
+// GPU holds its own copy of the array until the process ends; its address is
+// looked up once per device (with that device current) and kept.  Every call
+// on one GPU shares that GPU's array: the kernels write each element before
+// they read it, so nothing needs clearing, but calls that overlap on one GPU
+// (two streams, two threads) race on it.
+namespace {
+constexpr int NOALLOC_MAX_L = CCM_CONTACT_MAP_NOALLOC_MAX_L;
+constexpr int NOALLOC_MAX_DEV = 64;
+__device__ __align__(256) float scratch[NOALLOC_MAX_L];
+
+std::mutex scratch_mu;
+float* scratch_ptr[NOALLOC_MAX_DEV];  // device address of `scratch`, per device
+
+// device address of `scratch` on the current device (nullptr on error)
+float* scratch_on_current_device(int dev) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEV) return nullptr;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  return scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" int ccm_contact_map_noalloc_max_l(void) { return NOALLOC_MAX_L; }
+
+extern "C" int ccm_launch_contact_map_noalloc(const float* d_x, int L, int apc, int normalize,
+                                              float* d_out, ccm_stream_t s) {
+  if (L > NOALLOC_MAX_L) {
+    fprintf(stderr,
+            "ccm_launch_contact_map_noalloc: L = %d exceeds the fixed scratch capacity of %d "
+            "columns (use ccm_launch_contact_map with a work buffer)\n",
+            L, NOALLOC_MAX_L);
+    return CCM_ERR_SIZE;
+  }
+  if (!d_x || !d_out || L < 1) return CCM_ERR_ARG;
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return CCM_ERR_LAUNCH;
+  if (s) {
+    // the scratch address is per device: the stream must be the current device's
+    hipDevice_t sdev = -1;
+    if (hipStreamGetDevice((hipStream_t)s, &sdev) != hipSuccess || (int)sdev != dev)
+      return CCM_ERR_ARG;
+  }
+  float* work = scratch_on_current_device(dev);
+  if (!work) return CCM_ERR_LAUNCH;
+  return ccm_launch_contact_map(d_x, L, apc, normalize, d_out, work, s);
 }

@@ -142,6 +142,19 @@ int ccm_contact_map_work_len(int L);
 int ccm_launch_contact_map(const float *d_x, int L, int apc, int normalize, float *d_out,
                            float *d_work, ccm_stream_t s);
 
+/* ccm_launch_contact_map without the work buffer, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops).  No
+ * counterpart in the reference.  The L-float scratch is a fixed device array
+ * of CCM_CONTACT_MAP_NOALLOC_MAX_L floats (32 KiB), one per GPU, held until
+ * the process ends; L beyond it returns -5 with a message on stderr.  The
+ * stream must belong to the current device.  Calls that overlap on one GPU
+ * share that array and race; sequential calls on any streams are safe.
+ * Allocates nothing: no hipMalloc, hipFree or allocating library call. */
+#define CCM_CONTACT_MAP_NOALLOC_MAX_L 8192
+int ccm_contact_map_noalloc_max_l(void);
+int ccm_launch_contact_map_noalloc(const float *d_x, int L, int apc, int normalize, float *d_out,
+                                   ccm_stream_t s);
+
 /* Grouped forms for a batch of alignments (descriptors in
  * ccmpred_amd_batch.h).  No counterpart in the reference, which runs one
  * alignment per process.  Each step is one launch over every alignment of
