@@ -6,7 +6,8 @@ device-resident x and g (f read back to the host), for the seeded
 synthetic MSA with L=250 columns, N=5000 sequences and 60 planted contacts
 (seed 2), reweighting threshold 0.8, lambda_single 0.01,
 lambda_pair 0.2*(L-1), and x = init_bias plus symmetric N(0, 0.01^2)
-couplings.  `value` is the BF16-MFMA path; the plain FP32 path, the
+couplings.  `value` is the single-pass BF16-MFMA path (bf16f, the
+two-pass forward, is timed beside it); the plain FP32 path, the
 OpenMP CPU evaluator (the oracle's restatement of evaluate_cpu_omp.c) and
 the single-thread one (of evaluate_cpu.c) are timed in the same run on the
 same MSA and x.
@@ -48,6 +49,25 @@ def measured_counters(kernel_prefix):
     return None, None
 
 
+LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clock x 2.4 GHz = 78.6 TB/s
+
+
+def fp32_roofline(step_ms, L, N):
+    """The plain FP32 path's gather kernels against their LDS bound: each of
+    the N*L*21L coupling additions of a contraction reads one FP32 value
+    from LDS (4 bytes), 26.25 GB at L=250, N=5000."""
+    if not step_ms:
+        return None
+    lds_bytes = 4.0 * N * L * 21 * L
+    out = {"bound": "lds", "peak": round(LDS_PEAK_GBS, 1), "unit": "GB/s", "algorithmic_bytes": lds_bytes}
+    for k in ("gather_fwd", "gather_grad"):
+        if k in step_ms:
+            ach = lds_bytes / (step_ms[k] * 1e-3) / 1e9
+            out[k] = {"avg_ms": round(step_ms[k], 4), "achieved": round(ach, 1),
+                      "frac": round(ach / LDS_PEAK_GBS, 4)}
+    return out
+
+
 def make_x(msa):
     from ccmpred_amd.pll import init_bias, nsingle_padded
     L = msa.shape[1]
@@ -62,15 +82,22 @@ def make_x(msa):
     return x.astype(np.float32)
 
 
-def time_gpu(msa, x_host, precision, steps, warmup):
+def time_gpu(msa, x_host, precision, steps, warmup, warmup_seconds):
     import torch
     from ccmpred_amd.pll import PLL
     with PLL(msa, precision=precision) as p:
         xd = torch.from_numpy(x_host).cuda()
         gd = torch.empty_like(xd)
         torch.cuda.synchronize()
-        for _ in range(warmup):
+        # W warm-up steps, continued until warmup_seconds have passed: the
+        # GPU raises its clocks over the first tens of milliseconds of load
+        # (one evaluation took 725 us at the start of a run and 630 us 25
+        # evaluations later), and the timed steps measure the steady state
+        t_w = time.perf_counter()
+        done = 0
+        while done < warmup or time.perf_counter() - t_w < warmup_seconds:
             p.evaluate_ptr(xd.data_ptr(), gd.data_ptr())
+            done += 1
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         f = 0.0
@@ -86,13 +113,28 @@ def time_gpu(msa, x_host, precision, steps, warmup):
         ncalls, steps_ms = p.timing()
         p.set_timing(False)
     return dict(evals_per_s=steps / (t1 - t0), ms_per_eval=1e3 * (t1 - t0) / steps, f=f,
-                timed_calls=ncalls, step_ms=steps_ms)
+                timed_calls=ncalls, step_ms=steps_ms, warmup_steps=done)
+
+
+def physical_cores():
+    """Physical cores among the CPUs this process may run on (one per
+    thread_siblings group), and the number of those CPUs."""
+    cpus = sorted(os.sched_getaffinity(0))
+    groups = set()
+    for c in cpus:
+        try:
+            with open(f"/sys/devices/system/cpu/cpu{c}/topology/thread_siblings_list") as fh:
+                groups.add(fh.read().strip())
+        except OSError:
+            groups.add(str(c))
+    return len(groups), len(cpus)
 
 
 def time_cpu(msa, x_host, evals):
     from oracle import oracle as ora
     w = ora.calculate_weights(msa, 0.8)
     L = msa.shape[1]
+    # threads: OMP_NUM_THREADS, which cpu_legs sets to the physical cores
     ev = ora.OmpEvaluator(msa, w, 0.01, np.float32(0.2 * (L - 1)))
     g = np.zeros_like(x_host)
     ev(x_host, g)  # warm-up (page faults of the per-call buffers)
@@ -118,6 +160,33 @@ def time_cpu_serial(msa, x_host, evals):
         ora.evaluate_f32(msa, w, x_host, 0.01, np.float32(0.2 * (L - 1)))
         ts.append(time.perf_counter() - t0)
     return ts
+
+
+def run_cpu_child(args):
+    """The CPU legs, in a process of their own (see cpu_legs): prints one
+    JSON object."""
+    from ccmpred_amd.msa import synthetic_msa
+    msa, _ = synthetic_msa(L_COLS, N_SEQS, SEED, N_PAIRS)
+    x = make_x(msa)
+    cpu = time_cpu(msa, x, args.cpu_evals)
+    serial = time_cpu_serial(msa, x, args.cpu_serial_evals) if args.cpu_serial_evals > 0 else None
+    print(json.dumps({"cpu": cpu, "serial": serial}), flush=True)
+
+
+def cpu_legs(args):
+    """Times the OpenMP and single-thread CPU evaluators in a child process
+    started with one OpenMP thread per physical core, bound close
+    (OMP_PROC_BIND=close, OMP_PLACES=cores).  A child, because an OpenMP
+    runtime that reads those variables at start-up binds the process's
+    first thread to one place, and this process has loaded torch's."""
+    import subprocess
+    cores, _ = HOST_CPUS
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-child", "--cpu-evals", str(args.cpu_evals),
+           "--cpu-serial-evals", str(args.cpu_serial_evals)]
+    out = subprocess.run(cmd, env=env, capture_output=True, text=True, check=True).stdout
+    res = json.loads(out.strip().splitlines()[-1])
+    return res["cpu"], res["serial"]
 
 
 def cpu_model():
@@ -217,15 +286,25 @@ def run_batched(args):
     print(json.dumps(line), flush=True)
 
 
+HOST_CPUS = (0, 0)
+
+
 def main():
+    global HOST_CPUS
+    # counted before anything loads an OpenMP runtime that might bind this
+    # thread to fewer CPUs
+    HOST_CPUS = physical_cores()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--warmup-seconds", type=float, default=1.0,
+                    help="continue the warm-up steps until this much time has passed")
     ap.add_argument("--cpu-evals", type=int, default=3)
     ap.add_argument("--cpu-serial-evals", type=int, default=1,
                     help="single-thread evaluate_cpu restatement calls to time (0: none)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a device copy of this many MiB (counter calibration)")
     ap.add_argument("--batched", action="store_true",
@@ -239,6 +318,9 @@ def main():
     ap.add_argument("--sequential", action="store_true",
                     help="with --batched: also time the same solves one MSA at a time")
     args = ap.parse_args()
+    if args.cpu_child:
+        run_cpu_child(args)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path "
               "(the evaluate callback works on one device); run with --gpus 1", file=sys.stderr)
@@ -261,10 +343,11 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    bf16 = time_gpu(msa, x, "bf16", args.steps, args.warmup)
-    fp32 = time_gpu(msa, x, "fp32", max(10, args.steps // 4), max(2, args.warmup // 2))
-    cpu = None if args.no_cpu else time_cpu(msa, x, args.cpu_evals)
-    serial = None if args.no_cpu or args.cpu_serial_evals <= 0 else time_cpu_serial(msa, x, args.cpu_serial_evals)
+    bf16 = time_gpu(msa, x, "bf16", args.steps, args.warmup, args.warmup_seconds)
+    bf16f = time_gpu(msa, x, "bf16f", args.steps, args.warmup, args.warmup_seconds)
+    fp32 = time_gpu(msa, x, "fp32", max(10, args.steps // 4), max(2, args.warmup // 2),
+                    args.warmup_seconds)
+    cpu, serial = (None, None) if args.no_cpu else cpu_legs(args)
 
     L, N = L_COLS, N_SEQS
     flops = 2.0 * N * (21 * L) ** 2          # algorithmic FLOPs of one contraction
@@ -300,8 +383,12 @@ def main():
                      "algorithmic_flops": flops, "avg_ms": round(gemm_ms, 4),
                      "mfma_busy": mfma_busy,
                      "counters": os.path.relpath(COUNTERS_FILE, ROOT) if traffic else None},
+        "warmup_steps_run": bf16["warmup_steps"],
+        "bf16f_evals_per_s": round(bf16f["evals_per_s"], 2),
+        "bf16f_step_ms": {k: round(v, 4) for k, v in bf16f["step_ms"].items()},
         "fp32_evals_per_s": round(fp32["evals_per_s"], 2),
         "bf16_over_fp32": round(bf16["evals_per_s"] / fp32["evals_per_s"], 3),
+        "fp32_roofline": fp32_roofline(fp32["step_ms"], L, N),
         "bf16_step_ms": {k: round(v, 4) for k, v in bf16["step_ms"].items()},
         "fp32_step_ms": {k: round(v, 4) for k, v in fp32["step_ms"].items()},
         "f_bf16": bf16["f"], "f_fp32": fp32["f"],
@@ -310,10 +397,11 @@ def main():
         line["cpu_baseline"] = {"value": round(cpu["evals_per_s"], 4), "unit": "evals/s",
                                 "cores": cpu["threads"], "kind": "port",
                                 "sample": f"{len(cpu['seconds'])} evaluations of the full L=250,N=5000 "
-                                          "workload (median), OpenMP restatement of evaluate_cpu_omp.c"}
+                                          "workload (median), OpenMP restatement of evaluate_cpu_omp.c, "
+                                          "one thread per physical core, OMP_PROC_BIND=close"}
         line["bf16_over_cpu"] = round(bf16["evals_per_s"] / cpu["evals_per_s"], 1)
         line["cpu_model"] = cpu_model()
-        line["cpu_threads_available"] = os.cpu_count()
+        line["cpu_physical_cores"], line["cpu_logical_available"] = HOST_CPUS
     if serial is not None:
         line["cpu_single_thread"] = {"value": round(1.0 / float(np.median(serial)), 4), "unit": "evals/s",
                                      "cores": 1, "kind": "port",
