@@ -37,7 +37,7 @@ class UserData(C.Structure):
 class HipConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("bf16_passes", C.c_int),
                 ("bf16_fwd_passes", C.c_int), ("verbose", C.c_int), ("chunk_rows", C.c_int),
-                ("gemm_variant", C.c_int)]
+                ("gemm_variant", C.c_int), ("sparse_fwd", C.c_int)]
 
 
 class CGParam(C.Structure):
@@ -55,7 +55,7 @@ EXPORTED = [
     "ccm_nsingle_padded", "ccm_nvar_padded", "ccm_hip_default_config", "init_hip", "init_hip_ex",
     "init_hip_bf16", "evaluate_hip", "destroy_hip", "ccm_hip_evaluate", "ccm_hip_stream",
     "ccm_hip_set_timing", "ccm_hip_get_timing", "ccm_hip_step_names", "ccm_hip_objective", "ccm_hip_forward",
-    "ccm_hip_backward", "ccm_hip_device_bytes", "ccm_reg_only_blocks", "ccm_launch_reg_only",
+    "ccm_hip_backward", "ccm_hip_device_bytes", "ccm_hip_objective_resolution", "ccm_reg_only_blocks", "ccm_launch_reg_only",
     "ccm_conjugrad_init", "ccm_conjugrad_gpu", "ccm_conjugrad_gpu_ex", "ccm_read_msa",
     "ccm_init_bias", "ccm_sum_submatrices", "ccm_apc", "ccm_normalize", "ccm_write_matrix",
     "ccm_write_raw", "ccm_read_raw", "ccm_write_raw_msgpack",
@@ -71,6 +71,8 @@ EXPORTED = [
     "ccm_launch_sum3", "ccm_dots_partial_len", "ccm_launch_dots", "ccm_launch_axpy",
     "ccm_launch_direction", "ccm_launch_negate", "ccm_contact_map_work_len", "ccm_launch_contact_map",
     "ccm_contact_map_noalloc_max_l", "ccm_launch_contact_map_noalloc",
+    "ccm_sparse_ws_len", "ccm_sparse_steps", "ccm_sparse_index_ld", "ccm_sparse_fwd_fits",
+    "ccm_launch_convert_ws", "ccm_launch_sparse_index", "ccm_launch_sparse_fwd_softmax",
     "ccm_batch_create", "ccm_batch_destroy", "ccm_batch_size", "ccm_batch_x", "ccm_batch_g",
     "ccm_batch_write_x", "ccm_batch_read", "ccm_batch_contact_map", "ccm_batch_stream", "ccm_batch_device_bytes",
     "ccm_batch_set_variants", "ccm_batch_get_variants", "ccm_batch_autotune", "ccm_batch_evaluate",
@@ -125,6 +127,8 @@ def _declare(L):
     L.destroy_hip.restype = ip
     L.ccm_hip_evaluate.argtypes = [vp, vp, vp, ip, C.POINTER(C.c_double)]
     L.ccm_hip_evaluate.restype = ip
+    L.ccm_hip_objective_resolution.argtypes = [vp]
+    L.ccm_hip_objective_resolution.restype = C.c_double
     L.ccm_hip_stream.argtypes = [vp]
     L.ccm_hip_stream.restype = vp
     L.ccm_hip_objective.argtypes = [vp, vp, ip, C.POINTER(C.c_double)]

@@ -523,7 +523,7 @@ int ccm_batch_create(userdata *uds, int nprob, const ccm_hip_config *cfg_in, ccm
 	HIPCHK(balloc(b, (void **)&b->d_ctl, sizeof(ccm_cg_ctl) * (size_t)nprob));
 	HIPCHK(hipMemset(b->d_ctl, 0, sizeof(ccm_cg_ctl) * (size_t)nprob));
 	for (int i = 0; i < nprob; i++) {
-		if (ccm_init_hip_state(&uds[i], &cfg, b->fused) != 0) {
+		if (ccm_init_hip_state(&uds[i], &cfg, b->fused, 0) != 0) {
 			fprintf(stderr, "ccmpred_amd batch: alignment %d failed to initialise\n", i);
 			goto fail;
 		}
@@ -825,6 +825,7 @@ int ccm_batch_solve(ccm_batch *b, const ccm_conjugrad_parameter *param, int roun
 	prm.alpha_mul = param->alpha_mul;
 	prm.ftol = param->ftol;
 	prm.wolfe = param->wolfe;
+	prm.fres = ccm_hip_objective_resolution(&b->uds[0]);
 
 	/* per-iteration logs */
 	if (b->log_cap < param->max_iterations) {

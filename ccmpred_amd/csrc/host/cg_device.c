@@ -13,8 +13,9 @@
  *       [0.1 alpha, alpha_mul alpha] (or bisect [lo, hi] once a lower
  *       bound exists).  Sufficient decrease is the Armijo test
  *       f_t <= f0 + ftol alpha <g0,s>, or, below the objective's resolution
- *       (CG_FRES |f0|), the approximate Armijo test of Hager and Zhang:
- *       f_t <= f0 + CG_FRES |f0| and <g_t,s> <= (2 ftol - 1) <g0,s>
+ *       (fres |f0|), the approximate Armijo test of Hager and Zhang:
+ *       f_t <= f0 + fres |f0| and <g_t,s> <= (2 ftol - 1) <g0,s>; fres is
+ *       ccm_hip_objective_resolution of evaluate_hip's instance, else 1e-6
  *     strong Wolfe (|<g_t,s>| <= wolfe |<g0,s>|), or sufficient decrease on
  *       the last trial: accept
  *     otherwise <g_t,s> > 0 sets hi, < 0 sets lo; extend (alpha /= alpha_mul)
@@ -52,16 +53,17 @@ void ccm_conjugrad_init(ccm_conjugrad_parameter *p)
 	p->wolfe = 0.2;
 }
 
-/* Sufficient decrease (see above).  CG_FRES is the relative resolution
- * below which differences of f are not trusted: the BF16 forms quantise
- * the couplings, so their f moves in steps near the optimum while the
+/* Sufficient decrease (see above).  fres is the relative resolution below
+ * which differences of f are not trusted: the BF16 forms quantise the
+ * couplings, so their f moves in steps near the optimum while the
  * gradient, and with it the slope <g_t,s>, still resolves the search. */
-#define CG_FRES 1e-6
-static int cg_decrease(double ft, double f0, double alpha, double dg0, double dgt, double ftol)
+#define CG_FRES_DEFAULT 1e-6
+static int cg_decrease(double ft, double f0, double alpha, double dg0, double dgt, double ftol,
+                       double fres)
 {
 	if (ft <= f0 + ftol * alpha * dg0)
 		return 1;
-	return ft <= f0 + CG_FRES * fabs(f0) && dgt <= (2.0 * ftol - 1.0) * dg0;
+	return ft <= f0 + fres * fabs(f0) && dgt <= (2.0 * ftol - 1.0) * dg0;
 }
 
 /* Step after a trial failed the decrease test with no lower bracket: the
@@ -154,6 +156,8 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 	int ret = CONJUGRADERR_DEVICE, it = 0, nevals = 0;
 	double f = NAN;
 	c.n = n;
+	const double fres =
+	    evaluate == evaluate_hip && instance ? ccm_hip_objective_resolution(instance) : CG_FRES_DEFAULT;
 	if (hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking) != hipSuccess)
 		return CONJUGRADERR_DEVICE;
 	if (hipMalloc((void **)&g, sizeof(float) * (size_t)n) != hipSuccess ||
@@ -224,7 +228,7 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
 				ret = CONJUGRADERR_DEVICE;
 				goto done;
 			}
-			if (!cg_decrease(ft, f0, alpha, dg0, dgt, p->ftol)) {
+			if (!cg_decrease(ft, f0, alpha, dg0, dgt, p->ftol, fres)) {
 				/* not enough decrease: shrink, bisecting once bracketed */
 				hi = alpha;
 				alpha = lo > 0.0 ? 0.5 * (lo + hi) : cg_shrink(ft, f0, alpha, dg0, p->alpha_mul);

@@ -76,6 +76,7 @@ void ccm_hip_default_config(ccm_hip_config *cfg)
 	cfg->verbose = 1;
 	cfg->chunk_rows = 0;
 	cfg->gemm_variant = -1;
+	cfg->sparse_fwd = 1;
 }
 
 static hipError_t dalloc(ccm_hip_state *st, void **p, size_t bytes)
@@ -91,7 +92,7 @@ static void free_state(ccm_hip_state *st)
 	if (!st)
 		return;
 	void *bufs[] = {st->d_msap, st->d_msapt, st->d_segmeta, st->d_w, st->d_zd, st->d_graw,
-	                st->d_wt,   st->d_zt,    st->d_dt, st->d_gt,     st->d_hsym,
+	                st->d_wt,   st->d_sidx,  st->d_zt, st->d_dt,     st->d_gt, st->d_hsym,
 	                st->d_fpart, st->d_g1part, st->d_regpair, st->d_regsingle, st->d_f};
 	for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); i++)
 		if (bufs[i])
@@ -148,10 +149,10 @@ static const unsigned char *seg_chunk(const ccm_hip_state *st, int n0)
 
 int init_hip_ex(void *instance, const ccm_hip_config *cfg_in)
 {
-	return ccm_init_hip_state(instance, cfg_in, 1);
+	return ccm_init_hip_state(instance, cfg_in, 1, 1);
 }
 
-int ccm_init_hip_state(void *instance, const ccm_hip_config *cfg_in, int allow_fused)
+int ccm_init_hip_state(void *instance, const ccm_hip_config *cfg_in, int allow_fused, int allow_sparse)
 {
 	userdata *ud = (userdata *)instance;
 	ccm_hip_config cfg;
@@ -289,15 +290,25 @@ int ccm_init_hip_state(void *instance, const ccm_hip_config *cfg_in, int allow_f
 		st->n_fpart = ccm_softmax_rows_blocks(L, N);
 		st->n_g1blk = ccm_softmax_rows_nblk(N);
 	} else {
-		const int passes = cfg.bf16_passes;
-		st->ldwt = (long)st->cfg.bf16_fwd_passes * P;
+		const int passes = cfg.bf16_passes, fpasses = cfg.bf16_fwd_passes;
 		st->lddt = (long)passes * st->ccols;
 		/* the fused forward addresses the gathered rows of Wt with 32-bit
-		 * offsets */
-		st->fused = allow_fused && ccm_fwd_softmax_fits(Lp, st->cfg.bf16_fwd_passes, passes, st->ccols);
+		 * offsets; the sparse one exists in the fused form */
+		st->fused = allow_fused && ccm_fwd_softmax_fits(Lp, fpasses, passes, st->ccols);
+		st->sparse = st->fused && allow_sparse && cfg.sparse_fwd &&
+		             ccm_sparse_fwd_fits(Lp, fpasses, passes, st->ccols) &&
+		             (long)ccm_sparse_steps(Lp, fpasses) * 2 * ccm_sparse_index_ld(Np) * 4 <= 0x7FFFFFFFL;
+		st->ldwt = st->sparse ? ccm_sparse_ws_len(Lp, fpasses) : (long)fpasses * P;
 		HIPCHK(dalloc(st, (void **)&st->d_wt, sizeof(uint16_t) * (size_t)P * st->ldwt));
-		/* the rows of padding columns j >= L stay zero: convert_wt skips them */
+		/* the rows of padding columns j >= L stay zero: the conversions skip them */
 		HIPCHK(hipMemsetAsync(st->d_wt, 0, sizeof(uint16_t) * (size_t)P * st->ldwt, s));
+		if (st->sparse) {
+			/* the one-hot positions of the sparse forward: constant per MSA */
+			st->ldsidx = ccm_sparse_index_ld(Np);
+			HIPCHK(dalloc(st, (void **)&st->d_sidx,
+			              sizeof(uint32_t) * 2 * (size_t)ccm_sparse_steps(Lp, fpasses) * st->ldsidx));
+			KCHK(ccm_launch_sparse_index(st->d_msap, Lp, Np, fpasses, st->d_sidx, s));
+		}
 		if (!st->fused)
 			HIPCHK(dalloc(st, (void **)&st->d_zt, sizeof(float) * (size_t)P * st->ccols));
 		HIPCHK(dalloc(st, (void **)&st->d_dt, sizeof(uint16_t) * (size_t)P * st->lddt));
@@ -406,6 +417,15 @@ void *ccm_hip_stream(void *instance)
 	return st ? (void *)st->stream : NULL;
 }
 
+double ccm_hip_objective_resolution(void *instance)
+{
+	const userdata *ud = (const userdata *)instance;
+	const ccm_hip_state *st = ud ? (const ccm_hip_state *)ud->extra : NULL;
+	if (st && st->cfg.precision == CCM_PRECISION_BF16 && st->cfg.bf16_fwd_passes == 1)
+		return 1e-5;
+	return 1e-6;
+}
+
 size_t ccm_hip_device_bytes(void *instance)
 {
 	userdata *ud = (userdata *)instance;
@@ -431,6 +451,15 @@ static ccm_hip_state *get_state(void *instance, int nvar)
 		return NULL;
 	}
 	return st;
+}
+
+/* The forward's coupling operand from x: Wt (ccm_launch_convert_wt), or the
+ * sparse layout Ws (ccm_launch_convert_ws) */
+static int convert_couplings(ccm_hip_state *st, const float *d_x, hipStream_t s)
+{
+	return st->sparse
+	           ? ccm_launch_convert_ws(d_x, st->L, st->Lp, st->d_wt, st->ldwt, st->cfg.bf16_fwd_passes, s)
+	           : ccm_launch_convert_wt(d_x, st->L, st->Lp, st->d_wt, st->ldwt, st->cfg.bf16_fwd_passes, s);
 }
 
 /* Logits, softmax and (RUN_FULL) pair-gradient accumulation over all
@@ -463,7 +492,12 @@ static int run_chunks(ccm_hip_state *st, const float *d_x, int what, int *step)
 			const int ncols = (int)round_up(nc, 128);
 			if (first)
 				MARK((*step)++);
-			if (st->fused) {
+			if (st->sparse) {
+				KCHK(ccm_launch_sparse_fwd_softmax(
+				    st->d_wt, st->ldwt, Lp, st->cfg.bf16_fwd_passes, st->d_sidx + n0, st->ldsidx, ncols,
+				    d_x, st->d_msapt + n0, (long)Np, st->d_w + n0, L, nc, st->d_dt, (long)passes * ncols,
+				    (long)ncols, passes, mode, n0 / 128, st->d_fpart, st->d_g1part, s));
+			} else if (st->fused) {
 				KCHK(ccm_launch_gemm_fwd_softmax(st->d_wt, st->ldwt, Lp, st->cfg.bf16_fwd_passes,
 				                                 msap, ncols, d_x, st->d_msapt + n0, (long)Np,
 				                                 st->d_w + n0, L, nc, st->d_dt,
@@ -549,8 +583,7 @@ int ccm_hip_evaluate(void *instance, const float *d_x, float *d_g, int nvar, dou
 	st->forward_valid = 0;
 	if (st->cfg.precision != CCM_PRECISION_FP32) {
 		MARK(step++);
-		KCHK(ccm_launch_convert_wt(d_x, st->L, st->Lp, st->d_wt, st->ldwt, st->cfg.bf16_fwd_passes,
-		                           s));
+		KCHK(convert_couplings(st, d_x, s));
 	}
 	if (run_chunks(st, d_x, RUN_FULL, &step))
 		goto fail;
@@ -585,8 +618,7 @@ int ccm_hip_objective(void *instance, const float *d_x, int nvar, double *f)
 	st->timing = 0;
 	st->forward_valid = 0;
 	if (st->cfg.precision != CCM_PRECISION_FP32)
-		KCHK(ccm_launch_convert_wt(d_x, st->L, st->Lp, st->d_wt, st->ldwt, st->cfg.bf16_fwd_passes,
-		                           s));
+		KCHK(convert_couplings(st, d_x, s));
 	if (run_chunks(st, d_x, RUN_OBJECTIVE, &step))
 		goto fail;
 	KCHK(ccm_launch_reg_only(d_x, st->L, st->lam1, st->lam2, st->d_regpair, s));
@@ -613,8 +645,7 @@ int ccm_hip_forward(void *instance, const float *d_x, int nvar, double *f)
 	st->timing = 0;
 	st->forward_valid = 0;
 	if (st->cfg.precision != CCM_PRECISION_FP32)
-		KCHK(ccm_launch_convert_wt(d_x, st->L, st->Lp, st->d_wt, st->ldwt, st->cfg.bf16_fwd_passes,
-		                           s));
+		KCHK(convert_couplings(st, d_x, s));
 	if (run_chunks(st, d_x, RUN_FORWARD, &step))
 		goto fail;
 	KCHK(ccm_launch_reg_only(d_x, st->L, st->lam1, st->lam2, st->d_regpair, s));

@@ -34,8 +34,11 @@ typedef struct ccm_hip_state {
 	float *d_graw;
 	long ldg;
 	/* BF16 path */
-	uint16_t *d_wt;
+	uint16_t *d_wt; /* the forward's coupling operand: Wt, or Ws when sparse */
 	long ldwt;
+	int sparse;        /* forward on structured-sparse MFMA (fused only) */
+	uint32_t *d_sidx;  /* its one-hot positions, [step][half][ldsidx sequences] */
+	long ldsidx;
 	int fused;      /* forward contraction with the softmax as its epilogue (no d_zt) */
 	float *d_zt;    /* P x ccols, unfused only */
 	uint16_t *d_dt; /* P x passes*ccols */
@@ -62,8 +65,9 @@ typedef struct ccm_hip_state {
 	int timed_calls;
 } ccm_hip_state;
 
-/* init_hip_ex with the choice of the fused forward (the batched form runs
- * the grouped unfused kernels on the instance's buffers) */
-int ccm_init_hip_state(void *instance, const ccm_hip_config *cfg, int allow_fused);
+/* init_hip_ex with the choice of the fused and of the sparse forward (the
+ * batched form runs its grouped kernels, which take the dense operand, on
+ * the instance's buffers) */
+int ccm_init_hip_state(void *instance, const ccm_hip_config *cfg, int allow_fused, int allow_sparse);
 
 #endif

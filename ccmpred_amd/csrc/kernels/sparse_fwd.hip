@@ -1,0 +1,394 @@
+// The forward contraction on gfx950's 2:4 structured-sparse MFMA, with the
+// column softmax as its epilogue (evaluate_cpu.c:55-109; cu:30-141).
+//
+// Z[n][c] = sum_r onehot[n][r] * W[r][c] with the one-hot operand as the
+// sparse A operand of v_smfmac_f32_32x32x32_bf16 (rows = sequences n) and
+// the couplings as the dense B operand (columns c = a*Lp + j, the rows of
+// the coupling matrix in residue-major order).  The K dimension is laid out
+// in groups of four slots: slot 0 is an explicit zero, slots 1..3 hold
+// residues 3t, 3t+1, 3t+2 of one coupling column k, so the 21 residues of a
+// column take 7 groups (28 slots).  Per group the sparse operand stores two
+// values with 2-bit positions; they are always (0.0 at position 0,
+// 1.0 at position p): p = x_nk - 3t + 1 when sequence n has residue
+// x_nk in group t of column k, else p = 0, and the 1.0 then meets the zero
+// slot.  The operand's values are therefore constants, and only the
+// positions vary: 16 bits per lane and instruction, precomputed once from
+// the MSA (k_sparse_index).  One instruction covers 8 groups, i.e. 32 K
+// slots at the issue cost of a dense 32x32x16 instruction: 14 issue slots
+// per column against 21 for the dense one-hot, and no one-hot arithmetic in
+// the loop.  With the coupling split into a hi and a lo BF16 half (two
+// passes), each instruction's 32 slots are stored once with the hi halves
+// and once with the lo halves, and both use the same positions.
+//
+// Lane layout of the instruction (verified on the device): A lane l holds
+// row l % 32 and groups 4 (l / 32) .. 4 (l / 32) + 3 (K slots 16 (l/32) ..
+// +16), value v in group v / 2, its position in bits 2v..2v+1 of the
+// selected 16-bit half of the index operand; B lane l holds column l % 32
+// and K slots 8 (l/32) .. +8 (values 0..7) and 16 + 8 (l/32) .. +8
+// (values 8..15); D lane l element i is row (i/4)*8 + 4 (l/32) + i%4,
+// column l % 32.
+#include "dtl.h"
+
+#include <type_traits>
+
+#include "ccmpred_amd_kernels.h"
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x16_t __attribute__((ext_vector_type(16)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+namespace {
+
+struct frag32 {  // the two granules of a B fragment, values 0..7 then 8..15
+  uint4 lo, hi;
+};
+
+constexpr int SP_GROUPS = 7;           // groups of 4 K slots per coupling column
+constexpr int SP_KC = 4 * SP_GROUPS;   // K slots per coupling column
+constexpr int SP_TK = 64;              // K slots per LDS stage (two instructions)
+
+// positions of instruction i for the lane half q of sequence row x (the
+// row's residues, 255 past the alignment): 4 groups, 4 bits each
+__device__ __forceinline__ uint32_t sparse_word(const uint8_t* __restrict__ x, int i, int q) {
+  uint32_t w = 0;
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    const int G = 8 * i + 4 * q + g, k = G / SP_GROUPS, t = G - k * SP_GROUPS;
+    const int b = x ? x[k] : 255;
+    const int p = (b < CCM_NA && b / 3 == t) ? b - 3 * t + 1 : 0;
+    w |= (uint32_t)(p << 2) << (4 * g);
+  }
+  return w;
+}
+
+// idx[(st*2 + q)*ldidx + n]: the positions of step st (two instructions:
+// low and high 16 bits) for lane half q of sequence n.  One pass: step st
+// holds instructions 2 st and 2 st + 1; two passes: instruction st twice
+// (its hi and lo halves are the step's two instructions).
+__global__ __launch_bounds__(256) void k_sparse_index(const uint8_t* __restrict__ msap, int Lp, int Np,
+                                                      int passes, int nsteps, uint32_t* __restrict__ idx,
+                                                      long ldidx) {
+  const long total = (long)nsteps * 2 * ldidx;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const long n = e % ldidx, sq = e / ldidx;
+    const int q = (int)(sq & 1), st = (int)(sq >> 1);
+    const uint8_t* x = n < Np ? msap + n * Lp : nullptr;
+    const int i0 = passes == 1 ? 2 * st : st, i1 = passes == 1 ? 2 * st + 1 : st;
+    idx[e] = sparse_word(x, i0, q) | (sparse_word(x, i1, q) << 16);
+  }
+}
+
+// couplings -> Ws (bf16): row c = a*Lp + j, K slot of column k, group t,
+// position p >= 1 holds W(3t + p - 1, k, a, j) (zero for k >= L); each
+// instruction's 32 slots are followed by their lo halves when PASSES = 2.
+// A thread takes one x-layout column C = a*L + j (consecutive lanes,
+// consecutive C: every load of a wave reads 256 contiguous bytes) and the
+// 128 bytes of Ws row c that hold 128 / (64 PASSES) instructions, so that
+// each row segment it writes is one whole 128-byte line.  Rows of padding
+// columns j >= L are not written (zeroed once).
+template <int PASSES>
+__global__ __launch_bounds__(256) void k_convert_ws(const float* __restrict__ x2, int L, int Lp,
+                                                    uint16_t* __restrict__ ws, long lds) {
+  constexpr int IPT = 2 / PASSES;  // instructions per thread
+  const int C = blockIdx.x * 256 + threadIdx.x;
+  if (C >= CCM_NA * L) return;
+  const long ld2 = (long)CCM_NAP * L;
+  const int a = C / L;
+  const long c = (long)a * Lp + (C - a * L);
+  uint4* dst = reinterpret_cast<uint4*>(ws + c * lds + (long)PASSES * 32 * IPT * blockIdx.y);
+#pragma unroll
+  for (int ii = 0; ii < IPT; ii++) {
+    const int i = blockIdx.y * IPT + ii;
+    // the 24 couplings of the instruction's 8 groups, all loads first
+    float v[8][3];
+#pragma unroll
+    for (int g = 0; g < 8; g++) {
+      const int G = 8 * i + g, k = G / SP_GROUPS, t = G - k * SP_GROUPS;
+#pragma unroll
+      for (int p = 0; p < 3; p++)
+        v[g][p] = k < L ? x2[(long)((3 * t + p) * L + k) * ld2 + C] : 0.f;
+    }
+    uint32_t hi[16], lo[16];
+#pragma unroll
+    for (int g = 0; g < 8; g++) {
+      // slots 4g .. 4g+3: zero, then the group's three residues
+      hi[2 * g] = ccm_f2bf2(0.f, v[g][0]);
+      hi[2 * g + 1] = ccm_f2bf2(v[g][1], v[g][2]);
+      lo[2 * g] = ccm_f2bf2(0.f, v[g][0] - ccm_bf2f((uint16_t)(hi[2 * g] >> 16)));
+      lo[2 * g + 1] = ccm_f2bf2(v[g][1] - ccm_bf2f((uint16_t)hi[2 * g + 1]),
+                                v[g][2] - ccm_bf2f((uint16_t)(hi[2 * g + 1] >> 16)));
+    }
+    uint4* d = dst + ii * 4 * PASSES;
+#pragma unroll
+    for (int u = 0; u < 4; u++) d[u] = make_uint4(hi[4 * u], hi[4 * u + 1], hi[4 * u + 2], hi[4 * u + 3]);
+    if constexpr (PASSES > 1) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) d[4 + u] = make_uint4(lo[4 * u], lo[4 * u + 1], lo[4 * u + 2], lo[4 * u + 3]);
+    }
+  }
+}
+
+// One workgroup: TM = WM*RB*32 sequences x TN = WN*CB*32 gathered rows of Ws
+// (all 21 residues of JG = TN / 21 alignment columns, as the dense fused
+// forward gathers them), K in steps of 64 slots through three direct-to-LDS
+// stages.  A lane's four row blocks rb hold the sequences 4*row + rb of its
+// wave row's 128, so that one 16-byte load brings the positions of all four
+// (idx row layout [step][half][sequence]).
+template <int WM, int WN, int RB, int CB>
+__device__ __forceinline__ void sparse_fwd_tile(const uint16_t* __restrict__ Ws, long lds, int Lp,
+                                                int nsteps, const uint32_t* __restrict__ idx, long ldidx,
+                                                int Ncol, int mt, int nt, const ccm_fwd_epi& epi) {
+  constexpr int NW = WM * WN;
+  constexpr int TM = WM * RB * 32, TN = WN * CB * 32;
+  constexpr int G = SP_TK / 8;                 // 16-byte granules per staged row
+  constexpr int A_INS = TN * G / (64 * NW);    // DTL instructions per wave and stage
+  constexpr int PER_STEP = A_INS + 1;          // and one 4-byte DTL load of positions
+  constexpr int JG = TN / CCM_NA;
+  constexpr int ESTR = 33;                     // epilogue row stride (floats)
+  constexpr int W_BYTES = TN * SP_TK * 2;
+  constexpr int P_BYTES = 2 * TM * 4;          // positions: [half][sequence] words
+  constexpr int E_BYTES = TN * ESTR * 4;
+  constexpr int STAGE_BYTES = W_BYTES + P_BYTES > E_BYTES ? W_BYTES + P_BYTES : E_BYTES;
+  static_assert(RB == 4 && TM == WM * 128, "four row blocks: one 16-byte read of positions per step");
+  static_assert(P_BYTES == NW * 256, "one 4-byte DTL load of positions per wave and stage");
+  static_assert((TN * G) % (64 * NW) == 0, "dense loader mapping");
+  static_assert(PER_STEP < 64, "vmcnt range");
+
+  const int n0 = nt * TM;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+  const int l32 = lane & 31, q = lane >> 5;
+  const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+
+  // gathered rows of Ws: local row a*JG + jj <- row a*Lp + j, j = mt*JG + jj
+  // (rows past 21*JG and columns j >= L repeat a row whose results are dropped)
+  const ccm_rsrc_t wrsrc = ccm_make_rsrc(Ws);
+  uint32_t woff[A_INS];
+#pragma unroll
+  for (int i = 0; i < A_INS; i++) {
+    const int p = (wave * A_INS + i) * 64 + lane;
+    const int r = p / G, g = swz_a<G>(r, p % G);
+    const int a = r / JG < CCM_NA ? r / JG : 0;
+    const int j = mt * JG + (r - (r / JG) * JG);
+    const long row = (long)a * Lp + (j < epi.L ? j : epi.L - 1);
+    woff[i] = (uint32_t)((row * lds + g * 8) * 2);
+  }
+  // positions of a step: wave w loads half w / (NW/2), sequences
+  // n0 + (w % (NW/2))*64 + lane, into stage bytes W_BYTES + 256 w (i.e. word
+  // [half][sequence of the tile]); a lane reads the words of its sequences
+  // wm*128 + 4*l32 + (0..3) of half q
+  const ccm_rsrc_t irsrc = ccm_make_rsrc(idx);
+  const uint32_t ioff =
+      (uint32_t)(((long)(wave / (NW / 2)) * ldidx + n0 + (wave % (NW / 2)) * 64 + lane) * 4);
+  const uint32_t istep = (uint32_t)(2 * ldidx * 4);
+  const int pread = W_BYTES + (q * TM + wm * 128 + 4 * l32) * 4;
+
+  floatx16 acc[RB][CB];
+#pragma unroll
+  for (int i = 0; i < RB; i++)
+#pragma unroll
+    for (int j = 0; j < CB; j++) acc[i][j] = (floatx16)0.0f;
+  const bf16x8_t aconst = __builtin_bit_cast(bf16x8_t, make_uint4(0x3F800000u, 0x3F800000u, 0x3F800000u, 0x3F800000u));
+
+  auto issue = [&](uint8_t* __restrict__ sp, int st) {
+    const uint32_t c = (uint32_t)st * (SP_TK * 2);
+#pragma unroll
+    for (int i = 0; i < A_INS; i++) ccm_buffer_load_lds16(wrsrc, sp + (wave_u * A_INS + i) * 1024, woff[i], c);
+    ccm_buffer_load_lds4(irsrc, sp + W_BYTES + wave_u * 256, ioff, (uint32_t)st * istep);
+  };
+  auto ld_pos = [&](const uint8_t* __restrict__ sp) { return *reinterpret_cast<const uint4*>(sp + pread); };
+  // B fragments of instruction sub (0, 1) of a stage: two granules per lane
+  uint4 bf[2][CB][2];
+  auto ld_frags = [&](const uint8_t* __restrict__ sp, uint4(&b)[CB][2], int sub) {
+    const uint16_t* ws = reinterpret_cast<const uint16_t*>(sp);
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++) {
+      const int r = (wn * CB + cb) * 32 + l32;
+      b[cb][0] = *reinterpret_cast<const uint4*>(&ws[r * SP_TK + swz_a<G>(r, 4 * sub + q) * 8]);
+      b[cb][1] = *reinterpret_cast<const uint4*>(&ws[r * SP_TK + swz_a<G>(r, 4 * sub + 2 + q) * 8]);
+    }
+  };
+  auto mfmas = [&](const uint4(&b)[CB][2], const uint4& pr, auto sub_c) {
+    constexpr int SUB = decltype(sub_c)::value;
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++) {
+      const frag32 f{b[cb][0], b[cb][1]};
+      const bf16x16_t bv = __builtin_bit_cast(bf16x16_t, f);
+      const int p4[4] = {(int)pr.x, (int)pr.y, (int)pr.z, (int)pr.w};
+#pragma unroll
+      for (int rb = 0; rb < RB; rb++)
+        acc[rb][cb] = __builtin_amdgcn_smfmac_f32_32x32x32_bf16(aconst, bv, acc[rb][cb], p4[rb], 0, SUB);
+    }
+  };
+  // one step on the stage at cur; nxt is the next step's stage, pc the
+  // positions of this step.  Instruction 1's fragments are read before
+  // instruction 0's MFMAs; before instruction 1's MFMAs the step hands over:
+  // wait for the next stage, barrier, read the next step's positions and
+  // first fragments, refill cur with step st+3.
+  uint4 pc;
+  auto step = [&](uint8_t* cur, const uint8_t* nxt, int st) {
+    ld_frags(cur, bf[1], 1);
+    mfmas(bf[0], pc, std::integral_constant<int, 0>{});
+    const uint4 pcur = pc;
+    ccm_wait_vmcnt<PER_STEP>();
+    ccm_lds_barrier();
+    ld_frags(nxt, bf[0], 0);
+    pc = ld_pos(nxt);
+    __builtin_amdgcn_sched_barrier(0);
+    issue(cur, st + 3 < nsteps ? st + 3 : nsteps - 1);
+    mfmas(bf[1], pcur, std::integral_constant<int, 1>{});
+  };
+
+  uint8_t* s0 = dtl_stage<STAGE_BYTES, 0>();
+  uint8_t* s1 = dtl_stage<STAGE_BYTES, 1>();
+  uint8_t* s2 = dtl_stage<STAGE_BYTES, 2>();
+  issue(s0, 0);
+  issue(s1, nsteps > 1 ? 1 : 0);
+  issue(s2, nsteps > 2 ? 2 : 0);
+  ccm_wait_vmcnt<0>();
+  ccm_lds_barrier();
+  ld_frags(s0, bf[0], 0);
+  pc = ld_pos(s0);
+  // complete triples without conditions (one scheduling region each: with a
+  // conditional step the compiler loses count of the loads in flight and
+  // waits for all of them), then the last one or two steps
+  int st = 0;
+  for (; st + 3 <= nsteps; st += 3) {
+    step(s0, s1, st);
+    step(s1, s2, st + 1);
+    step(s2, s0, st + 2);
+  }
+  if (st < nsteps) {
+    step(s0, s1, st);
+    if (st + 1 < nsteps) step(s1, s2, st + 1);
+  }
+  ccm_wait_vmcnt<0>();
+
+  // Softmax epilogue in rounds of 128 sequences (wave rows): the waves of
+  // round r put their accumulators into four LDS buffers, sequence-block
+  // major as fwd_softmax_round reads them: output row rho of block rb is
+  // sequence 4*rho + rb of the round, i.e. buffer rho / 8 = i / 4, column
+  // 4*(rho % 8) + rb.
+  ccm_lds_barrier();
+  float* const b0 = reinterpret_cast<float*>(s0);
+  float* const b1 = reinterpret_cast<float*>(s1);
+  float* const b2 = reinterpret_cast<float*>(s2);
+  float* const b3 = reinterpret_cast<float*>(dtl_stage<STAGE_BYTES, 3>());
+  constexpr int NSLOT = WM * 2 * 4 * JG;
+  __shared__ float gsum[NSLOT * 20];
+  __shared__ double fsum[NSLOT];
+  __shared__ float xs[20 * JG];
+  for (int e = tid; e < 20 * JG; e += NW * 64) {
+    const int a = e / JG, j = mt * JG + (e - a * JG);
+    xs[e] = j < epi.L ? epi.x1[(long)a * epi.L + j] : 0.f;
+  }
+  const ccm_rsrc_t drsrc = ccm_make_rsrc(epi.dt);
+  const uint32_t dstep = (uint32_t)((long)Lp * epi.lddt * 2), dlo = (uint32_t)(epi.lo_off * 2);
+  for (int r = 0; r < WM; r++) {
+    if (wm == r) {
+#pragma unroll
+      for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++) {
+          const int R = (wn * CB + cb) * 32 + l32;
+#pragma unroll
+          for (int i = 0; i < 16; i++) {
+            float* const bb = (i >> 2) == 0 ? b0 : (i >> 2) == 1 ? b1 : (i >> 2) == 2 ? b2 : b3;
+            bb[R * ESTR + 16 * q + 4 * (i & 3) + rb] = acc[rb][cb][i];
+          }
+        }
+    }
+    __syncthreads();
+    fwd_softmax_round<JG, NW, ESTR>(b0, b1, b2, b3, xs, gsum, fsum, epi, mt, n0, Ncol, r, wave_u, lane,
+                                    drsrc, dstep, dlo);
+    __syncthreads();
+  }
+  fwd_softmax_partials<JG, NW * 64, WM>(gsum, fsum, epi, mt, n0, Ncol, tid);
+}
+
+template <int WM, int WN, int RB, int CB>
+__global__ __launch_bounds__(64 * WM * WN, 1) void k_sparse_fwd(
+    const uint16_t* __restrict__ Ws, long lds, int Lp, int nsteps, const uint32_t* __restrict__ idx,
+    long ldidx, int Ncol, uint32_t tiles_n, uint32_t tiles, uint32_t total, ccm_fwd_epi epi) {
+  const uint32_t qid = ccm_xcd_remap(blockIdx.x, total);
+  if (qid >= tiles) return;
+  sparse_fwd_tile<WM, WN, RB, CB>(Ws, lds, Lp, nsteps, idx, ldidx, Ncol, qid / tiles_n, qid % tiles_n, epi);
+}
+
+constexpr int SF_TM = 256, SF_TN = 256;
+
+}  // namespace
+
+extern "C" {
+
+long ccm_sparse_ws_len(int Lp, int passes) { return (long)passes * SP_KC * Lp; }
+
+int ccm_sparse_steps(int Lp, int passes) { return Lp % 64 ? 0 : passes * SP_GROUPS * Lp / 16; }
+
+long ccm_sparse_index_ld(int Np) { return ((long)Np + SF_TM - 1) / SF_TM * SF_TM; }
+
+int ccm_sparse_fwd_fits(int Lp, int passes_fwd, int passes, int ncols) {
+  const long P = (long)CCM_NA * Lp;
+  return (P + 1) * ccm_sparse_ws_len(Lp, passes_fwd) * 2 <= 0x7FFFFFFFL &&
+         P * passes * (long)ncols * 2 <= 0x7FFFFFFFL;
+}
+
+int ccm_launch_sparse_index(const uint8_t* d_msap, int Lp, int Np, int passes, uint32_t* d_idx,
+                            ccm_stream_t stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const int nsteps = ccm_sparse_steps(Lp, passes);
+  if (!d_msap || !d_idx || nsteps <= 0 || Np < 1 || passes < 1 || passes > 2) return CCM_ERR_ARG;
+  const long ld = ccm_sparse_index_ld(Np);
+  const long total = (long)nsteps * 2 * ld;
+  const int blocks = (int)((total + 255) / 256 < 65536 ? (total + 255) / 256 : 65536);
+  hipLaunchKernelGGL(k_sparse_index, dim3(blocks), dim3(256), 0, s, d_msap, Lp, Np, passes, nsteps, d_idx, ld);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_convert_ws(const float* d_x, int L, int Lp, uint16_t* d_ws, long lds, int passes,
+                          ccm_stream_t stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (Lp % 64 || Lp < L || L < 1 || passes < 1 || passes > 2 || lds < ccm_sparse_ws_len(Lp, passes) ||
+      lds % 8)
+    return CCM_ERR_ARG;
+  // 7 Lp / 8 instructions (a multiple of 56), 2 per thread row with one pass
+  const dim3 grid((CCM_NA * L + 255) / 256, SP_GROUPS * Lp / 8 / (passes == 1 ? 2 : 1));
+  if (passes == 1)
+    hipLaunchKernelGGL(k_convert_ws<1>, grid, dim3(256), 0, s, d_x + ccm_nsp(L), L, Lp, d_ws, lds);
+  else
+    hipLaunchKernelGGL(k_convert_ws<2>, grid, dim3(256), 0, s, d_x + ccm_nsp(L), L, Lp, d_ws, lds);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_sparse_fwd_softmax(const uint16_t* d_ws, long lds, int Lp, int passes_fwd,
+                                  const uint32_t* d_idx, long ldidx, int ncols, const float* d_x1,
+                                  const uint8_t* d_msapt, long ldmt, const float* d_w, int L, int Nc,
+                                  uint16_t* d_dt, long lddt, long lo_off, int passes, int mode, int blk0,
+                                  double* d_fpart, double* d_g1part, ccm_stream_t stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const long P = (long)CCM_NA * Lp;
+  const int nsteps = ccm_sparse_steps(Lp, passes_fwd);
+  if (Lp % 64 || Lp < L || L < 1 || ncols % 128 || ncols <= 0 || Nc < 1 || Nc > ncols || nsteps <= 0 ||
+      lds < ccm_sparse_ws_len(Lp, passes_fwd) || lds % 8 || !d_idx || ldidx < ncols || !d_fpart ||
+      (mode && (!d_dt || !d_g1part)) || blk0 < 0)
+    return CCM_ERR_ARG;
+  if (mode && (passes < 1 || passes > 2 || lddt < (long)passes * ncols || (passes > 1 && lo_off < ncols)))
+    return CCM_ERR_ARG;
+  if (!ccm_sparse_fwd_fits(Lp, passes_fwd, mode ? passes : 1, mode ? ncols : 0)) return CCM_ERR_ARG;
+  // the positions are read with 32-bit offsets from d_idx
+  if ((long)nsteps * 2 * ldidx * 4 > 0x7FFFFFFFL) return CCM_ERR_ARG;
+  ccm_fwd_epi epi{d_x1, d_msapt, ldmt, d_w, L, Nc, d_dt, lddt, lo_off, passes, mode, blk0, d_fpart, d_g1part};
+  constexpr int JG = SF_TN / CCM_NA;
+  const uint32_t tiles_n = (uint32_t)((ncols + SF_TM - 1) / SF_TM);
+  const uint32_t tiles = (uint32_t)((L + JG - 1) / JG) * tiles_n, total = (tiles + 7u) & ~7u;
+  // the sequence tiles read positions up to the next multiple of 256
+  if (ldidx < (long)tiles_n * SF_TM) return CCM_ERR_ARG;
+  hipLaunchKernelGGL((k_sparse_fwd<2, 4, 4, 2>), dim3(total), dim3(512), 0, s, d_ws, lds, Lp, nsteps, d_idx,
+                     ldidx, ncols, tiles_n, tiles, total, epi);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+}  // extern "C"

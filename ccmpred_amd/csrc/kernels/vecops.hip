@@ -357,15 +357,15 @@ __global__ __launch_bounds__(256) void k_vec_pad(float* __restrict__ y, const fl
 
 // The sufficient-decrease test and the shrinking step of the host loop in
 // cg_device.c (cg_decrease, cg_shrink), operation for operation: the Armijo
-// test, or below the objective's relative resolution CG_FRES the approximate
+// test, or below the objective's relative resolution fres the approximate
 // Armijo test on the slope (Hager and Zhang 2005); after a failure with no
 // lower bracket, the minimiser of the quadratic through f0, dg0 and ft
 // within [0.1, alpha_mul] alpha.
-#define CG_FRES 1e-6
-__device__ bool cg_decrease(double ft, double f0, double alpha, double dg0, double dgt, double ftol) {
+__device__ bool cg_decrease(double ft, double f0, double alpha, double dg0, double dgt, double ftol,
+                            double fres) {
 #pragma clang fp contract(off)
   if (ft <= f0 + ftol * alpha * dg0) return true;
-  return ft <= f0 + CG_FRES * fabs(f0) && dgt <= (2.0 * ftol - 1.0) * dg0;
+  return ft <= f0 + fres * fabs(f0) && dgt <= (2.0 * ftol - 1.0) * dg0;
 }
 
 __device__ double cg_shrink(double ft, double f0, double alpha, double dg0, double alpha_mul) {
@@ -416,7 +416,7 @@ __device__ void cg_control(ccm_cg_ctl* c, double* log, int log_cap, const ccm_cg
     c->steepest = 1;
   } else if (c->phase == CCM_CG_FORCED) {
     accept = true;
-  } else if (!cg_decrease(ft, c->f0, c->alpha, c->dg0, dgt, p.ftol)) {
+  } else if (!cg_decrease(ft, c->f0, c->alpha, c->dg0, dgt, p.ftol, p.fres)) {
     // not enough decrease: shrink, bisecting once bracketed
     c->hi = c->alpha;
     c->alpha = c->lo > 0.0 ? 0.5 * (c->lo + c->hi) : cg_shrink(ft, c->f0, c->alpha, c->dg0, p.alpha_mul);

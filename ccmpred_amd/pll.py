@@ -83,7 +83,7 @@ class PLL:
 
     def __init__(self, msa, theta=0.8, lambda_single=0.01, lambda_pair=None,
                  lambda_pair_factor=0.2, precision="fp32", device=0, verbose=False, chunk_rows=0,
-                 gemm_variant=-1):
+                 gemm_variant=-1, sparse=True):
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {list(PRECISIONS)}")
         self.msa = np.ascontiguousarray(msa, dtype=np.uint8)
@@ -115,6 +115,7 @@ class PLL:
         cfg.verbose = 1 if verbose else 0
         cfg.chunk_rows = int(chunk_rows)
         cfg.gemm_variant = int(gemm_variant)
+        cfg.sparse_fwd = 1 if sparse else 0
         rc = _lib.lib().init_hip_ex(self._uptr, C.byref(cfg))
         if rc != 0:
             raise RuntimeError("init_hip_ex failed")

@@ -70,6 +70,8 @@ typedef struct ccm_hip_config {
 	int verbose;     /* 1: print the reweighting summary like the reference */
 	int chunk_rows;  /* sequences per chunk (rounded up to 256 in FP32, 512 in BF16); 0 = all */
 	int gemm_variant; /* MFMA tile variant (0..ccm_gemm_variants()-1); -1 = default */
+	int sparse_fwd;  /* BF16: forward contraction on 2:4 structured-sparse MFMA (1, the
+	                  * default, where the shape allows it) or dense (0) */
 } ccm_hip_config;
 
 void ccm_hip_default_config(ccm_hip_config *cfg);
@@ -103,6 +105,13 @@ int ccm_hip_objective(void *instance, const float *d_x, int nvar, double *f);
  * into the tied gradient.  No counterpart in the reference. */
 int ccm_hip_forward(void *instance, const float *d_x, int nvar, double *f);
 int ccm_hip_backward(void *instance, const float *d_x, float *d_g, int nvar);
+
+/* Relative resolution of the instance's objective: the size of f changes
+ * below which its differences are rounding, 1e-5 for a single-pass BF16
+ * forward (its logits sum L BF16-rounded couplings) and 1e-6 otherwise.  The
+ * device CG's approximate decrease test uses it when the callback is
+ * evaluate_hip.  No counterpart in the reference. */
+double ccm_hip_objective_resolution(void *instance);
 
 /* Device bytes the instance holds (all buffers allocated at init). */
 size_t ccm_hip_device_bytes(void *instance);

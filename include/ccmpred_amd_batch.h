@@ -82,6 +82,7 @@ typedef struct ccm_group {
 typedef struct ccm_cg_prm {
 	int32_t max_iterations, k, max_linesearch, pad;
 	double epsilon, alpha_mul, ftol, wolfe;
+	double fres; /* relative resolution of f (ccm_hip_objective_resolution) */
 } ccm_cg_prm;
 
 #endif

@@ -84,6 +84,32 @@ int ccm_launch_gemm_fwd_softmax(const uint16_t *d_wt, long lda, int Lp, int pass
                                 uint16_t *d_dt, long lddt, long lo_off, int passes, int mode,
                                 int blk0, double *d_fpart, double *d_g1part, int variant,
                                 ccm_stream_t s);
+/* The fused forward on 2:4 structured-sparse MFMA (v_smfmac_f32_32x32x32_bf16),
+ * same outputs as ccm_launch_gemm_fwd_softmax (cu:30-141 in one launch; no
+ * counterpart kernel).  The one-hot operand is the sparse one: every
+ * coupling column takes 7 groups of 4 K slots (a zero slot and three
+ * residues), and per lane only the slot positions vary, precomputed from the
+ * MSA.  convert_ws writes the couplings in that K layout
+ * (ccm_sparse_ws_len(Lp, passes) bf16 per row c = a*Lp + j, hi and lo halves
+ * per instruction for passes = 2; rows j >= L are not written: zero them
+ * once).  sparse_index writes the positions of the Np padded MSA rows:
+ * ccm_sparse_steps(Lp, passes) x 2 rows of ccm_sparse_index_ld(Np) words.
+ * The forward takes d_idx at the chunk's first sequence (a multiple of 256
+ * sequences into the rows); the rows must hold ceil(ncols / 256) * 256
+ * sequences from there. */
+long ccm_sparse_ws_len(int Lp, int passes);
+int ccm_sparse_steps(int Lp, int passes);
+long ccm_sparse_index_ld(int Np);
+int ccm_sparse_fwd_fits(int Lp, int passes_fwd, int passes, int ncols);
+int ccm_launch_convert_ws(const float *d_x, int L, int Lp, uint16_t *d_ws, long lds, int passes,
+                          ccm_stream_t s);
+int ccm_launch_sparse_index(const uint8_t *d_msap, int Lp, int Np, int passes, uint32_t *d_idx,
+                            ccm_stream_t s);
+int ccm_launch_sparse_fwd_softmax(const uint16_t *d_ws, long lds, int Lp, int passes_fwd,
+                                  const uint32_t *d_idx, long ldidx, int ncols, const float *d_x1,
+                                  const uint8_t *d_msapt, long ldmt, const float *d_w, int L, int Nc,
+                                  uint16_t *d_dt, long lddt, long lo_off, int passes, int mode,
+                                  int blk0, double *d_fpart, double *d_g1part, ccm_stream_t s);
 int ccm_launch_gemm_grad(const uint16_t *d_dt, long lda, int Lp, int passes,
                          const uint8_t *d_msapt, long ldmt, int kcols, float *d_gt, long ldc,
                          int accumulate, int variant, ccm_stream_t s);

@@ -475,16 +475,17 @@ static double ora_dot(const float *a, const float *b, long n)
 /* Sufficient decrease of a trial at step alpha (f0, dg0: objective and
  * slope at the start of the search; ft, dgt: at the trial).  The Armijo
  * test, or, where f differences sink below the objective's resolution
- * (ORA_CG_FRES relative), the approximate form of Hager and Zhang (SIAM J.
- * Optim. 16(1), 2005, eq. 4.1): the slope at the trial has not risen above
- * (1 - 2 ftol) |dg0| -- the Armijo test of a quadratic, written with
- * derivatives -- and f has not risen by more than that resolution. */
-#define ORA_CG_FRES 1e-6
-static int ora_cg_decrease(double ft, double f0, double alpha, double dg0, double dgt, double ftol)
+ * (fres relative; 1e-6 when the parameter leaves it 0), the approximate
+ * form of Hager and Zhang (SIAM J. Optim. 16(1), 2005, eq. 4.1): the slope
+ * at the trial has not risen above (1 - 2 ftol) |dg0| -- the Armijo test of
+ * a quadratic, written with derivatives -- and f has not risen by more than
+ * that resolution. */
+static int ora_cg_decrease(double ft, double f0, double alpha, double dg0, double dgt, double ftol,
+                           double fres)
 {
 	if (ft <= f0 + ftol * alpha * dg0)
 		return 1;
-	return ft <= f0 + ORA_CG_FRES * fabs(f0) && dgt <= (2.0 * ftol - 1.0) * dg0;
+	return ft <= f0 + (fres > 0.0 ? fres : 1e-6) * fabs(f0) && dgt <= (2.0 * ftol - 1.0) * dg0;
 }
 
 /* Next step after a trial that failed the decrease test with no lower
@@ -540,7 +541,7 @@ int ora_cg(int n, float *x, double *fx, ora_eval_fn eval, void *ctx, const ora_c
 			ft = eval(ctx, x, g, n);
 			nevals++;
 			const double dgt = ora_dot(g, s, n);
-			if (!ora_cg_decrease(ft, f0, alpha, dg0, dgt, p->ftol)) {
+			if (!ora_cg_decrease(ft, f0, alpha, dg0, dgt, p->ftol, p->fres)) {
 				/* not enough decrease: shrink, bisecting once bracketed */
 				hi = alpha;
 				alpha = lo > 0.0 ? 0.5 * (lo + hi)

@@ -118,6 +118,7 @@ typedef struct ora_cg_param {
 	double alpha_mul;
 	double ftol;
 	double wolfe;
+	double fres; /* relative resolution of f for the approximate decrease test; 0: 1e-6 */
 } ora_cg_param;
 int ora_cg(int n, float *x, double *fx, ora_eval_fn eval, void *ctx, const ora_cg_param *p,
            int *iterations, int *evaluations, double *fhist);

@@ -42,7 +42,7 @@ _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 class CGParam(C.Structure):
     _fields_ = [("max_iterations", C.c_int), ("epsilon", C.c_double), ("k", C.c_int),
                 ("max_linesearch", C.c_int), ("alpha_mul", C.c_double), ("ftol", C.c_double),
-                ("wolfe", C.c_double)]
+                ("wolfe", C.c_double), ("fres", C.c_double)]
 
 
 def _declare(L):
@@ -209,10 +209,10 @@ def normalize(mat):
 
 
 def cg_pll_f32(msa, w, x0, lambda_single, lambda_pair, max_iterations=50, epsilon=0.0, k=5,
-               max_linesearch=5, alpha_mul=0.5, ftol=1e-4, wolfe=0.2, evaluator="f32"):
+               max_linesearch=5, alpha_mul=0.5, ftol=1e-4, wolfe=0.2, evaluator="f32", fres=1e-6):
     msa = _msa(msa)
     x = np.array(x0, dtype=np.float32, copy=True)
-    p = CGParam(max_iterations, epsilon, k, max_linesearch, alpha_mul, ftol, wolfe)
+    p = CGParam(max_iterations, epsilon, k, max_linesearch, alpha_mul, ftol, wolfe, fres)
     fx = C.c_double()
     it = C.c_int()
     ne = C.c_int()

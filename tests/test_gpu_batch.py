@@ -75,7 +75,7 @@ def test_batch_evaluate_bitwise(prec):
         gb3 = [b.get_g(i) for i in range(b.n)]
         b.set_variants(*nv)
         for i, (m, x) in enumerate(zip(msas, xs)):
-            with P.PLL(m, precision=prec) as p:
+            with P.PLL(m, precision=prec, sparse=False) as p:
                 assert np.array_equal(p.weights, b.weights[i])
                 fs, gs = p.evaluate(torch.from_numpy(x).cuda())
                 gs = gs.cpu().numpy()
@@ -111,7 +111,7 @@ def test_batch_solve_matches_single():
                                       P.contact_map(xb[i], L, apc=apc, normalize=norm)), (i, apc, norm)
     say(f"  rounds={rounds}")
     for i, (m, x) in enumerate(zip(msas, xs)):
-        with P.PLL(m, precision="bf16") as p:
+        with P.PLL(m, precision="bf16", sparse=False) as p:
             xd = torch.from_numpy(x).cuda()
             r = p.conjugrad(xd, max_iterations=its, epsilon=0.0)
             xs_ = xd.cpu().numpy()
@@ -143,7 +143,7 @@ def test_batch_autotune_then_solve():
         res = [b.result(i) for i in range(b.n)]
         xb = [b.get_x(i) for i in range(b.n)]
     for i, (m, x) in enumerate(zip(msas, xs)):
-        with P.PLL(m, precision="bf16") as p:
+        with P.PLL(m, precision="bf16", sparse=False) as p:
             xd = torch.from_numpy(x).cuda()
             r = p.conjugrad(xd, max_iterations=its, epsilon=0.0)
             xs_ = xd.cpu().numpy()
