@@ -272,7 +272,7 @@ def run_batched(args):
         total = 0.0
         same = 0
         for i, m in enumerate(msas):
-            with PLL(m, precision=args.batch_precision, sparse=False) as p:
+            with PLL(m, precision=args.batch_precision) as p:
                 xd = torch.from_numpy(init_bias(m)).cuda()
                 torch.cuda.synchronize()
                 ts = time.perf_counter()

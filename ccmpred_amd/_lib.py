@@ -73,6 +73,8 @@ EXPORTED = [
     "ccm_contact_map_noalloc_max_l", "ccm_launch_contact_map_noalloc",
     "ccm_sparse_ws_len", "ccm_sparse_steps", "ccm_sparse_index_ld", "ccm_sparse_fwd_fits",
     "ccm_launch_convert_ws", "ccm_launch_sparse_index", "ccm_launch_sparse_fwd_softmax",
+    "ccm_convert_ws_blocks", "ccm_sparse_fwd_tile_rows", "ccm_sparse_fwd_tile_count",
+    "ccm_launch_convert_ws_grouped", "ccm_launch_sparse_fwd_softmax_grouped",
     "ccm_batch_create", "ccm_batch_destroy", "ccm_batch_size", "ccm_batch_x", "ccm_batch_g",
     "ccm_batch_write_x", "ccm_batch_read", "ccm_batch_contact_map", "ccm_batch_stream", "ccm_batch_device_bytes",
     "ccm_batch_set_variants", "ccm_batch_get_variants", "ccm_batch_autotune", "ccm_batch_evaluate",

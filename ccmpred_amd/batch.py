@@ -26,7 +26,7 @@ class Batch:
     reference's defaults (theta, lambda_single, lambda_pair = factor (L-1))."""
 
     def __init__(self, msas, theta=0.8, lambda_single=0.01, lambda_pair_factor=0.2,
-                 precision="bf16", device=0):
+                 precision="bf16", device=0, sparse=True):
         if precision not in PRECISIONS or PRECISIONS[precision][0] != 1:
             raise ValueError("the batched form runs a BF16 precision form (bf16, bf16f, bf16x2)")
         self.msas = [np.ascontiguousarray(m, dtype=np.uint8) for m in msas]
@@ -58,6 +58,7 @@ class Batch:
         cfg.device = int(device)
         cfg.precision, cfg.bf16_passes, cfg.bf16_fwd_passes = PRECISIONS[precision]
         cfg.verbose = 0
+        cfg.sparse_fwd = 1 if sparse else 0
         h = C.c_void_p()
         if _lib.lib().ccm_batch_create(C.addressof(self._uds), n, C.byref(cfg), C.byref(h)) != 0:
             raise RuntimeError("ccm_batch_create failed")

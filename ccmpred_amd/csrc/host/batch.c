@@ -48,6 +48,7 @@ typedef struct {
 struct ccm_batch {
 	int n;
 	int fused; /* forward contractions with the softmax fused (every alignment fits) */
+	int sparse; /* and on structured-sparse MFMA (every alignment fits) */
 	ccm_hip_config cfg;
 	hipStream_t s;
 	userdata *uds;
@@ -104,7 +105,9 @@ enum { K_CONV, K_SOFT, K_ASM, K_ALL, K_DOTS, K_VEC };
 static uint32_t blocks_of(const ccm_bprob *p, int kind)
 {
 	switch (kind) {
-	case K_CONV: return (uint32_t)ccm_convert_wt_blocks(p->L, p->Lp);
+	case K_CONV:
+		return (uint32_t)(p->sparse ? ccm_convert_ws_blocks(p->L, p->Lp, p->passes_fwd)
+		                            : ccm_convert_wt_blocks(p->L, p->Lp));
 	case K_SOFT: return (uint32_t)ccm_softmax_cols_blocks(p->L, p->N);
 	case K_ASM: return (uint32_t)ccm_assemble_blocks(p->L);
 	case K_ALL: return 1;
@@ -150,6 +153,16 @@ static int panel_cmp(const void *a, const void *b)
 	return x->row < y->row ? -1 : (x->row > y->row);
 }
 
+/* row panels of a grouped contraction's tiles for one alignment */
+static int panel_rows(const ccm_bprob *p, int mode, int variant, int tm)
+{
+	if (mode == 3)
+		return (int)ccm_sparse_fwd_tile_rows(p->L);
+	if (mode == 2)
+		return ccm_fwd_softmax_tile_rows(variant, p->L);
+	return (p->P + tm - 1) / tm;
+}
+
 /* Tile map of a grouped contraction.  Workgroups are dispatched to the 8
  * XCDs round-robin (workgroup b to XCD b % 8), and the tiles of a group
  * differ in cost (their K length is the alignment's).  Whole row panels go
@@ -160,15 +173,15 @@ static int panel_cmp(const void *a, const void *b)
 static uint32_t push_tile_map(tabbuf *t, const ccm_bprob *probs, const int *list, int np, int mode,
                               int variant, const uint32_t *offs)
 {
-	int tm, tn;
-	if (ccm_gemm_tile_shape(variant, &tm, &tn))
+	int tm = 256, tn = 256;
+	if (mode != 3 && ccm_gemm_tile_shape(variant, &tm, &tn))
 		return 0;
 	/* mode 2: the fused forward, whose row tiles cover tm / 21 alignment
-	 * columns each */
+	 * columns each; mode 3: the sparse fused forward (panels of 12
+	 * columns, 256 sequences per tile) */
 	size_t npan = 0;
 	for (int i = 0; i < np; i++)
-		npan += (size_t)(mode == 2 ? ccm_fwd_softmax_tile_rows(variant, probs[list[i]].L)
-		                           : (probs[list[i]].P + tm - 1) / tm);
+		npan += (size_t)panel_rows(&probs[list[i]], mode, variant, tm);
 	const size_t ntiles = offs[np];
 	panel *pan = (panel *)malloc(sizeof(panel) * (npan ? npan : 1));
 	uint32_t *xl = (uint32_t *)malloc(sizeof(uint32_t) * 8 * (ntiles ? ntiles : 1));
@@ -182,8 +195,10 @@ static uint32_t push_tile_map(tabbuf *t, const ccm_bprob *probs, const int *list
 		const ccm_bprob *p = &probs[list[i]];
 		const int ncol = mode != 1 ? p->ncols : p->P;
 		const int cols = (ncol + tn - 1) / tn;
-		const int rows = mode == 2 ? ccm_fwd_softmax_tile_rows(variant, p->L) : (p->P + tm - 1) / tm;
-		const double steps = mode != 1 ? (double)p->passes_fwd * p->P / 64.0 : (double)p->passes * p->ncols / 64.0;
+		const int rows = panel_rows(p, mode, variant, tm);
+		const double steps = mode == 3   ? (double)ccm_sparse_steps(p->Lp, p->passes_fwd)
+		                     : mode != 1 ? (double)p->passes_fwd * p->P / 64.0
+		                                 : (double)p->passes * p->ncols / 64.0;
 		for (int r = 0; r < rows; r++) {
 			pan[npan].work = steps * cols;
 			pan[npan].prob = i;
@@ -231,7 +246,8 @@ static int add_group(tabbuf *t, bgroup *bg, const ccm_bprob *probs, const int *l
 		const ccm_bprob *p = &probs[list[i]];
 		uint32_t nb;
 		if (kind < 0)
-			nb = mode == 2   ? ccm_fwd_softmax_tile_count(variant, p->L, p->ncols)
+			nb = mode == 3   ? ccm_sparse_fwd_tile_count(p->L, p->ncols)
+			     : mode == 2 ? ccm_fwd_softmax_tile_count(variant, p->L, p->ncols)
 			     : mode == 0 ? ccm_gemm_tile_count(variant, p->P, p->ncols)
 			                 : ccm_gemm_tile_count(variant, p->P, p->P);
 		else
@@ -293,8 +309,11 @@ static int build_tables(ccm_batch *b)
 	    add_group(&t, &b->g_vec, b->h_probs, all, b->n, K_VEC, 0, 0))
 		goto out;
 	b->n_fwd = b->n_grad = 0;
+	/* the sparse forward: one launch over every alignment */
+	if (b->sparse && add_group(&t, &b->g_fwd[b->n_fwd++], b->h_probs, all, b->n, -1, 3, -1))
+		goto out;
 	/* one GEMM launch per tile variant in use */
-	for (int mode = 0; mode < 2; mode++) {
+	for (int mode = b->sparse ? 1 : 0; mode < 2; mode++) {
 		const int *var = mode == 0 ? b->var_fwd : b->var_grad;
 		for (int v = 0; v < ccm_gemm_variants(); v++) {
 			int np = 0;
@@ -373,6 +392,9 @@ static void bprob_from_state(ccm_bprob *p, const ccm_hip_state *st)
 	p->g1part = st->d_g1part;
 	p->regpair = st->d_regpair;
 	p->regsingle = st->d_regsingle;
+	p->sparse = st->sparse;
+	p->sidx = st->d_sidx;
+	p->ldsidx = st->ldsidx;
 }
 
 int ccm_batch_size(const ccm_batch *b) { return b ? b->n : 0; }
@@ -507,9 +529,14 @@ int ccm_batch_create(userdata *uds, int nprob, const ccm_hip_config *cfg_in, ccm
 	{
 		const int fp = cfg.bf16_fwd_passes ? cfg.bf16_fwd_passes : cfg.bf16_passes;
 		b->fused = 1;
-		for (int i = 0; i < nprob; i++)
-			b->fused &= ccm_fwd_softmax_fits((uds[i].ncol + 63) / 64 * 64, fp, cfg.bf16_passes,
-			                                 (uds[i].nrow + 127) / 128 * 128);
+		b->sparse = cfg.sparse_fwd != 0;
+		for (int i = 0; i < nprob; i++) {
+			const int Lp = (uds[i].ncol + 63) / 64 * 64, Np = (uds[i].nrow + 127) / 128 * 128;
+			b->fused &= ccm_fwd_softmax_fits(Lp, fp, cfg.bf16_passes, Np);
+			b->sparse &= ccm_sparse_fwd_fits(Lp, fp, cfg.bf16_passes, Np) &&
+			             (long)ccm_sparse_steps(Lp, fp) * 2 * ccm_sparse_index_ld(Np) * 4 <= 0x7FFFFFFFL;
+		}
+		b->sparse &= b->fused;
 	}
 	b->h_probs = (ccm_bprob *)calloc((size_t)nprob, sizeof(ccm_bprob));
 	b->var_fwd = (int *)malloc(sizeof(int) * (size_t)nprob);
@@ -523,13 +550,17 @@ int ccm_batch_create(userdata *uds, int nprob, const ccm_hip_config *cfg_in, ccm
 	HIPCHK(balloc(b, (void **)&b->d_ctl, sizeof(ccm_cg_ctl) * (size_t)nprob));
 	HIPCHK(hipMemset(b->d_ctl, 0, sizeof(ccm_cg_ctl) * (size_t)nprob));
 	for (int i = 0; i < nprob; i++) {
-		if (ccm_init_hip_state(&uds[i], &cfg, b->fused, 0) != 0) {
+		if (ccm_init_hip_state(&uds[i], &cfg, b->fused, b->sparse) != 0) {
 			fprintf(stderr, "ccmpred_amd batch: alignment %d failed to initialise\n", i);
 			goto fail;
 		}
 		const ccm_hip_state *st = (const ccm_hip_state *)uds[i].extra;
 		ccm_bprob *p = &b->h_probs[i];
 		bprob_from_state(p, st);
+		if (p->sparse != b->sparse) {
+			fprintf(stderr, "ccmpred_amd batch: alignment %d chose another forward form\n", i);
+			goto fail;
+		}
 		const size_t vb = sizeof(float) * (size_t)p->nvp;
 		HIPCHK(balloc(b, (void **)&p->x, vb));
 		HIPCHK(balloc(b, (void **)&p->g, vb));
@@ -597,9 +628,10 @@ static int enqueue_gemms(ccm_batch *b, int mode)
 	const int ng = mode == 0 ? b->n_fwd : b->n_grad;
 	const bgroup *gs = mode == 0 ? b->g_fwd : b->g_grad;
 	for (int i = 0; i < ng; i++) {
-		int r = mode == 1  ? ccm_launch_gemm_grad_grouped(b->d_probs, gs[i].g, gs[i].variant, b->s)
-		        : b->fused ? ccm_launch_gemm_fwd_softmax_grouped(b->d_probs, gs[i].g, gs[i].variant, b->s)
-		                   : ccm_launch_gemm_fwd_grouped(b->d_probs, gs[i].g, gs[i].variant, b->s);
+		int r = mode == 1   ? ccm_launch_gemm_grad_grouped(b->d_probs, gs[i].g, gs[i].variant, b->s)
+		        : b->sparse ? ccm_launch_sparse_fwd_softmax_grouped(b->d_probs, gs[i].g, b->s)
+		        : b->fused  ? ccm_launch_gemm_fwd_softmax_grouped(b->d_probs, gs[i].g, gs[i].variant, b->s)
+		                    : ccm_launch_gemm_fwd_grouped(b->d_probs, gs[i].g, gs[i].variant, b->s);
 		if (r)
 			return r;
 	}
@@ -608,7 +640,10 @@ static int enqueue_gemms(ccm_batch *b, int mode)
 
 static int enqueue_eval(ccm_batch *b)
 {
-	KCHK(ccm_launch_convert_wt_grouped(b->d_probs, b->g_conv.g, b->s));
+	if (b->sparse)
+		KCHK(ccm_launch_convert_ws_grouped(b->d_probs, b->g_conv.g, b->h_probs[0].passes_fwd, b->s));
+	else
+		KCHK(ccm_launch_convert_wt_grouped(b->d_probs, b->g_conv.g, b->s));
 	KCHK(enqueue_gemms(b, 0));
 	if (!b->fused)
 		KCHK(ccm_launch_softmax_cols_grouped(b->d_probs, b->g_soft.g, b->s));
@@ -727,6 +762,12 @@ int ccm_batch_autotune(ccm_batch *b, int reps, double *ms_out)
 		memcpy(cur, var, sizeof(int) * (size_t)n);
 		if (time_gemms(b, mode, reps, &best))
 			goto out;
+		if (mode == 0 && b->sparse) {
+			/* the sparse forward has one tile shape */
+			if (ms_out)
+				ms_out[0] = best;
+			continue;
+		}
 		for (int v = 0; v < nv; v++) {
 			int any = 0;
 			if (!ccm_gemm_variant_direct(v))

@@ -87,18 +87,18 @@ __global__ __launch_bounds__(256) void k_sparse_index(const uint8_t* __restrict_
 // each row segment it writes is one whole 128-byte line.  Rows of padding
 // columns j >= L are not written (zeroed once).
 template <int PASSES>
-__global__ __launch_bounds__(256) void k_convert_ws(const float* __restrict__ x2, int L, int Lp,
-                                                    uint16_t* __restrict__ ws, long lds) {
+__device__ __forceinline__ void convert_ws_block(const float* __restrict__ x2, int L, int Lp,
+                                                 uint16_t* __restrict__ ws, long lds, int bx, int by) {
   constexpr int IPT = 2 / PASSES;  // instructions per thread
-  const int C = blockIdx.x * 256 + threadIdx.x;
+  const int C = bx * 256 + threadIdx.x;
   if (C >= CCM_NA * L) return;
   const long ld2 = (long)CCM_NAP * L;
   const int a = C / L;
   const long c = (long)a * Lp + (C - a * L);
-  uint4* dst = reinterpret_cast<uint4*>(ws + c * lds + (long)PASSES * 32 * IPT * blockIdx.y);
+  uint4* dst = reinterpret_cast<uint4*>(ws + c * lds + (long)PASSES * 32 * IPT * by);
 #pragma unroll
   for (int ii = 0; ii < IPT; ii++) {
-    const int i = blockIdx.y * IPT + ii;
+    const int i = by * IPT + ii;
     // the 24 couplings of the instruction's 8 groups, all loads first
     float v[8][3];
 #pragma unroll
@@ -126,6 +126,30 @@ __global__ __launch_bounds__(256) void k_convert_ws(const float* __restrict__ x2
       for (int u = 0; u < 4; u++) d[4 + u] = make_uint4(lo[4 * u], lo[4 * u + 1], lo[4 * u + 2], lo[4 * u + 3]);
     }
   }
+}
+
+// blocks of one conversion: x-layout column tiles of 256 by instruction rows
+__host__ __device__ __forceinline__ int convert_ws_tiles_c(int L) { return (CCM_NA * L + 255) / 256; }
+__host__ __device__ __forceinline__ int convert_ws_rows(int Lp, int passes) {
+  return SP_GROUPS * Lp / 8 / (passes == 1 ? 2 : 1);
+}
+
+template <int PASSES>
+__global__ __launch_bounds__(256) void k_convert_ws(const float* __restrict__ x2, int L, int Lp,
+                                                    uint16_t* __restrict__ ws, long lds) {
+  convert_ws_block<PASSES>(x2, L, Lp, ws, lds, blockIdx.x, blockIdx.y);
+}
+
+// every alignment of a group (x buffers of a batch are whole alignments)
+template <int PASSES>
+__global__ __launch_bounds__(256) void k_convert_ws_grouped(const ccm_bprob* __restrict__ probs,
+                                                            const uint32_t* __restrict__ offs,
+                                                            const int32_t* __restrict__ pidx, int np) {
+  const int gi = ccm_group_find(offs, np, blockIdx.x);
+  const ccm_bprob& B = probs[pidx[gi]];
+  if (!B.ctl->eval_on) return;
+  const int t = (int)(blockIdx.x - offs[gi]), tc = convert_ws_tiles_c(B.L);
+  convert_ws_block<PASSES>(B.x + B.nsp, B.L, B.Lp, B.wt, B.ldwt, t % tc, t / tc);
 }
 
 // One workgroup: TM = WM*RB*32 sequences x TN = WN*CB*32 gathered rows of Ws
@@ -317,6 +341,27 @@ __global__ __launch_bounds__(64 * WM * WN, 1) void k_sparse_fwd(
 
 constexpr int SF_TM = 256, SF_TN = 256;
 
+// The sparse fused forward over a group of alignments (whole alignments as
+// one chunk each, x1 = x), tiles per alignment ceil(L / 12) x ceil(ncols /
+// 256) in the single-alignment order, workgroups through the group's tile
+// map when it has one
+__global__ __launch_bounds__(512, 1) void k_sparse_fwd_grouped(const ccm_bprob* __restrict__ probs,
+                                                               const uint32_t* __restrict__ offs,
+                                                               const int32_t* __restrict__ pidx, int np,
+                                                               uint32_t total, const uint32_t* __restrict__ qmap) {
+  const uint32_t qid = qmap ? qmap[blockIdx.x] : ccm_xcd_remap(blockIdx.x, total);
+  if (qid >= offs[np]) return;
+  const int gi = ccm_group_find(offs, np, qid);
+  const ccm_bprob& B = probs[pidx[gi]];
+  if (!B.ctl->eval_on) return;
+  const uint32_t t = qid - offs[gi];
+  const uint32_t tiles_n = (uint32_t)((B.ncols + SF_TM - 1) / SF_TM);
+  const ccm_fwd_epi epi{B.x, B.msapt, B.ldmt, B.w, B.L, B.N, B.dt, B.lddt, (long)B.ncols, B.passes, 1, 0,
+                        B.fpart, B.g1part};
+  sparse_fwd_tile<2, 4, 4, 2>(B.wt, B.ldwt, B.Lp, B.passes_fwd * SP_GROUPS * B.Lp / 16, B.sidx, B.ldsidx,
+                              B.ncols, (int)(t / tiles_n), (int)(t % tiles_n), epi);
+}
+
 }  // namespace
 
 extern "C" {
@@ -353,7 +398,7 @@ int ccm_launch_convert_ws(const float* d_x, int L, int Lp, uint16_t* d_ws, long 
       lds % 8)
     return CCM_ERR_ARG;
   // 7 Lp / 8 instructions (a multiple of 56), 2 per thread row with one pass
-  const dim3 grid((CCM_NA * L + 255) / 256, SP_GROUPS * Lp / 8 / (passes == 1 ? 2 : 1));
+  const dim3 grid(convert_ws_tiles_c(L), convert_ws_rows(Lp, passes));
   if (passes == 1)
     hipLaunchKernelGGL(k_convert_ws<1>, grid, dim3(256), 0, s, d_x + ccm_nsp(L), L, Lp, d_ws, lds);
   else
@@ -387,6 +432,36 @@ int ccm_launch_sparse_fwd_softmax(const uint16_t* d_ws, long lds, int Lp, int pa
   if (ldidx < (long)tiles_n * SF_TM) return CCM_ERR_ARG;
   hipLaunchKernelGGL((k_sparse_fwd<2, 4, 4, 2>), dim3(total), dim3(512), 0, s, d_ws, lds, Lp, nsteps, d_idx,
                      ldidx, ncols, tiles_n, tiles, total, epi);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+// grouped forms (batched alignments; descriptors in ccmpred_amd_batch.h)
+int ccm_convert_ws_blocks(int L, int Lp, int passes) { return convert_ws_tiles_c(L) * convert_ws_rows(Lp, passes); }
+
+uint32_t ccm_sparse_fwd_tile_rows(int L) { return L < 1 ? 0u : (uint32_t)((L + SF_TN / CCM_NA - 1) / (SF_TN / CCM_NA)); }
+
+uint32_t ccm_sparse_fwd_tile_count(int L, int ncols) {
+  return ncols < 1 ? 0u : ccm_sparse_fwd_tile_rows(L) * (uint32_t)((ncols + SF_TM - 1) / SF_TM);
+}
+
+int ccm_launch_convert_ws_grouped(const ccm_bprob* d_probs, ccm_group g, int passes, ccm_stream_t stream) {
+  if (g.np <= 0 || g.total == 0 || passes < 1 || passes > 2) return CCM_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  if (passes == 1)
+    hipLaunchKernelGGL(k_convert_ws_grouped<1>, dim3(g.total), dim3(256), 0, s, d_probs, g.offs, g.pidx, g.np);
+  else
+    hipLaunchKernelGGL(k_convert_ws_grouped<2>, dim3(g.total), dim3(256), 0, s, d_probs, g.offs, g.pidx, g.np);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_sparse_fwd_softmax_grouped(const ccm_bprob* d_probs, ccm_group g, ccm_stream_t stream) {
+  if (g.np <= 0 || g.total == 0) return CCM_ERR_ARG;
+  // with a tile map the grid is the map; else padded to whole XCD rounds
+  const uint32_t total = g.qmap ? g.total : (g.total + 7u) & ~7u;
+  hipLaunchKernelGGL(k_sparse_fwd_grouped, dim3(total), dim3(512), 0, (hipStream_t)stream, d_probs, g.offs,
+                     g.pidx, g.np, total, g.qmap);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }

@@ -61,7 +61,9 @@ typedef struct ccm_bprob {
 	ccm_cg_ctl *ctl;
 	double *log;     /* per accepted iteration: f, step, line-search trials, <g,g> */
 	int32_t log_cap; /* iterations the log holds */
-	int32_t pad;
+	int32_t sparse;  /* wt holds the sparse forward's layout; sidx its positions */
+	const uint32_t *sidx;
+	int64_t ldsidx;
 } ccm_bprob;
 
 /* A grouped launch: problems pidx[0..np) get consecutive block ranges

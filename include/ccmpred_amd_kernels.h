@@ -105,6 +105,13 @@ int ccm_launch_convert_ws(const float *d_x, int L, int Lp, uint16_t *d_ws, long 
                           ccm_stream_t s);
 int ccm_launch_sparse_index(const uint8_t *d_msap, int Lp, int Np, int passes, uint32_t *d_idx,
                             ccm_stream_t s);
+/* grouped forms over a batch: one alignment per group entry, descriptors
+ * with sparse set (wt = Ws, sidx / ldsidx the positions) */
+int ccm_convert_ws_blocks(int L, int Lp, int passes);
+uint32_t ccm_sparse_fwd_tile_rows(int L);
+uint32_t ccm_sparse_fwd_tile_count(int L, int ncols);
+int ccm_launch_convert_ws_grouped(const ccm_bprob *d_probs, ccm_group g, int passes, ccm_stream_t s);
+int ccm_launch_sparse_fwd_softmax_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
 int ccm_launch_sparse_fwd_softmax(const uint16_t *d_ws, long lds, int Lp, int passes_fwd,
                                   const uint32_t *d_idx, long ldidx, int ncols, const float *d_x1,
                                   const uint8_t *d_msapt, long ldmt, const float *d_w, int L, int Nc,

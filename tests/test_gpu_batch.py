@@ -1,6 +1,7 @@
 """Batched form against the single-alignment path.
 
 The grouped kernels run the device code of the single-alignment launchers
+(the sparse forward and, with sparse=False, the dense one)
 and the device-side CG control makes the host loop's decisions in the same
 FP64 arithmetic, so every alignment of a batch must reproduce the
 single-alignment BF16 results bit for bit: objective and gradient of one
@@ -57,10 +58,11 @@ def make_set(shapes):
 
 
 @pytest.mark.parametrize("prec", ["bf16", "bf16x2"])
-def test_batch_evaluate_bitwise(prec):
+@pytest.mark.parametrize("sparse", [True, False])
+def test_batch_evaluate_bitwise(prec, sparse):
     msas, xs = make_set(SHAPES)
-    say(f"batch evaluate {prec}: shapes {[(m.shape[1], m.shape[0]) for m in msas]}")
-    with Batch(msas, precision=prec) as b:
+    say(f"batch evaluate {prec} sparse={sparse}: shapes {[(m.shape[1], m.shape[0]) for m in msas]}")
+    with Batch(msas, precision=prec, sparse=sparse) as b:
         for i, x in enumerate(xs):
             b.set_x(i, x)
         f = b.evaluate()
@@ -75,7 +77,7 @@ def test_batch_evaluate_bitwise(prec):
         gb3 = [b.get_g(i) for i in range(b.n)]
         b.set_variants(*nv)
         for i, (m, x) in enumerate(zip(msas, xs)):
-            with P.PLL(m, precision=prec, sparse=False) as p:
+            with P.PLL(m, precision=prec, sparse=sparse) as p:
                 assert np.array_equal(p.weights, b.weights[i])
                 fs, gs = p.evaluate(torch.from_numpy(x).cuda())
                 gs = gs.cpu().numpy()
@@ -87,7 +89,8 @@ def test_batch_evaluate_bitwise(prec):
             assert np.array_equal(gb3[i], gs)
 
 
-def test_batch_solve_matches_single():
+@pytest.mark.parametrize("sparse", [True, False])
+def test_batch_solve_matches_single(sparse):
     shapes = SHAPES[:4]
     msas, xs = make_set(shapes)
     # values in the a >= 21 padding of one start point: the batched vector
@@ -96,8 +99,8 @@ def test_batch_solve_matches_single():
     pad = xs[0][nsingle_padded(L0):].reshape(NA, L0, NAP, L0)
     pad[:, :, NA:, :] = np.random.default_rng(7).normal(0, 1, (NA, L0, NAP - NA, L0)).astype(np.float32)
     its = 12
-    say(f"batch CG {its} iterations: shapes {[(m.shape[1], m.shape[0]) for m in msas]}")
-    with Batch(msas, precision="bf16") as b:
+    say(f"batch CG {its} iterations sparse={sparse}: shapes {[(m.shape[1], m.shape[0]) for m in msas]}")
+    with Batch(msas, precision="bf16", sparse=sparse) as b:
         for i, x in enumerate(xs):
             b.set_x(i, x)
         rounds = b.solve(max_iterations=its, epsilon=0.0, rounds_per_graph=3)
@@ -111,7 +114,7 @@ def test_batch_solve_matches_single():
                                       P.contact_map(xb[i], L, apc=apc, normalize=norm)), (i, apc, norm)
     say(f"  rounds={rounds}")
     for i, (m, x) in enumerate(zip(msas, xs)):
-        with P.PLL(m, precision="bf16", sparse=False) as p:
+        with P.PLL(m, precision="bf16", sparse=sparse) as p:
             xd = torch.from_numpy(x).cuda()
             r = p.conjugrad(xd, max_iterations=its, epsilon=0.0)
             xs_ = xd.cpu().numpy()
@@ -143,7 +146,7 @@ def test_batch_autotune_then_solve():
         res = [b.result(i) for i in range(b.n)]
         xb = [b.get_x(i) for i in range(b.n)]
     for i, (m, x) in enumerate(zip(msas, xs)):
-        with P.PLL(m, precision="bf16", sparse=False) as p:
+        with P.PLL(m, precision="bf16") as p:
             xd = torch.from_numpy(x).cuda()
             r = p.conjugrad(xd, max_iterations=its, epsilon=0.0)
             xs_ = xd.cpu().numpy()

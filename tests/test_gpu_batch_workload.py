@@ -114,7 +114,7 @@ def test_batched_workload_64():
         # the same solves one alignment at a time: bit for bit
         same = 0
         for i, m in enumerate(msas):
-            with P.PLL(m, precision=prec, sparse=False) as p:
+            with P.PLL(m, precision=prec) as p:
                 xd = torch.from_numpy(P.init_bias(m)).cuda()
                 r = p.conjugrad(xd, max_iterations=50, epsilon=0.0)
                 xs = xd.cpu().numpy()
