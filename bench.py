@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 L_COLS, N_SEQS, SEED, N_PAIRS = 250, 5000, 2, 60
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+MFMA_BF16_SPARSE_PEAK_TFLOPS = 5000.0  # 2:4 structured-sparse BF16 MFMA
 HBM_PEAK_GBS = 8000.0
 
 
@@ -43,7 +44,7 @@ def measured_counters(kernel_prefix):
         return None, None
     for ln in lines:
         cells = [c.strip() for c in ln.strip().strip("|").split("|")]
-        if len(cells) >= 5 and cells[0].strip("`").startswith(kernel_prefix):
+        if len(cells) >= 5 and kernel_prefix in cells[0].strip("`"):
             busy = float(cells[4]) if cells[4] else None
             return float(cells[3]), busy
     return None, None
@@ -300,6 +301,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--warmup-seconds", type=float, default=1.0,
                     help="continue the warm-up steps until this much time has passed")
+    ap.add_argument("--forms", default="bf16,bf16f,fp32",
+                    help="GPU forms to time beside the headline bf16 (comma list of bf16f, fp32)")
     ap.add_argument("--cpu-evals", type=int, default=3)
     ap.add_argument("--cpu-serial-evals", type=int, default=1,
                     help="single-thread evaluate_cpu restatement calls to time (0: none)")
@@ -343,24 +346,43 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    forms = set(args.forms.split(","))
     bf16 = time_gpu(msa, x, "bf16", args.steps, args.warmup, args.warmup_seconds)
-    bf16f = time_gpu(msa, x, "bf16f", args.steps, args.warmup, args.warmup_seconds)
-    fp32 = time_gpu(msa, x, "fp32", max(10, args.steps // 4), max(2, args.warmup // 2),
-                    args.warmup_seconds)
+    bf16f = (time_gpu(msa, x, "bf16f", args.steps, args.warmup, args.warmup_seconds)
+             if "bf16f" in forms else None)
+    fp32 = (time_gpu(msa, x, "fp32", max(10, args.steps // 4), max(2, args.warmup // 2),
+                     args.warmup_seconds) if "fp32" in forms else None)
     cpu, serial = (None, None) if args.no_cpu else cpu_legs(args)
 
     L, N = L_COLS, N_SEQS
     flops = 2.0 * N * (21 * L) ** 2          # algorithmic FLOPs of one contraction
     # the slower of the two contraction kernels (the forward one carries the
     # column softmax as its epilogue when fused)
-    fwd_key = "gemm_fwd_softmax" if "gemm_fwd_softmax" in bf16["step_ms"] else "gemm_fwd"
+    fwd_key = next(k for k in ("sparse_fwd_softmax", "gemm_fwd_softmax", "gemm_fwd") if k in bf16["step_ms"])
     gemm = max((fwd_key, "gemm_grad"), key=lambda k: bf16["step_ms"][k])
     gemm_ms = bf16["step_ms"][gemm]
     achieved = flops / (gemm_ms * 1e-3) / 1e12
-    mode = 1 if gemm == "gemm_grad" else (2 if gemm == "gemm_fwd_softmax" else 0)
-    traffic, mfma_busy = measured_counters("void k_gemm_onehot_dtl<%d," % mode)
-    if traffic is None:
-        traffic, mfma_busy = measured_counters("void k_gemm_onehot<%d," % mode)
+    prefix = {"sparse_fwd_softmax": "k_sparse_fwd<", "gemm_fwd_softmax": "void k_gemm_onehot_dtl<2,",
+              "gemm_fwd": "void k_gemm_onehot_dtl<0,", "gemm_grad": "void k_gemm_onehot_dtl<1,"}[gemm]
+    traffic, mfma_busy = measured_counters(prefix)
+    fwd_roofline = None
+    if fwd_key == "sparse_fwd_softmax":
+        # the forward on structured-sparse MFMA: the dense algorithmic FLOPs
+        # against the dense peak, and the instruction work it executes (7
+        # groups of 4 K slots per coupling column, Lp = 256 columns, 5120
+        # sequence rows) against the sparse peak
+        fms = bf16["step_ms"][fwd_key]
+        executed = 2.0 * 5120 * (21 * 256) * (28 * 256)
+        f_traffic, f_busy = measured_counters("k_sparse_fwd<")
+        fwd_roofline = {"kernel": fwd_key, "avg_ms": round(fms, 4), "algorithmic_flops": flops,
+                        "achieved_dense_equivalent": round(flops / (fms * 1e-3) / 1e12, 2),
+                        "peak_dense": MFMA_BF16_PEAK_TFLOPS,
+                        "frac_dense": round(flops / (fms * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS, 4),
+                        "executed_flops": executed,
+                        "achieved_executed": round(executed / (fms * 1e-3) / 1e12, 2),
+                        "peak_sparse": MFMA_BF16_SPARSE_PEAK_TFLOPS,
+                        "frac_sparse": round(executed / (fms * 1e-3) / 1e12 / MFMA_BF16_SPARSE_PEAK_TFLOPS, 4),
+                        "unit": "TFLOP/s", "traffic": f_traffic, "mfma_busy": f_busy}
     line = {
         "metric": "PLL f+grad evals/s at L=250,N=5000: BF16-MFMA vs FP32 GPU vs host CPU",
         "value": round(bf16["evals_per_s"], 2),
@@ -383,16 +405,20 @@ def main():
                      "algorithmic_flops": flops, "avg_ms": round(gemm_ms, 4),
                      "mfma_busy": mfma_busy,
                      "counters": os.path.relpath(COUNTERS_FILE, ROOT) if traffic else None},
+        "roofline_forward": fwd_roofline,
         "warmup_steps_run": bf16["warmup_steps"],
-        "bf16f_evals_per_s": round(bf16f["evals_per_s"], 2),
-        "bf16f_step_ms": {k: round(v, 4) for k, v in bf16f["step_ms"].items()},
-        "fp32_evals_per_s": round(fp32["evals_per_s"], 2),
-        "bf16_over_fp32": round(bf16["evals_per_s"] / fp32["evals_per_s"], 3),
-        "fp32_roofline": fp32_roofline(fp32["step_ms"], L, N),
         "bf16_step_ms": {k: round(v, 4) for k, v in bf16["step_ms"].items()},
-        "fp32_step_ms": {k: round(v, 4) for k, v in fp32["step_ms"].items()},
-        "f_bf16": bf16["f"], "f_fp32": fp32["f"],
+        "f_bf16": bf16["f"],
     }
+    if bf16f is not None:
+        line["bf16f_evals_per_s"] = round(bf16f["evals_per_s"], 2)
+        line["bf16f_step_ms"] = {k: round(v, 4) for k, v in bf16f["step_ms"].items()}
+    if fp32 is not None:
+        line["fp32_evals_per_s"] = round(fp32["evals_per_s"], 2)
+        line["bf16_over_fp32"] = round(bf16["evals_per_s"] / fp32["evals_per_s"], 3)
+        line["fp32_roofline"] = fp32_roofline(fp32["step_ms"], L, N)
+        line["fp32_step_ms"] = {k: round(v, 4) for k, v in fp32["step_ms"].items()}
+        line["f_fp32"] = fp32["f"]
     if cpu is not None:
         line["cpu_baseline"] = {"value": round(cpu["evals_per_s"], 4), "unit": "evals/s",
                                 "cores": cpu["threads"], "kind": "port",

@@ -704,11 +704,15 @@ int ccm_hip_step_names(void *instance, const char **names, int n)
 	static const char *fp32[] = {"gather_fwd", "softmax_rows", "gather_grad", "assemble"};
 	static const char *bf16[] = {"convert_wt", "gemm_fwd", "softmax_cols", "gemm_grad", "assemble"};
 	static const char *fused[] = {"convert_wt", "gemm_fwd_softmax", "gemm_grad", "assemble"};
+	static const char *sparse[] = {"convert_ws", "sparse_fwd_softmax", "gemm_grad", "assemble"};
 	userdata *ud = (userdata *)instance;
 	ccm_hip_state *st = (ccm_hip_state *)ud->extra;
 	if (!st)
 		return -1;
-	const char **v = st->cfg.precision == CCM_PRECISION_FP32 ? fp32 : st->fused ? fused : bf16;
+	const char **v = st->cfg.precision == CCM_PRECISION_FP32 ? fp32
+	                 : st->sparse                             ? sparse
+	                 : st->fused                              ? fused
+	                                                          : bf16;
 	const int cnt = st->cfg.precision == CCM_PRECISION_FP32 ? 4 : st->fused ? 4 : 5;
 	for (int i = 0; i < n && i < cnt; i++)
 		names[i] = v[i];

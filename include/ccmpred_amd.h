@@ -126,7 +126,8 @@ void *ccm_hip_stream(void *instance);
  * returns the number of timed evaluations.  Steps, FP32 path: gather_fwd,
  * softmax, gather_grad, assembly (pair epilogue, singles and the objective's
  * sum); BF16 path: convert_wt, gemm_fwd (with the softmax when fused),
- * softmax (unfused only), gemm_grad, assembly. */
+ * softmax (unfused only), gemm_grad, assembly; with the sparse forward
+ * convert_ws, sparse_fwd_softmax, gemm_grad, assemble. */
 int ccm_hip_set_timing(void *instance, int on);
 int ccm_hip_get_timing(void *instance, double *avg_ms, int n);
 /* Names of those steps (static strings; up to n filled); returns their
