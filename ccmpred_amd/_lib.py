@@ -43,7 +43,7 @@ class HipConfig(C.Structure):
 class CGParam(C.Structure):
     _fields_ = [("max_iterations", C.c_int), ("epsilon", C.c_double), ("k", C.c_int),
                 ("max_linesearch", C.c_int), ("alpha_mul", C.c_double), ("ftol", C.c_double),
-                ("wolfe", C.c_double)]
+                ("wolfe", C.c_double), ("line_search", C.c_int)]
 
 
 EVALUATE_T = C.CFUNCTYPE(C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int)

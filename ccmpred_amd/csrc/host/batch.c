@@ -49,6 +49,7 @@ struct ccm_batch {
 	int n;
 	int fused; /* forward contractions with the softmax fused (every alignment fits) */
 	int sparse; /* and on structured-sparse MFMA (every alignment fits) */
+	int borrowed; /* one alignment whose instance and x belong to the caller */
 	ccm_hip_config cfg;
 	hipStream_t s;
 	userdata *uds;
@@ -473,13 +474,13 @@ void ccm_batch_destroy(ccm_batch *b)
 		(void)hipStreamSynchronize(b->s);
 	for (int i = 0; i < b->n; i++) {
 		ccm_bprob *p = &b->h_probs[i];
-		float *v[] = {p->x, p->g, p->gprev, p->s};
+		float *v[] = {b->borrowed ? NULL : p->x, p->g, p->gprev, p->s};
 		for (int k = 0; k < 4; k++)
 			if (v[k])
 				(void)hipFree(v[k]);
 		if (p->dpart)
 			(void)hipFree(p->dpart);
-		if (b->uds[i].extra)
+		if (!b->borrowed && b->uds[i].extra)
 			destroy_hip(&b->uds[i]);
 	}
 	if (b->d_probs)
@@ -502,6 +503,95 @@ void ccm_batch_destroy(ccm_batch *b)
 	free(b->var_fwd);
 	free(b->var_grad);
 	free(b);
+}
+
+/* The CG of ccm_conjugrad_gpu_ex on the device (CCM_LINE_SEARCH_DEVICE):
+ * returns 0 with the results, 1 when the instance does not qualify or the
+ * set-up failed (nothing has run), -1 when the solve failed. */
+/* A batch of one alignment over an existing evaluate_hip instance (BF16,
+ * unchunked) and the caller's x (16-byte aligned, nvar_padded floats): the
+ * batched form's device-side CG then drives that instance's kernels.  The
+ * batch owns only its work vectors and tables; destroying it leaves the
+ * instance and x alone. */
+static int batch_borrow(userdata *ud, float *d_x, ccm_batch **out)
+{
+	*out = NULL;
+	const ccm_hip_state *st = (const ccm_hip_state *)ud->extra;
+	if (!st || st->cfg.precision != CCM_PRECISION_BF16 || st->chunk < st->N ||
+	    ((uintptr_t)d_x & 15u) != 0)
+		return 1;
+	ccm_batch *b = (ccm_batch *)calloc(1, sizeof(*b));
+	if (!b)
+		return 1;
+	b->n = 1;
+	b->borrowed = 1;
+	b->cfg = st->cfg;
+	b->uds = ud;
+	b->fused = st->fused;
+	b->sparse = st->sparse;
+	b->h_probs = (ccm_bprob *)calloc(1, sizeof(ccm_bprob));
+	b->var_fwd = (int *)malloc(sizeof(int));
+	b->var_grad = (int *)malloc(sizeof(int));
+	if (!b->h_probs || !b->var_fwd || !b->var_grad)
+		goto fail;
+	HIPCHK(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
+	HIPCHK(balloc(b, (void **)&b->d_ctl, sizeof(ccm_cg_ctl)));
+	HIPCHK(hipMemset(b->d_ctl, 0, sizeof(ccm_cg_ctl)));
+	ccm_bprob *p = &b->h_probs[0];
+	bprob_from_state(p, st);
+	p->x = d_x;
+	const size_t vb = sizeof(float) * (size_t)p->nvp;
+	HIPCHK(balloc(b, (void **)&p->g, vb));
+	HIPCHK(balloc(b, (void **)&p->gprev, vb));
+	HIPCHK(balloc(b, (void **)&p->s, vb));
+	HIPCHK(hipMemset(p->g, 0, vb));
+	HIPCHK(hipMemset(p->gprev, 0, vb));
+	HIPCHK(hipMemset(p->s, 0, vb));
+	HIPCHK(balloc(b, (void **)&p->dpart, sizeof(double) * 3 * (size_t)p->n_dots));
+	p->ctl = b->d_ctl;
+	b->var_fwd[0] = st->cfg.gemm_variant >= 0 ? st->cfg.gemm_variant
+	                                          : ccm_gemm_default_variant_for(0, p->Lp, p->ncols);
+	b->var_grad[0] = st->cfg.gemm_variant >= 0 ? st->cfg.gemm_variant
+	                                           : ccm_gemm_default_variant_for(1, p->Lp, p->ncols);
+	HIPCHK(balloc(b, (void **)&b->d_probs, sizeof(ccm_bprob)));
+	HIPCHK(hipMemcpy(b->d_probs, b->h_probs, sizeof(ccm_bprob), hipMemcpyHostToDevice));
+	HIPCHK(balloc(b, (void **)&b->d_nactive, sizeof(int32_t)));
+	HIPCHK(hipHostMalloc((void **)&b->h_nactive, sizeof(int32_t), hipHostMallocDefault));
+	if (build_tables(b))
+		goto fail;
+	*out = b;
+	return 0;
+fail:
+	ccm_batch_destroy(b);
+	return 1;
+}
+
+int ccm_conjugrad_instance_device(void *instance, float *d_x, const ccm_conjugrad_parameter *param,
+                                  double *fx, int *iterations, int *evaluations, double *fhist,
+                                  int *status)
+{
+	ccm_batch *b = NULL;
+	double *log = NULL;
+	int it = 0, rc = -1;
+	if (param->max_iterations <= 0 || param->max_linesearch <= 0)
+		return 1;
+	log = (double *)malloc(sizeof(double) * 4 * (size_t)param->max_iterations);
+	if (!log || batch_borrow((userdata *)instance, d_x, &b)) {
+		free(log);
+		return 1; /* nothing has run: x is unchanged */
+	}
+	if (ccm_batch_solve(b, param, 8, NULL) || ccm_batch_result(b, 0, status, fx, &it, evaluations, log))
+		goto out;
+	if (iterations)
+		*iterations = it;
+	if (fhist)
+		for (int i = 0; i < it && i < param->max_iterations; i++)
+			fhist[i] = log[4L * i];
+	rc = 0;
+out:
+	free(log);
+	ccm_batch_destroy(b);
+	return rc;
 }
 
 int ccm_batch_create(userdata *uds, int nprob, const ccm_hip_config *cfg_in, ccm_batch **out)

@@ -42,6 +42,8 @@
 #include "ccmpred_amd.h"
 #include "ccmpred_amd_kernels.h"
 
+#include "state.h"
+
 void ccm_conjugrad_init(ccm_conjugrad_parameter *p)
 {
 	p->max_iterations = 250;
@@ -51,7 +53,9 @@ void ccm_conjugrad_init(ccm_conjugrad_parameter *p)
 	p->alpha_mul = 0.5;
 	p->ftol = 1e-4;
 	p->wolfe = 0.2;
+	p->line_search = CCM_LINE_SEARCH_DEVICE;
 }
+
 
 /* Sufficient decrease (see above).  fres is the relative resolution below
  * which differences of f are not trusted: the BF16 forms quantise the
@@ -150,6 +154,16 @@ int ccm_conjugrad_gpu_ex(int n, conjugrad_float_t *d_x, double *fx, conjugrad_ev
                          const ccm_conjugrad_parameter *p, int *iterations, int *evaluations,
                          double *fhist)
 {
+	if (p->line_search == CCM_LINE_SEARCH_DEVICE && !progress && evaluate == evaluate_hip && instance &&
+	    ((const userdata *)instance)->ncol > 0 &&
+	    (long)ccm_nvar_padded(((const userdata *)instance)->ncol) == n) {
+		int status = CONJUGRADERR_DEVICE;
+		const int r = ccm_conjugrad_instance_device(instance, d_x, p, fx, iterations, evaluations,
+		                                            fhist, &status);
+		if (r <= 0)
+			return r == 0 ? status : CONJUGRADERR_DEVICE;
+		/* an instance the device-side CG does not take: the host loop */
+	}
 	cg_ctx c = {0};
 	float *g = NULL, *gprev = NULL, *sdir = NULL;
 	double *hist = NULL;

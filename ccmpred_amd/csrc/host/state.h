@@ -70,4 +70,11 @@ typedef struct ccm_hip_state {
  * the instance's buffers) */
 int ccm_init_hip_state(void *instance, const ccm_hip_config *cfg, int allow_fused, int allow_sparse);
 
+/* batch.c: the batched form's device-side CG on one evaluate_hip instance
+ * and the caller's x (ccm_conjugrad_gpu_ex with CCM_LINE_SEARCH_DEVICE):
+ * 0 done, 1 the instance does not qualify (nothing ran), -1 failed */
+int ccm_conjugrad_instance_device(void *instance, float *d_x, const ccm_conjugrad_parameter *param,
+                                  double *fx, int *iterations, int *evaluations, double *fhist,
+                                  int *status);
+
 #endif

@@ -203,13 +203,19 @@ class PLL:
         return n, {names[i].decode(): out[i] for i in range(cnt)}
 
     def conjugrad(self, x, max_iterations=50, epsilon=0.0, k=5, max_linesearch=5, alpha_mul=0.5,
-                  ftol=1e-4, wolfe=0.2):
-        """Device CG from x (modified in place); returns status and counters."""
+                  ftol=1e-4, wolfe=0.2, line_search="device"):
+        """Device CG from x (modified in place); returns status and counters.
+        line_search "device": decisions on the device where the instance
+        allows it (CCM_LINE_SEARCH_DEVICE), "host": after every evaluation
+        on the host; both give the same results bit for bit."""
         import torch
+        if line_search not in ("device", "host"):
+            raise ValueError("line_search must be 'device' or 'host'")
         p = _lib.CGParam()
         _lib.lib().ccm_conjugrad_init(C.byref(p))
         p.max_iterations, p.epsilon, p.k = int(max_iterations), float(epsilon), int(k)
         p.max_linesearch, p.alpha_mul, p.ftol, p.wolfe = int(max_linesearch), alpha_mul, ftol, wolfe
+        p.line_search = 0 if line_search == "device" else 1
         torch.cuda.current_stream(x.device).synchronize()
         fx = C.c_double()
         it = C.c_int()

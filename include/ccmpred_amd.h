@@ -153,12 +153,25 @@ typedef struct ccm_conjugrad_parameter {
 	double alpha_mul;
 	double ftol;
 	double wolfe;
+	int line_search; /* CCM_LINE_SEARCH_*: where the line search runs (no counterpart) */
 } ccm_conjugrad_parameter;
+
+/* CCM_LINE_SEARCH_DEVICE (the default): when the callback is evaluate_hip
+ * on a BF16, unchunked instance, no progress callback is given and d_x is
+ * 16-byte aligned, the whole CG -- vector steps, evaluations, dot products,
+ * line-search and direction decisions -- runs on the device as the batched
+ * form's graph-captured state machine on that instance, with one host
+ * synchronisation per 8 rounds (a round: one evaluation); the results are
+ * the host loop's bit for bit.  Otherwise, and with CCM_LINE_SEARCH_HOST,
+ * the host makes the decisions after each evaluation. */
+#define CCM_LINE_SEARCH_DEVICE 0
+#define CCM_LINE_SEARCH_HOST 1
 
 /* ccmpred.c:526-534 defaults */
 void ccm_conjugrad_init(ccm_conjugrad_parameter *param);
 
-/* Nonlinear CG on device vectors (conjugrad_gpu at ccmpred.c:632).
+/* Nonlinear CG on device vectors (conjugrad_gpu at ccmpred.c:632); see
+ * CCM_LINE_SEARCH_* for where its decisions are made.
  * Polak-Ribiere+ direction, backtracking line search with an Armijo test
  * (ftol) and a curvature test (wolfe), FP64 scalars and dot products.
  * d_x: device pointer, n floats; *fx receives the final objective. */

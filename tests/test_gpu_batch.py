@@ -116,7 +116,7 @@ def test_batch_solve_matches_single(sparse):
     for i, (m, x) in enumerate(zip(msas, xs)):
         with P.PLL(m, precision="bf16", sparse=sparse) as p:
             xd = torch.from_numpy(x).cuda()
-            r = p.conjugrad(xd, max_iterations=its, epsilon=0.0)
+            r = p.conjugrad(xd, max_iterations=its, epsilon=0.0, line_search="host")
             xs_ = xd.cpu().numpy()
         rb = res[i]
         say(f"  [{i}] single ret={r['ret']} it={r['iterations']} ev={r['evaluations']} f={r['fx']:.10g} | "
@@ -148,7 +148,7 @@ def test_batch_autotune_then_solve():
     for i, (m, x) in enumerate(zip(msas, xs)):
         with P.PLL(m, precision="bf16") as p:
             xd = torch.from_numpy(x).cuda()
-            r = p.conjugrad(xd, max_iterations=its, epsilon=0.0)
+            r = p.conjugrad(xd, max_iterations=its, epsilon=0.0, line_search="host")
             xs_ = xd.cpu().numpy()
         say(f"  [{i}] it={res[i]['iterations']}/{r['iterations']} f={res[i]['fx']:.10g}/{r['fx']:.10g}")
         assert res[i]["fx"] == r["fx"] and res[i]["evaluations"] == r["evaluations"]

@@ -6,7 +6,8 @@ with autotuned tiles and graph-captured rounds.
 Checked:
   * every alignment runs all 50 iterations;
   * every alignment's x, f and counters equal the single-alignment device
-    CG's (ccm_conjugrad_gpu_ex over evaluate_hip) bit for bit;
+    CG's (ccm_conjugrad_gpu_ex over evaluate_hip, line search on the host)
+    bit for bit;
   * f and g of one grouped evaluation against the FP64 oracle
     (evaluate_cpu.c:16-166 restated) for the three smallest alignments: at
     init_bias + symmetric N(0, 0.01^2) couplings before the solve (f 1e-5,
@@ -116,7 +117,7 @@ def test_batched_workload_64():
         for i, m in enumerate(msas):
             with P.PLL(m, precision=prec) as p:
                 xd = torch.from_numpy(P.init_bias(m)).cuda()
-                r = p.conjugrad(xd, max_iterations=50, epsilon=0.0)
+                r = p.conjugrad(xd, max_iterations=50, epsilon=0.0, line_search="host")
                 xs = xd.cpu().numpy()
             ok = (r["fx"] == res[i]["fx"] and r["iterations"] == res[i]["iterations"]
                   and r["evaluations"] == res[i]["evaluations"] and r["ret"] == res[i]["ret"]
