@@ -1,152 +1,21 @@
-// Gradient assembly and the objective's global sum.
-//
-// Pairs (evaluate_cpu.c:131-163): for every padded pair entry (b,k,a,j)
-//   g2 = M(b,k,a,j) + M(a,j,b,k) [+ hsym(b,k,a,j)] for a < 21 and k != j,
-//   g2 = 0 for self-edges (k == j) and for padding (a >= 21),
-//   then g2 += 2 lambda_pair x and reg += 0.5 lambda_pair x^2 over the whole
-//   padded block.  M is the raw (unsymmetrised) pair gradient in either the
-//   FP32 layout (Lm = L, M(b,k,a,j) = M[b*L + k][a*L + j]) or the padded
-//   BF16 layout (Lm = Lp, M(b,k,a,j) = M[a*Lp + j][b*Lp + k]); only the sum
-//   of both orientations enters g2, so either works.
-// Singles (evaluate_cpu.c:102-109,153-156): g1 = sum of per-block partials
-//   + 2 lambda_single V for the 20L real singles, 0 on the padding.
-#include "common.h"
+// Gradient assembly and the objective's global sum: the assembly kernels
+// over the work items of assemble.h (evaluate_cpu.c:131-163 pairs,
+// evaluate_cpu.c:102-109,153-156 singles).
+#include "assemble.h"
 
-#define EP_T 64
-// threads of an assembly block (pair tiles and singles): 8 waves, so that a
-// thread holds 8 rows of a tile and more waves fit on a SIMD
-#define EP_NTH 512
-#define EP_W (EP_NTH / 64)
-
-// ---- symmetric form: one block per pair of mirrored tiles ------------------
-//
-// In the x2 index space (rows R = b*L + k, columns C = a*L + j, both below
-// 21L) g2[R][C] = M[phi(C)][phi(R)] + M[phi(R)][phi(C)] + hsym[R][C] + 2 l x[R][C],
-// phi(a*L + j) = a*Lm + j, which is symmetric in (R, C) apart from hsym and
-// x.  A block takes the tile pair (I, J), I <= J, of 64 x 64 tiles: it
-// stages M[phi(I)][phi(J)] + hsym[I][J] and M[phi(J)][phi(I)] in LDS once
-// and writes both output tiles, so M is read once and, hsym being
-// symmetric, half of hsym.  Blocks past the tile pairs write the padding
-// columns C in [64 NT, 32L) (2 l x only).
-__device__ __forceinline__ int ccm_phi(int X, int L, int Lm) {
-  const int a = X / L;
-  return a * Lm + (X - a * L);
-}
-
-// tile pairs (ti <= tj) of NT tiles, row by row
-__device__ __forceinline__ void ccm_tri_pair(uint32_t idx, int NT, int& ti, int& tj) {
-  // ti: largest t with t*NT - t*(t-1)/2 <= idx
-  const double n = (double)NT + 0.5;
-  int t = (int)(n - sqrt(n * n - 2.0 * (double)idx));
-  if (t < 0) t = 0;
-  if (t > NT - 1) t = NT - 1;
-  while (t > 0 && (uint32_t)(t * NT - t * (t - 1) / 2) > idx) t--;
-  while (t + 1 < NT && (uint32_t)((t + 1) * NT - (t + 1) * t / 2) <= idx) t++;
-  ti = t;
-  tj = t + (int)(idx - (uint32_t)(t * NT - t * (t - 1) / 2));
-}
-
-__device__ __forceinline__ int ccm_pair_tiles(int L) { return (CCM_NA * L + EP_T - 1) / EP_T; }
-
+// one pair item per workgroup
 __device__ __forceinline__ void pair_sym_block(const float* __restrict__ M, long ldm, int Lm,
                                                const float* __restrict__ hsym,
                                                const float* __restrict__ x2, float* __restrict__ g2,
                                                int L, float lam2, double* __restrict__ regpart,
                                                uint32_t blk) {
-  constexpr int RPT = EP_T / EP_W;  // rows per thread: rows wv, wv + EP_W, ...
-  __shared__ float D1[EP_T][EP_T + 1];
-  __shared__ float D2[EP_T][EP_T + 1];
+  __shared__ float D1s[EP_TILE_FLOATS];
+  __shared__ float D2s[EP_TILE_FLOATS];
   __shared__ double red[EP_W];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int C21 = CCM_NA * L, ld2 = CCM_NAP * L;
-  const int NT = ccm_pair_tiles(L);
-  const uint32_t npairs = (uint32_t)(NT * (NT + 1) / 2);
-  const float two_l = 2.0f * lam2, half_l = 0.5f * lam2;
-  double reg = 0.0;
-  // one output tile: rows R0 + rr, column C; S(rr) from the staged blocks.
-  // All loads of the tile are issued before any result is formed.
-  auto out_tile = [&](int R0, int C, bool pair_col, int jC, bool first) {
-    float xv[RPT];
-#pragma unroll
-    for (int i = 0; i < RPT; i++) {
-      const int R = R0 + wv + EP_W * i;
-      xv[i] = R < C21 ? x2[(long)R * ld2 + C] : 0.f;
-    }
-#pragma unroll
-    for (int i = 0; i < RPT; i++) {
-      const int rr = wv + EP_W * i, R = R0 + rr;
-      if (R >= C21) continue;
-      const int kR = R - (R / L) * L;
-      float v = 0.f;
-      if (pair_col && kR != jC)
-        v = first ? D1[rr][lane] + D2[lane][rr] : D2[rr][lane] + D1[lane][rr];
-      g2[(long)R * ld2 + C] = v + two_l * xv[i];
-      reg += (double)(half_l * xv[i] * xv[i]);
-    }
-  };
-  if (blk < npairs) {
-    int ti, tj;
-    ccm_tri_pair(blk, NT, ti, tj);
-    // D1[r][c] = M[phi(ti*64 + r)][phi(tj*64 + c)] + hsym[ti*64 + r][tj*64 + c],
-    // D2[r][c] = M[phi(tj*64 + r)][phi(ti*64 + c)] (hsym split when ti == tj)
-    {
-      const int Ci = ti * EP_T + lane, Cj = tj * EP_T + lane;
-      const int pci = Ci < C21 ? ccm_phi(Ci, L, Lm) : 0, pcj = Cj < C21 ? ccm_phi(Cj, L, Lm) : 0;
-      float d1[RPT], d2[RPT], h1[RPT];
-#pragma unroll
-      for (int i = 0; i < RPT; i++) {
-        const int rr = wv + EP_W * i;
-        const int Ri = ti * EP_T + rr, Rj = tj * EP_T + rr;
-        const bool ok1 = Ri < C21 && Cj < C21;
-        d1[i] = ok1 ? M[(long)ccm_phi(Ri, L, Lm) * ldm + pcj] : 0.f;
-        h1[i] = (ok1 && hsym) ? hsym[(long)Ri * ld2 + Cj] : 0.f;
-        d2[i] = (Rj < C21 && Ci < C21) ? M[(long)ccm_phi(Rj, L, Lm) * ldm + pci] : 0.f;
-      }
-      // a diagonal pair (one tile, mirrored onto itself) gives half of hsym
-      // to each staged block, so that both orientations add the same two
-      // operands and g2 stays exactly symmetric
-      const float h1s = ti == tj ? 0.5f : 1.0f, h2s = ti == tj ? 0.5f : 0.0f;
-#pragma unroll
-      for (int i = 0; i < RPT; i++) {
-        D1[wv + EP_W * i][lane] = d1[i] + h1s * h1[i];
-        D2[wv + EP_W * i][lane] = d2[i] + h2s * h1[i];
-      }
-    }
-    __syncthreads();
-    // tile (ti, tj): rows ti*64 + rr, column tj*64 + lane
-    {
-      const int C = tj * EP_T + lane;
-      if (C < ld2) out_tile(ti * EP_T, C, C < C21, C - (C / L) * L, true);
-    }
-    // mirrored tile (tj, ti): rows tj*64 + rr, column ti*64 + lane (below C21)
-    if (ti != tj) {
-      const int C = ti * EP_T + lane;
-      out_tile(tj * EP_T, C, true, C - (C / L) * L, false);
-    }
-  } else {
-    // padding columns past the tiles: row tile tr, columns 64 NT + 64 tc + lane
-    const int npc = (ld2 - NT * EP_T + EP_T - 1) / EP_T;
-    const uint32_t q = blk - npairs;
-    const int tr = (int)(q / (uint32_t)npc), tc = (int)(q % (uint32_t)npc);
-    const int C = NT * EP_T + tc * EP_T + lane;
-    if (C < ld2) out_tile(tr * EP_T, C, false, 0, true);
-  }
-  reg = ccm_wave_sum_f64(reg);
-  if (lane == 0) red[wv] = reg;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-#pragma unroll
-    for (int w = 0; w < EP_W; w++) t += red[w];
-    regpart[blk] = t;
-  }
-}
-
-// blocks of the symmetric pair assembly: tile pairs, then padding tiles
-__host__ __device__ __forceinline__ int ccm_pair_sym_blocks(int L) {
-  const int NT = (CCM_NA * L + EP_T - 1) / EP_T;
-  const int npc = (CCM_NAP * L - NT * EP_T + EP_T - 1) / EP_T;
-  return NT * (NT + 1) / 2 + NT * (npc > 0 ? npc : 0);
+  const uint32_t blks[1] = {blk};
+  float* const d1[1] = {D1s};
+  float* const d2[1] = {D2s};
+  pair_sym_items<1>(M, ldm, Lm, hsym, x2, g2, L, lam2, regpart, blks, d1, d2, red);
 }
 
 __global__ __launch_bounds__(EP_NTH) void k_pair_epilogue(const float* __restrict__ M, long ldm, int Lm,
@@ -204,37 +73,7 @@ __device__ __forceinline__ void singles_block(const double* __restrict__ g1part,
                                               int L, long nsp, float lam1, double* __restrict__ regpart,
                                               uint32_t blk) {
   __shared__ double red[EP_W];
-  const long v = (long)blk * EP_NTH + threadIdx.x;
-  const long ns = 20L * L;
-  double reg = 0.0;
-  if (v < ns) {
-    // blocks summed in order; eight loads in flight at a time
-    double s = 0.0;
-    int b = 0;
-    for (; b + 8 <= nblk; b += 8) {
-      double t[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++) t[u] = g1part[(long)(b + u) * ns + v];
-#pragma unroll
-      for (int u = 0; u < 8; u++) s += t[u];
-    }
-    for (; b < nblk; b++) s += g1part[(long)b * ns + v];
-    const float xv = x1[v];
-    g1[v] = (float)s + (2.0f * lam1) * xv;
-    reg = (double)(lam1 * xv * xv);
-  } else if (v < nsp) {
-    g1[v] = 0.f;
-  }
-  reg = ccm_wave_sum_f64(reg);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) red[wv] = reg;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-#pragma unroll
-    for (int w = 0; w < EP_W; w++) t += red[w];
-    regpart[blk] = t;
-  }
+  singles_item(g1part, nblk, x1, g1, L, nsp, lam1, regpart, blk, red);
 }
 
 __global__ __launch_bounds__(EP_NTH) void k_singles_finalize(const double* __restrict__ g1part, int nblk,
