@@ -81,58 +81,102 @@ __global__ __launch_bounds__(256) void k_sparse_index(const uint8_t* __restrict_
 // couplings -> Ws (bf16): row c = a*Lp + j, K slot of column k, group t,
 // position p >= 1 holds W(3t + p - 1, k, a, j) (zero for k >= L); each
 // instruction's 32 slots are followed by their lo halves when PASSES = 2.
-// A thread takes one x-layout column C = a*L + j (consecutive lanes,
-// consecutive C: every load of a wave reads 256 contiguous bytes) and the
-// 128 bytes of Ws row c that hold 128 / (64 PASSES) instructions, so that
-// each row segment it writes is one whole 128-byte line.  Rows of padding
-// columns j >= L are not written (zeroed once).
+// Ws row c is column C = a*L + j of the x layout, so the conversion is a
+// transpose, done by tiles through LDS: a workgroup reads the CW x-layout
+// columns C0.. of the 168 rows (b, k) of CWS_CK = 8 coupling columns k
+// with 16-byte loads (CW/4 lanes per row segment), stages them as BF16
+// (hi, and lo for two passes) in LDS with row = C and the 168 values of a
+// row in group order, and writes each Ws row's 56 groups (7 instructions)
+// as contiguous 16-byte stores.  Rows of padding columns j >= L are not
+// written (zeroed once).
+constexpr int CWS_CK = 8;                   // coupling columns per tile
+constexpr int CWS_R = CCM_NA * CWS_CK;      // staged values per Ws row (56 groups of 3)
+constexpr int CWS_LD = CWS_R + 2;           // uint16 per LDS row: 340 bytes, 4-byte aligned
+template <int PASSES>
+struct cws_shape {
+  static constexpr int CW = PASSES == 1 ? 128 : 64;  // x-layout columns (Ws rows) per tile
+};
+
 template <int PASSES>
 __device__ __forceinline__ void convert_ws_block(const float* __restrict__ x2, int L, int Lp,
                                                  uint16_t* __restrict__ ws, long lds, int bx, int by) {
-  constexpr int IPT = 2 / PASSES;  // instructions per thread
-  const int C = bx * 256 + threadIdx.x;
-  if (C >= CCM_NA * L) return;
+  constexpr int CW = cws_shape<PASSES>::CW;
+  constexpr int F4 = CW / 4;                  // 16-byte loads per row segment
+  constexpr int NE = CWS_R * F4;              // loads of the tile
+  constexpr int PER = (NE + 255) / 256;       // per thread
+  constexpr int Q = 4 * SP_GROUPS;            // 16-byte stores per Ws row and pass: 56 groups
+  __shared__ __attribute__((aligned(16))) uint16_t sh[PASSES][CW * CWS_LD];
+  const int C21 = CCM_NA * L;
   const long ld2 = (long)CCM_NAP * L;
-  const int a = C / L;
-  const long c = (long)a * Lp + (C - a * L);
-  uint4* dst = reinterpret_cast<uint4*>(ws + c * lds + (long)PASSES * 32 * IPT * by);
+  const int C0 = bx * CW, k0 = by * CWS_CK;
+  const bool al = ((uintptr_t)x2 & 15u) == 0;
+  // all loads first: element e = staged row (b, kk) x 16-byte column f
+  float4 v[PER];
 #pragma unroll
-  for (int ii = 0; ii < IPT; ii++) {
-    const int i = by * IPT + ii;
-    // the 24 couplings of the instruction's 8 groups, all loads first
-    float v[8][3];
-#pragma unroll
-    for (int g = 0; g < 8; g++) {
-      const int G = 8 * i + g, k = G / SP_GROUPS, t = G - k * SP_GROUPS;
-#pragma unroll
-      for (int p = 0; p < 3; p++)
-        v[g][p] = k < L ? x2[(long)((3 * t + p) * L + k) * ld2 + C] : 0.f;
+  for (int u = 0; u < PER; u++) {
+    const int e = (int)threadIdx.x + 256 * u;
+    v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (e < NE) {
+      const int row = e / F4, f = e - row * F4;
+      const int b = row / CWS_CK, kk = row - b * CWS_CK;
+      const int k = k0 + kk, C = C0 + 4 * f;
+      if (k < L && C < C21) {
+        // (the row is 32 L floats long: C + 3 stays inside it)
+        const float* src = x2 + (long)(b * L + k) * ld2 + C;
+        v[u] = al ? *reinterpret_cast<const float4*>(src) : make_float4(src[0], src[1], src[2], src[3]);
+      }
     }
-    uint32_t hi[16], lo[16];
+  }
 #pragma unroll
-    for (int g = 0; g < 8; g++) {
-      // slots 4g .. 4g+3: zero, then the group's three residues
-      hi[2 * g] = ccm_f2bf2(0.f, v[g][0]);
-      hi[2 * g + 1] = ccm_f2bf2(v[g][1], v[g][2]);
-      lo[2 * g] = ccm_f2bf2(0.f, v[g][0] - ccm_bf2f((uint16_t)(hi[2 * g] >> 16)));
-      lo[2 * g + 1] = ccm_f2bf2(v[g][1] - ccm_bf2f((uint16_t)hi[2 * g + 1]),
-                                v[g][2] - ccm_bf2f((uint16_t)(hi[2 * g + 1] >> 16)));
+  for (int u = 0; u < PER; u++) {
+    const int e = (int)threadIdx.x + 256 * u;
+    if (e < NE) {
+      const int row = e / F4, f = e - row * F4;
+      const int b = row / CWS_CK, kk = row - b * CWS_CK;
+      const int o = 4 * f * CWS_LD + kk * CCM_NA + b;
+      const uint32_t h01 = ccm_f2bf2(v[u].x, v[u].y), h23 = ccm_f2bf2(v[u].z, v[u].w);
+      sh[0][o] = (uint16_t)h01;
+      sh[0][o + CWS_LD] = (uint16_t)(h01 >> 16);
+      sh[0][o + 2 * CWS_LD] = (uint16_t)h23;
+      sh[0][o + 3 * CWS_LD] = (uint16_t)(h23 >> 16);
+      if constexpr (PASSES > 1) {
+        const uint32_t l01 = ccm_f2bf2(v[u].x - ccm_bf2f((uint16_t)h01), v[u].y - ccm_bf2f((uint16_t)(h01 >> 16)));
+        const uint32_t l23 = ccm_f2bf2(v[u].z - ccm_bf2f((uint16_t)h23), v[u].w - ccm_bf2f((uint16_t)(h23 >> 16)));
+        sh[1][o] = (uint16_t)l01;
+        sh[1][o + CWS_LD] = (uint16_t)(l01 >> 16);
+        sh[1][o + 2 * CWS_LD] = (uint16_t)l23;
+        sh[1][o + 3 * CWS_LD] = (uint16_t)(l23 >> 16);
+      }
     }
-    uint4* d = dst + ii * 4 * PASSES;
+  }
+  __syncthreads();
+  // store q of a Ws row: groups 2q, 2q+1, i.e. staged values 6q .. 6q+5,
+  // as the slots (0, v0, v1, v2, 0, v3, v4, v5); consecutive lanes take
+  // consecutive q, so that a row's stores are contiguous
+  for (int e = (int)threadIdx.x; e < CW * Q; e += 256) {
+    const int cl = e / Q, q = e - cl * Q, C = C0 + cl;
+    if (C >= C21) continue;
+    const int a = C / L;
+    uint16_t* row = ws + ((long)a * Lp + (C - a * L)) * lds;
 #pragma unroll
-    for (int u = 0; u < 4; u++) d[u] = make_uint4(hi[4 * u], hi[4 * u + 1], hi[4 * u + 2], hi[4 * u + 3]);
-    if constexpr (PASSES > 1) {
-#pragma unroll
-      for (int u = 0; u < 4; u++) d[4 + u] = make_uint4(lo[4 * u], lo[4 * u + 1], lo[4 * u + 2], lo[4 * u + 3]);
+    for (int p = 0; p < PASSES; p++) {
+      const uint32_t* w = reinterpret_cast<const uint32_t*>(&sh[p][cl * CWS_LD + 6 * q]);
+      const uint32_t w0 = w[0], w1 = w[1], w2 = w[2];
+      const uint4 o = make_uint4(w0 << 16, (w0 >> 16) | (w1 << 16), w1 & 0xFFFF0000u, w2);
+      // instruction 7 (k0 / 8) + q / 4 of the row, its 16-byte quarter q % 4
+      const long slot = PASSES == 1 ? 28L * k0 + 8 * q
+                                    : 64L * (7 * (k0 / CWS_CK) + q / 4) + 8 * (q % 4) + 32 * p;
+      *reinterpret_cast<uint4*>(row + slot) = o;
     }
   }
 }
 
-// blocks of one conversion: x-layout column tiles of 256 by instruction rows
-__host__ __device__ __forceinline__ int convert_ws_tiles_c(int L) { return (CCM_NA * L + 255) / 256; }
-__host__ __device__ __forceinline__ int convert_ws_rows(int Lp, int passes) {
-  return SP_GROUPS * Lp / 8 / (passes == 1 ? 2 : 1);
+// blocks of one conversion: x-layout column tiles by coupling-column tiles
+__host__ __device__ __forceinline__ int convert_ws_tiles_c(int L, int passes) {
+  const int cw = passes == 1 ? cws_shape<1>::CW : cws_shape<2>::CW;
+  return (CCM_NA * L + cw - 1) / cw;
 }
+__host__ __device__ __forceinline__ int convert_ws_rows(int Lp) { return Lp / CWS_CK; }
 
 template <int PASSES>
 __global__ __launch_bounds__(256) void k_convert_ws(const float* __restrict__ x2, int L, int Lp,
@@ -148,7 +192,7 @@ __global__ __launch_bounds__(256) void k_convert_ws_grouped(const ccm_bprob* __r
   const int gi = ccm_group_find(offs, np, blockIdx.x);
   const ccm_bprob& B = probs[pidx[gi]];
   if (!B.ctl->eval_on) return;
-  const int t = (int)(blockIdx.x - offs[gi]), tc = convert_ws_tiles_c(B.L);
+  const int t = (int)(blockIdx.x - offs[gi]), tc = convert_ws_tiles_c(B.L, PASSES);
   convert_ws_block<PASSES>(B.x + B.nsp, B.L, B.Lp, B.wt, B.ldwt, t % tc, t / tc);
 }
 
@@ -411,8 +455,7 @@ int ccm_launch_convert_ws(const float* d_x, int L, int Lp, uint16_t* d_ws, long 
   if (Lp % 64 || Lp < L || L < 1 || passes < 1 || passes > 2 || lds < ccm_sparse_ws_len(Lp, passes) ||
       lds % 8)
     return CCM_ERR_ARG;
-  // 7 Lp / 8 instructions (a multiple of 56), 2 per thread row with one pass
-  const dim3 grid(convert_ws_tiles_c(L), convert_ws_rows(Lp, passes));
+  const dim3 grid(convert_ws_tiles_c(L, passes), convert_ws_rows(Lp));
   if (passes == 1)
     hipLaunchKernelGGL(k_convert_ws<1>, grid, dim3(256), 0, s, d_x + ccm_nsp(L), L, Lp, d_ws, lds);
   else
@@ -451,7 +494,7 @@ int ccm_launch_sparse_fwd_softmax(const uint16_t* d_ws, long lds, int Lp, int pa
 }
 
 // grouped forms (batched alignments; descriptors in ccmpred_amd_batch.h)
-int ccm_convert_ws_blocks(int L, int Lp, int passes) { return convert_ws_tiles_c(L) * convert_ws_rows(Lp, passes); }
+int ccm_convert_ws_blocks(int L, int Lp, int passes) { return convert_ws_tiles_c(L, passes) * convert_ws_rows(Lp); }
 
 uint32_t ccm_sparse_fwd_tile_rows(int L) { return L < 1 ? 0u : (uint32_t)((L + SF_TN / CCM_NA - 1) / (SF_TN / CCM_NA)); }
 

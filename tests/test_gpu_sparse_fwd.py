@@ -93,3 +93,68 @@ def test_sparse_forward_chunked(chunk):
         g1 = g1.cpu().numpy()
     print(f"chunk {chunk}: f {abs(f1 - f0) / abs(f0):.2e} g {rel_l2(g1, g0):.2e}", flush=True)
     assert abs(f1 - f0) / abs(f0) <= 1e-9 and rel_l2(g1, g0) <= 1e-6
+
+
+def bf16_bits(v):
+    """FP32 -> BF16 bits, round to nearest even (finite values)"""
+    u = np.ascontiguousarray(v, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def ws_expected(x, L, Lp, passes, lds, fill):
+    """The sparse forward's coupling operand from x in numpy: row c = a Lp + j
+    (x-layout column C = a L + j), group G = 7 k + t of column k holds the
+    slots (0, W(3t, k), W(3t+1, k), W(3t+2, k)); with two passes each
+    instruction's 32 slots (8 groups) are followed by their lo halves."""
+    nsp = nsingle_padded(L)
+    x2 = x[nsp:].reshape(NA * L, NAP * L)
+    ws = np.full((NA * Lp, lds), fill, np.uint16)
+    G = np.arange(7 * Lp)
+    k, t = G // 7, G % 7
+    for a in range(NA):
+        for j in range(L):
+            C = a * L + j
+            vals = np.zeros((7 * Lp, 4), np.float32)
+            for pp in range(1, 4):
+                ok = k < L
+                vals[ok, pp] = x2[(3 * t[ok] + pp - 1) * L + k[ok], C]
+            hi = bf16_bits(vals)
+            if passes == 1:
+                ws[a * Lp + j, :28 * Lp] = hi.reshape(-1)
+            else:
+                lo = bf16_bits(vals - (hi.astype(np.uint32) << 16).view(np.float32))
+                inst = np.concatenate([hi.reshape(-1, 32), lo.reshape(-1, 32)], axis=1)
+                ws[a * Lp + j, :56 * Lp] = inst.reshape(-1)
+    return ws
+
+
+@pytest.mark.parametrize("passes", [1, 2])
+@pytest.mark.parametrize("L,offset", [(1, 0), (5, 0), (37, 1), (64, 0), (130, 0), (250, 0), (250, 1)])
+def test_convert_ws_layout(passes, L, offset):
+    """ccm_launch_convert_ws against the layout built in numpy, bit for bit:
+    L not a multiple of 64 and of the tile widths, x not 16-byte aligned
+    (offset 1 float: the scalar loads), rows of padding columns j >= L and
+    slots past the operand left alone."""
+    import ctypes as C
+    from ccmpred_amd import _lib
+    lib = _lib.lib()
+    vp = C.c_void_p
+    lib.ccm_launch_convert_ws.argtypes = [vp, C.c_int, C.c_int, vp, C.c_long, C.c_int, vp]
+    lib.ccm_launch_convert_ws.restype = C.c_int
+    print(f"convert_ws passes={passes} L={L} offset={offset}", flush=True)
+    Lp = (L + 63) // 64 * 64
+    nvp = nsingle_padded(L) + L * L * NA * NAP
+    rng = np.random.default_rng(L * 10 + passes + offset)
+    x = rng.normal(0, 1, nvp).astype(np.float32) * np.float32(3.0) ** rng.integers(-8, 8, nvp).astype(np.float32)
+    lds = 28 * Lp * passes + 8
+    fill = 0xABCD
+    xd = torch.zeros(nvp + offset, dtype=torch.float32, device="cuda")
+    xd[offset:] = torch.from_numpy(x).cuda()
+    wsd = torch.full((NA * Lp * lds,), fill, dtype=torch.int32, device="cuda").to(torch.int16)
+    st = vp(torch.cuda.current_stream().cuda_stream)
+    assert lib.ccm_launch_convert_ws(xd[offset:].data_ptr(), L, Lp, wsd.data_ptr(), lds, passes, st) == 0
+    torch.cuda.synchronize()
+    got = wsd.cpu().numpy().view(np.uint16).reshape(NA * Lp, lds)
+    want = ws_expected(x, L, Lp, passes, lds, fill)
+    bad = np.argwhere(got != want)
+    assert bad.size == 0, (bad[:5], got[tuple(bad[0])], want[tuple(bad[0])])
