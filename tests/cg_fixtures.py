@@ -25,6 +25,14 @@ def msa_sha256(msa):
     return hashlib.sha256(np.ascontiguousarray(msa, dtype=np.uint8).tobytes()).hexdigest()
 
 
+def has(name):
+    """whether the fixture of case name has been written"""
+    if not os.path.exists(JSON):
+        return False
+    with open(JSON) as fh:
+        return name in json.load(fh) and os.path.exists(os.path.join(GOLDEN, "cg_maps", f"{name}.npy"))
+
+
 def load(name):
     with open(JSON) as fh:
         meta = json.load(fh)[name]

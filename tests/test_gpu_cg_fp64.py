@@ -64,6 +64,8 @@ def case_msa(name):
 @pytest.mark.parametrize("prec", ["fp32", "bf16f", "bf16"])
 @pytest.mark.parametrize("name", list(CASES))
 def test_cg_against_fp64(name, prec):
+    if not F.has(name):
+        pytest.skip(f"no FP64 fixture for {name} (tests/golden/make_cg_maps.py writes it)")
     meta, m64 = F.load(name)
     msa = case_msa(name)
     assert F.msa_sha256(msa) == meta["msa_sha256"]
