@@ -117,6 +117,28 @@ def time_gpu(msa, x_host, precision, steps, warmup, warmup_seconds):
                 timed_calls=ncalls, step_ms=steps_ms, warmup_steps=done)
 
 
+def time_cg(msa, precision, iterations):
+    """The device CG of config C2: `iterations` iterations (epsilon 0) from
+    init_bias with the reference's other parameters, the line search on the
+    device; the second of two runs is timed (the first warms the clocks)."""
+    import torch
+    from ccmpred_amd.pll import PLL, init_bias
+    x0 = init_bias(msa)
+    with PLL(msa, precision=precision) as p:
+        out = None
+        for _ in range(2):
+            xd = torch.from_numpy(x0).cuda()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = p.conjugrad(xd, max_iterations=iterations, epsilon=0.0)
+            torch.cuda.synchronize()
+            out = dict(precision=precision, seconds=round(time.perf_counter() - t0, 4), ret=int(r["ret"]),
+                       iterations=int(r["iterations"]), evaluations=int(r["evaluations"]),
+                       f=float(r["fx"]))
+    out["evaluations_per_s"] = round(out["evaluations"] / out["seconds"], 1)
+    return out
+
+
 def physical_cores():
     """Physical cores among the CPUs this process may run on (one per
     thread_siblings group), and the number of those CPUs."""
@@ -303,6 +325,8 @@ def main():
                     help="continue the warm-up steps until this much time has passed")
     ap.add_argument("--forms", default="bf16,bf16f,fp32",
                     help="GPU forms to time beside the headline bf16 (comma list of bf16f, fp32)")
+    ap.add_argument("--cg-iterations", type=int, default=50,
+                    help="device CG iterations timed at L=250, N=5000 for the bf16 form (0: none)")
     ap.add_argument("--cpu-evals", type=int, default=3)
     ap.add_argument("--cpu-serial-evals", type=int, default=1,
                     help="single-thread evaluate_cpu restatement calls to time (0: none)")
@@ -352,6 +376,7 @@ def main():
              if "bf16f" in forms else None)
     fp32 = (time_gpu(msa, x, "fp32", max(10, args.steps // 4), max(2, args.warmup // 2),
                      args.warmup_seconds) if "fp32" in forms else None)
+    cg = time_cg(msa, "bf16", args.cg_iterations) if args.cg_iterations > 0 else None
     cpu, serial = (None, None) if args.no_cpu else cpu_legs(args)
 
     L, N = L_COLS, N_SEQS
@@ -410,6 +435,8 @@ def main():
         "bf16_step_ms": {k: round(v, 4) for k, v in bf16["step_ms"].items()},
         "f_bf16": bf16["f"],
     }
+    if cg is not None:
+        line["cg_bf16"] = cg
     if bf16f is not None:
         line["bf16f_evals_per_s"] = round(bf16f["evals_per_s"], 2)
         line["bf16f_step_ms"] = {k: round(v, 4) for k, v in bf16f["step_ms"].items()}
