@@ -88,6 +88,8 @@ EXPORTED = [
     "ccm_launch_pad_scan_x", "ccm_launch_dots_padded", "ccm_launch_axpy_padded",
     "ccm_launch_direction_padded", "ccm_launch_negate_padded",
     "ccm_assemble_blocks", "ccm_launch_assemble", "ccm_launch_assemble_grouped",
+    "ccm_convert_ws_reg_parts", "ccm_launch_convert_ws_reg", "ccm_assemble_fs_blocks",
+    "ccm_launch_assemble_fs", "ccm_launch_sparse_fwd_softmax_pad",
 ]
 
 

@@ -110,7 +110,8 @@ static uint32_t blocks_of(const ccm_bprob *p, int kind)
 		return (uint32_t)(p->sparse ? ccm_convert_ws_blocks(p->L, p->Lp, p->passes_fwd)
 		                            : ccm_convert_wt_blocks(p->L, p->Lp));
 	case K_SOFT: return (uint32_t)ccm_softmax_cols_blocks(p->L, p->N);
-	case K_ASM: return (uint32_t)ccm_assemble_blocks(p->L);
+	case K_ASM:
+		return (uint32_t)(p->sparse ? ccm_assemble_fs_blocks(p->L) : ccm_assemble_blocks(p->L));
 	case K_ALL: return 1;
 	case K_DOTS: return (uint32_t)p->n_dots;
 	default: return (uint32_t)ccm_cg_vec_blocks(p->nvp);
@@ -371,8 +372,10 @@ static void bprob_from_state(ccm_bprob *p, const ccm_hip_state *st)
 	p->passes = st->cfg.bf16_passes;
 	p->n_fpart = st->n_fpart;
 	p->n_g1blk = st->n_g1blk;
-	p->n_regpair = ccm_pair_epilogue_blocks(st->L);
-	p->n_regsingle = st->n_regsingle;
+	/* the sparse form's conversion writes the regularisation partials */
+	p->n_regpair = st->sparse ? ccm_convert_ws_reg_parts(st->L, st->Lp, st->cfg.bf16_fwd_passes)
+	                          : ccm_pair_epilogue_blocks(st->L);
+	p->n_regsingle = st->sparse ? 1 : st->n_regsingle;
 	p->n_dots = ccm_dots_blocks(st->nvp);
 	p->nsp = st->nsp;
 	p->nvp = st->nvp;

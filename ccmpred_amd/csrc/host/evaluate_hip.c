@@ -274,6 +274,10 @@ int ccm_init_hip_state(void *instance, const ccm_hip_config *cfg_in, int allow_f
 	st->n_regpair = ccm_pair_epilogue_blocks(L);
 	if (st->n_regpair < ccm_reg_only_blocks())
 		st->n_regpair = ccm_reg_only_blocks();
+	/* the sparse form's conversion leaves its own partials (either pass count) */
+	for (int pf = 1; pf <= 2; pf++)
+		if (st->n_regpair < ccm_convert_ws_reg_parts(L, st->Lp, pf))
+			st->n_regpair = ccm_convert_ws_reg_parts(L, st->Lp, pf);
 	st->n_regsingle = ccm_singles_blocks(L);
 	HIPCHK(dalloc(st, (void **)&st->d_regpair, sizeof(double) * (size_t)st->n_regpair));
 	HIPCHK(dalloc(st, (void **)&st->d_regsingle, sizeof(double) * (size_t)st->n_regsingle));
@@ -464,8 +468,10 @@ static int convert_couplings(ccm_hip_state *st, const float *d_x, hipStream_t s)
 
 /* Logits, softmax and (RUN_FULL) pair-gradient accumulation over all
  * sequence chunks.  *step advances through the timing marks: one mark
- * before each of the three per-chunk kernels of the first chunk. */
-static int run_chunks(ccm_hip_state *st, const float *d_x, int what, int *step)
+ * before each of the three per-chunk kernels of the first chunk.  d_g_pad
+ * (sparse form): the first chunk's forward also runs the coupling
+ * conversion's padding tiles, writing g's a >= 21 padding. */
+static int run_chunks(ccm_hip_state *st, const float *d_x, int what, int *step, float *d_g_pad)
 {
 	hipStream_t s = st->stream;
 	const int L = st->L, N = st->N, Lp = st->Lp, Np = st->Np, P = st->P;
@@ -493,10 +499,12 @@ static int run_chunks(ccm_hip_state *st, const float *d_x, int what, int *step)
 			if (first)
 				MARK((*step)++);
 			if (st->sparse) {
-				KCHK(ccm_launch_sparse_fwd_softmax(
+				const int pad = first && d_g_pad;
+				KCHK(ccm_launch_sparse_fwd_softmax_pad(
 				    st->d_wt, st->ldwt, Lp, st->cfg.bf16_fwd_passes, st->d_sidx + n0, st->ldsidx, ncols,
 				    d_x, st->d_msapt + n0, (long)Np, st->d_w + n0, L, nc, st->d_dt, (long)passes * ncols,
-				    (long)ncols, passes, mode, n0 / 128, st->d_fpart, st->d_g1part, s));
+				    (long)ncols, passes, mode, n0 / 128, st->d_fpart, st->d_g1part, pad ? d_x : NULL,
+				    pad ? d_g_pad : NULL, st->lam2, pad ? st->d_regpair : NULL, s));
 			} else if (st->fused) {
 				KCHK(ccm_launch_gemm_fwd_softmax(st->d_wt, st->ldwt, Lp, st->cfg.bf16_fwd_passes,
 				                                 msap, ncols, d_x, st->d_msapt + n0, (long)Np,
@@ -545,12 +553,22 @@ fail:
 
 /* g from the raw pair gradient and the node-gradient partials, and the
  * objective (softmax partials + regularisation) into *f_out (device
- * address) */
-static int assemble_gradient(ccm_hip_state *st, const float *d_x, float *d_g, double *f_out)
+ * address).  reg_done: the coupling conversion of this evaluation
+ * (ccm_launch_convert_ws_reg) wrote the regularisation partials and the
+ * a >= 21 padding of g, and the assembly forms f in its own launch. */
+static int assemble_gradient(ccm_hip_state *st, const float *d_x, float *d_g, double *f_out, int reg_done)
 {
 	hipStream_t s = st->stream;
 	const int L = st->L;
 	const int fp32 = st->cfg.precision == CCM_PRECISION_FP32;
+	if (reg_done) {
+		KCHK(ccm_launch_assemble_fs(st->d_gt, (long)st->P, st->Lp, st->d_hsym, d_x, d_g, L, st->lam1,
+		                            st->lam2, st->d_g1part, st->n_g1blk, st->d_fpart, st->n_fpart,
+		                            st->d_regpair,
+		                            ccm_convert_ws_reg_parts(L, st->Lp, st->cfg.bf16_fwd_passes),
+		                            st->d_regsingle, 1, f_out, s));
+		return 0;
+	}
 	KCHK(ccm_launch_assemble(fp32 ? st->d_graw : st->d_gt, fp32 ? st->ldg : (long)st->P,
 	                         fp32 ? L : st->Lp, fp32 ? NULL : st->d_hsym, d_x, d_g, L, st->lam1,
 	                         st->lam2, st->d_g1part, st->n_g1blk, st->d_fpart, st->n_fpart,
@@ -581,16 +599,24 @@ int ccm_hip_evaluate(void *instance, const float *d_x, float *d_g, int nvar, dou
 	hipStream_t s = st->stream;
 	int step = 0;
 	st->forward_valid = 0;
+	/* the sparse form's conversion also writes the regularisation partials
+	 * and g's a >= 21 padding */
+	const int reg_done = st->cfg.precision != CCM_PRECISION_FP32 && st->sparse;
 	if (st->cfg.precision != CCM_PRECISION_FP32) {
 		MARK(step++);
-		KCHK(convert_couplings(st, d_x, s));
+		if (reg_done)
+			KCHK(ccm_launch_convert_ws_reg(d_x, d_g, st->L, st->Lp, st->d_wt, st->ldwt,
+			                               st->cfg.bf16_fwd_passes, st->lam1, st->lam2, st->d_regpair,
+			                               st->d_regsingle, 0, s));
+		else
+			KCHK(convert_couplings(st, d_x, s));
 	}
-	if (run_chunks(st, d_x, RUN_FULL, &step))
+	if (run_chunks(st, d_x, RUN_FULL, &step, reg_done ? d_g : NULL))
 		goto fail;
 	MARK(step++);
 	/* the objective's last kernel writes f into mapped host memory: no
 	 * device-to-host copy before the synchronisation */
-	if (assemble_gradient(st, d_x, d_g, st->dh_f))
+	if (assemble_gradient(st, d_x, d_g, st->dh_f, reg_done))
 		goto fail;
 	MARK(step);
 	HIPCHK(hipStreamSynchronize(s));
@@ -619,7 +645,7 @@ int ccm_hip_objective(void *instance, const float *d_x, int nvar, double *f)
 	st->forward_valid = 0;
 	if (st->cfg.precision != CCM_PRECISION_FP32)
 		KCHK(convert_couplings(st, d_x, s));
-	if (run_chunks(st, d_x, RUN_OBJECTIVE, &step))
+	if (run_chunks(st, d_x, RUN_OBJECTIVE, &step, NULL))
 		goto fail;
 	KCHK(ccm_launch_reg_only(d_x, st->L, st->lam1, st->lam2, st->d_regpair, s));
 	if (read_f(st, ccm_reg_only_blocks(), 0, f))
@@ -646,7 +672,7 @@ int ccm_hip_forward(void *instance, const float *d_x, int nvar, double *f)
 	st->forward_valid = 0;
 	if (st->cfg.precision != CCM_PRECISION_FP32)
 		KCHK(convert_couplings(st, d_x, s));
-	if (run_chunks(st, d_x, RUN_FORWARD, &step))
+	if (run_chunks(st, d_x, RUN_FORWARD, &step, NULL))
 		goto fail;
 	KCHK(ccm_launch_reg_only(d_x, st->L, st->lam1, st->lam2, st->d_regpair, s));
 	if (read_f(st, ccm_reg_only_blocks(), 0, f))
@@ -668,7 +694,7 @@ int ccm_hip_backward(void *instance, const float *d_x, float *d_g, int nvar)
 		fprintf(stderr, "ccmpred_amd: ccm_hip_backward without a preceding ccm_hip_forward\n");
 		return -1;
 	}
-	if (pair_gradient_from_forward(st) || assemble_gradient(st, d_x, d_g, st->d_f))
+	if (pair_gradient_from_forward(st) || assemble_gradient(st, d_x, d_g, st->d_f, 0))
 		return -1;
 	if (hipStreamSynchronize(st->stream) != hipSuccess)
 		return -1;

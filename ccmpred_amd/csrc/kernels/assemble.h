@@ -12,6 +12,9 @@
 //   of both orientations enters g2, so either works.
 // Singles: g1 = sum of per-block partials + 2 lambda_single V for the 20L
 //   real singles, 0 on the padding.
+// The FS form of the items (the sparse BF16 path) leaves out the a >= 21
+// padding columns and the regularisation: its coupling conversion and
+// forward write those (kernels/sparse_fwd.hip).
 #pragma once
 
 #include "common.h"
@@ -80,7 +83,10 @@ __device__ __forceinline__ int ccm_pair_tiles(int L) { return (CCM_NA * L + EP_T
 // result is formed, and each item's arithmetic, its partial sum's order
 // included, is that of the item alone, so that K changes no bit.
 // regpart[blks[k]] receives the item's regularisation partial.
-template <int K>
+// FS (the form whose regularisation and padding gradient the coupling
+// conversion wrote): items cover the a < 21 columns only and leave no
+// regularisation partial.
+template <int K, bool FS = false>
 __device__ __forceinline__ void pair_sym_items(const float* __restrict__ M, long ldm, int Lm,
                                                const float* __restrict__ hsym,
                                                const float* __restrict__ x2, float* __restrict__ g2,
@@ -110,7 +116,7 @@ __device__ __forceinline__ void pair_sym_items(const float* __restrict__ M, long
       ccm_tri_pair(blk, NT, ti, tj);
       Ra0[k] = ti * EP_T;
       Ca[k] = tj * EP_T + lane;
-      has_a[k] = Ca[k] < ld2;
+      has_a[k] = Ca[k] < (FS ? C21 : ld2);
       pc_a[k] = Ca[k] < C21;
       ja[k] = dl.mod(Ca[k]);
       has_b[k] = ti != tj;
@@ -181,7 +187,7 @@ __device__ __forceinline__ void pair_sym_items(const float* __restrict__ M, long
         float v = 0.f;
         if (pc_a[k] && kR != ja[k]) v = D1[k][rr * LD + lane] + D2[k][lane * LD + rr];
         g2[(long)R * ld2 + Ca[k]] = v + two_l * xa[k][i];
-        reg += (double)(half_l * xa[k][i] * xa[k][i]);
+        if (!FS) reg += (double)(half_l * xa[k][i] * xa[k][i]);
       }
     }
     if (has_b[k]) {
@@ -193,12 +199,14 @@ __device__ __forceinline__ void pair_sym_items(const float* __restrict__ M, long
         float v = 0.f;
         if (kR != jb[k]) v = D2[k][rr * LD + lane] + D1[k][lane * LD + rr];
         g2[(long)R * ld2 + Cb[k]] = v + two_l * xb[k][i];
-        reg += (double)(half_l * xb[k][i] * xb[k][i]);
+        if (!FS) reg += (double)(half_l * xb[k][i] * xb[k][i]);
       }
     }
+    if (FS) continue;
     reg = ccm_wave_sum_f64(reg);
     if (lane == 0) red[k * EP_W + wv] = reg;
   }
+  if (FS) return;
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
@@ -218,7 +226,15 @@ __host__ __device__ __forceinline__ int ccm_pair_sym_blocks(int L) {
   return NT * (NT + 1) / 2 + NT * (npc > 0 ? npc : 0);
 }
 
-// singles item blk: EP_NTH singles (red: EP_W doubles)
+// items of the FS form: tile pairs only
+__host__ __device__ __forceinline__ int ccm_pair_sym_blocks_fs(int L) {
+  const int NT = (CCM_NA * L + EP_T - 1) / EP_T;
+  return NT * (NT + 1) / 2;
+}
+
+// singles item blk: EP_NTH singles (red: EP_W doubles); FS: no
+// regularisation partial
+template <bool FS = false>
 __device__ __forceinline__ void singles_item(const double* __restrict__ g1part, int nblk,
                                              const float* __restrict__ x1, float* __restrict__ g1, int L,
                                              long nsp, float lam1, double* __restrict__ regpart,
@@ -244,6 +260,7 @@ __device__ __forceinline__ void singles_item(const double* __restrict__ g1part, 
   } else if (v < nsp) {
     g1[v] = 0.f;
   }
+  if (FS) return;
   reg = ccm_wave_sum_f64(reg);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) red[wv] = reg;

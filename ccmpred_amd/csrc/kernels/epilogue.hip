@@ -3,19 +3,30 @@
 // evaluate_cpu.c:102-109,153-156 singles).
 #include "assemble.h"
 
+// LDS of an assembly workgroup: the two staged tiles of a pair item (the
+// objective's sum block of the FS form reuses them)
+struct ep_lds {
+  float D1s[EP_TILE_FLOATS];
+  float D2s[EP_TILE_FLOATS];
+  double red[EP_W];
+};
+__device__ __forceinline__ ep_lds& ep_shared() {
+  __shared__ ep_lds sh;
+  return sh;
+}
+
 // one pair item per workgroup
+template <bool FS = false>
 __device__ __forceinline__ void pair_sym_block(const float* __restrict__ M, long ldm, int Lm,
                                                const float* __restrict__ hsym,
                                                const float* __restrict__ x2, float* __restrict__ g2,
                                                int L, float lam2, double* __restrict__ regpart,
                                                uint32_t blk) {
-  __shared__ float D1s[EP_TILE_FLOATS];
-  __shared__ float D2s[EP_TILE_FLOATS];
-  __shared__ double red[EP_W];
+  ep_lds& sh = ep_shared();
   const uint32_t blks[1] = {blk};
-  float* const d1[1] = {D1s};
-  float* const d2[1] = {D2s};
-  pair_sym_items<1>(M, ldm, Lm, hsym, x2, g2, L, lam2, regpart, blks, d1, d2, red);
+  float* const d1[1] = {sh.D1s};
+  float* const d2[1] = {sh.D2s};
+  pair_sym_items<1, FS>(M, ldm, Lm, hsym, x2, g2, L, lam2, regpart, blks, d1, d2, sh.red);
 }
 
 __global__ __launch_bounds__(EP_NTH) void k_pair_epilogue(const float* __restrict__ M, long ldm, int Lm,
@@ -68,12 +79,12 @@ __global__ __launch_bounds__(256) void k_hsym(const float* __restrict__ Mh, long
   }
 }
 
+template <bool FS = false>
 __device__ __forceinline__ void singles_block(const double* __restrict__ g1part, int nblk,
                                               const float* __restrict__ x1, float* __restrict__ g1,
                                               int L, long nsp, float lam1, double* __restrict__ regpart,
                                               uint32_t blk) {
-  __shared__ double red[EP_W];
-  singles_item(g1part, nblk, x1, g1, L, nsp, lam1, regpart, blk, red);
+  singles_item<FS>(g1part, nblk, x1, g1, L, nsp, lam1, regpart, blk, ep_shared().red);
 }
 
 __global__ __launch_bounds__(EP_NTH) void k_singles_finalize(const double* __restrict__ g1part, int nblk,
@@ -120,12 +131,16 @@ __global__ __launch_bounds__(256) void k_reg_only(const float* __restrict__ x, l
 // a|b|c (eight loads issued before their adds), then wave 0 folds the
 // SUM_NT thread sums, SUM_NT/64 per lane, and a lane tree combines the 64
 // (sum, compensation) pairs with exact two-sums.  out[0] = total (double)
+// A workgroup of NT threads (a divisor of SUM_NT) computes the same sum
+// bit for bit: thread t takes the SUM_NT / NT virtual threads t, t + NT, ..
+// in turn.  sh, shc: SUM_NT doubles of LDS each.
 #define SUM_NT 1024
-__device__ __forceinline__ void sum3_block(const double* __restrict__ a, int na,
-                                           const double* __restrict__ b, int nb,
-                                           const double* __restrict__ c, int nc,
-                                           double* __restrict__ out) {
-  __shared__ double sh[SUM_NT], shc[SUM_NT];
+template <int NT>
+__device__ __forceinline__ void sum3_block_lds(const double* __restrict__ a, int na,
+                                               const double* __restrict__ b, int nb,
+                                               const double* __restrict__ c, int nc,
+                                               double* __restrict__ out, double* sh, double* shc) {
+  static_assert(SUM_NT % NT == 0 && NT >= 64, "virtual threads");
   double s = 0.0, comp = 0.0;
   auto add = [&](double v) {
     const double t = s + v;
@@ -136,19 +151,23 @@ __device__ __forceinline__ void sum3_block(const double* __restrict__ a, int na,
     s = t;
   };
   const int n = na + nb + nc;
-  for (int i0 = threadIdx.x; i0 < n; i0 += SUM_NT * 8) {
-    double v[8];
+  for (int vt = threadIdx.x; vt < SUM_NT; vt += NT) {
+    s = 0.0;
+    comp = 0.0;
+    for (int i0 = vt; i0 < n; i0 += SUM_NT * 8) {
+      double v[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int i = i0 + u * SUM_NT;
-      v[u] = i < na ? a[i] : i < na + nb ? b[i - na] : i < n ? c[i - na - nb] : 0.0;
+      for (int u = 0; u < 8; u++) {
+        const int i = i0 + u * SUM_NT;
+        v[u] = i < na ? a[i] : i < na + nb ? b[i - na] : i < n ? c[i - na - nb] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (i0 + u * SUM_NT < n) add(v[u]);
     }
-#pragma unroll
-    for (int u = 0; u < 8; u++)
-      if (i0 + u * SUM_NT < n) add(v[u]);
+    sh[vt] = s;
+    shc[vt] = comp;
   }
-  sh[threadIdx.x] = s;
-  shc[threadIdx.x] = comp;
   __syncthreads();
   if (threadIdx.x < 64) {
     constexpr int PER = SUM_NT / 64;
@@ -170,6 +189,14 @@ __device__ __forceinline__ void sum3_block(const double* __restrict__ a, int na,
     }
     if (l == 0) out[0] = s + comp;
   }
+}
+
+__device__ __forceinline__ void sum3_block(const double* __restrict__ a, int na,
+                                           const double* __restrict__ b, int nb,
+                                           const double* __restrict__ c, int nc,
+                                           double* __restrict__ out) {
+  __shared__ double sh[SUM_NT], shc[SUM_NT];
+  sum3_block_lds<SUM_NT>(a, na, b, nb, c, nc, out, sh, shc);
 }
 
 __global__ __launch_bounds__(SUM_NT) void k_sum3(const double* __restrict__ a, int na,
@@ -214,6 +241,44 @@ __device__ __forceinline__ void assemble_block(const float* __restrict__ M, long
     pair_sym_block(M, ldm, Lm, hsym, x + nsp, g + nsp, L, lam2, regpair, blk - nsing);
 }
 
+// FS form (the coupling conversion wrote the regularisation partials and
+// the a >= 21 padding of g): singles, then the tile pairs of the a < 21
+// columns, no partials
+__device__ __forceinline__ void assemble_block_fs(const float* __restrict__ M, long ldm, int Lm,
+                                                  const float* __restrict__ hsym,
+                                                  const float* __restrict__ x, float* __restrict__ g,
+                                                  int L, long nsp, float lam1, float lam2,
+                                                  const double* __restrict__ g1part, int n_g1blk,
+                                                  uint32_t blk) {
+  const uint32_t nsing = (uint32_t)((nsp + EP_NTH - 1) / EP_NTH);
+  if (blk < nsing)
+    singles_block<true>(g1part, n_g1blk, x, g, L, nsp, lam1, nullptr, blk);
+  else
+    pair_sym_block<true>(M, ldm, Lm, hsym, x + nsp, g + nsp, L, lam2, nullptr, blk - nsing);
+}
+
+// the single-alignment FS launch: block 0 forms the objective (the sum of
+// k_sum3, over the softmax partials and the conversion's regularisation
+// partials, which earlier launches wrote) while the others assemble g
+__global__ __launch_bounds__(EP_NTH) void k_assemble_fs(const float* __restrict__ M, long ldm, int Lm,
+                                                     const float* __restrict__ hsym,
+                                                     const float* __restrict__ x, float* __restrict__ g,
+                                                     int L, long nsp, float lam1, float lam2,
+                                                     const double* __restrict__ g1part, int n_g1blk,
+                                                     const double* __restrict__ fpart, int n_fpart,
+                                                     const double* __restrict__ regpair, int n_regpair,
+                                                     const double* __restrict__ regsingle,
+                                                     int n_regsingle, double* __restrict__ f_out) {
+  if (blockIdx.x == 0) {
+    static_assert(EP_TILE_FLOATS * sizeof(float) >= SUM_NT * sizeof(double), "sum in the tile buffers");
+    ep_lds& sh = ep_shared();
+    sum3_block_lds<EP_NTH>(fpart, n_fpart, regpair, n_regpair, regsingle, n_regsingle, f_out,
+                           reinterpret_cast<double*>(sh.D1s), reinterpret_cast<double*>(sh.D2s));
+    return;
+  }
+  assemble_block_fs(M, ldm, Lm, hsym, x, g, L, nsp, lam1, lam2, g1part, n_g1blk, blockIdx.x - 1);
+}
+
 __global__ __launch_bounds__(EP_NTH) void k_assemble(const float* __restrict__ M, long ldm, int Lm,
                                                   const float* __restrict__ hsym,
                                                   const float* __restrict__ x, float* __restrict__ g,
@@ -232,14 +297,40 @@ __global__ __launch_bounds__(EP_NTH) void k_assemble_grouped(const ccm_bprob* __
   const int gi = ccm_group_find(offs, np, blockIdx.x);
   const ccm_bprob& B = probs[pidx[gi]];
   if (!B.ctl->eval_on) return;
-  assemble_block(B.gt, B.P, B.Lp, B.hsym, B.x, B.g, B.L, B.nsp, B.lam1, B.lam2, B.g1part, B.n_g1blk,
-                 B.regpair, B.regsingle, blockIdx.x - offs[gi]);
+  // the sparse form's conversion wrote the regularisation partials and the padding
+  if (B.sparse)
+    assemble_block_fs(B.gt, B.P, B.Lp, B.hsym, B.x, B.g, B.L, B.nsp, B.lam1, B.lam2, B.g1part, B.n_g1blk,
+                      blockIdx.x - offs[gi]);
+  else
+    assemble_block(B.gt, B.P, B.Lp, B.hsym, B.x, B.g, B.L, B.nsp, B.lam1, B.lam2, B.g1part, B.n_g1blk,
+                   B.regpair, B.regsingle, blockIdx.x - offs[gi]);
 }
 
 extern "C" {
 
 // blocks of the fused assembly: pair tiles, then singles
 int ccm_assemble_blocks(int L) { return ccm_pair_epilogue_blocks(L) + ccm_singles_blocks(L); }
+
+// blocks of the FS assembly of one alignment in a grouped launch (the
+// single-alignment launch has one more, the objective's sum)
+int ccm_assemble_fs_blocks(int L) { return ccm_pair_sym_blocks_fs(L) + ccm_singles_blocks(L); }
+
+// FS form: g from the raw pair gradient, hsym and the node-gradient
+// partials for the singles and the a < 21 pair columns (the coupling
+// conversion ccm_launch_convert_ws_reg wrote the a >= 21 padding of g and
+// the regularisation partials), and in the same launch
+// *d_f = sum of d_fpart | d_regpair | d_regsingle (the order of k_sum3)
+int ccm_launch_assemble_fs(const float* d_m, long ldm, int Lm, const float* d_hsym, const float* d_x,
+                           float* d_g, int L, float lam1, float lam2, const double* d_g1part, int n_g1blk,
+                           const double* d_fpart, int n_fpart, const double* d_regpair, int n_regpair,
+                           const double* d_regsingle, int n_regsingle, double* d_f, hipStream_t s) {
+  if (L <= 0 || !d_m || !d_x || !d_g || !d_f || n_fpart < 0 || n_regpair < 0 || n_regsingle < 0) return CCM_ERR_ARG;
+  hipLaunchKernelGGL(k_assemble_fs, dim3(1 + ccm_assemble_fs_blocks(L)), dim3(EP_NTH), 0, s, d_m, ldm, Lm, d_hsym,
+                     d_x, d_g, L, ccm_nsp(L), lam1, lam2, d_g1part, n_g1blk, d_fpart, n_fpart, d_regpair,
+                     n_regpair, d_regsingle, n_regsingle, d_f);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
 
 // g (pairs and singles) from the raw pair gradient M (FP32 layout Lm = L,
 // or the padded BF16 layout Lm = Lp with hsym), the node-gradient partials

@@ -92,6 +92,101 @@ __global__ __launch_bounds__(256) void k_sparse_index(const uint8_t* __restrict_
 constexpr int CWS_CK = 8;                   // coupling columns per tile
 constexpr int CWS_R = CCM_NA * CWS_CK;      // staged values per Ws row (56 groups of 3)
 constexpr int CWS_LD = CWS_R + 2;           // uint16 per LDS row: 340 bytes, 4-byte aligned
+// nontemporal 16-byte load (streaming data: see convert_ws_pad_tile)
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 cws_load4(const float* p) {
+  const f4v t = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
+  return make_float4(t.x, t.y, t.z, t.w);
+}
+
+// block sum of a 256-thread workgroup's FP64 terms into *out (thread 0).
+// In a workgroup of 512 threads the halves are two such blocks (waves 0-3
+// and 4-7), each with its own out (null: no store); every thread calls it.
+__device__ __forceinline__ void cws_block_sum(double r, double* __restrict__ out) {
+  __shared__ double red[8];
+  r = ccm_wave_sum_f64(r);
+  const int w = (int)(threadIdx.x >> 6);
+  if ((threadIdx.x & 63) == 0) red[w] = r;
+  __syncthreads();
+  if ((threadIdx.x & 255) == 0 && out) {
+    const int h = 4 * (w >> 2);
+    *out = (red[h] + red[h + 1]) + (red[h + 2] + red[h + 3]);
+  }
+}
+
+// The padding columns a >= 21 of the pair block: their gradient is
+// 2 lambda_pair x and their regularisation 0.5 lambda_pair x^2
+// (evaluate_cpu.c:158-161).  Tile (bxp, by): the 168 rows (b, k),
+// k in [8 by, 8 by + 8), over the 128 x-layout columns from
+// Cs = 4 floor(21 L / 4) + 128 bxp (16-byte granules; in the first one
+// only the columns >= 21 L count), every load of the tile issued first.
+// Each 256 threads take one tile (tile false: none; the sum is still joined).
+constexpr int CWP = 128;
+__device__ __forceinline__ void convert_ws_pad_tile(const float* __restrict__ x2, float* __restrict__ g2, int L,
+                                                    float lam2, int bxp, int by, bool tile,
+                                                    double* __restrict__ regpart) {
+  constexpr int F4 = CWP / 4, NE = CWS_R * F4, PER = NE / 256;
+  static_assert(NE % 256 == 0, "granules per thread");
+  const int C21 = CCM_NA * L;
+  const long ld2 = (long)CCM_NAP * L;
+  const int k0 = by * CWS_CK, Cs = (C21 & ~3) + bxp * CWP;
+  const int t = (int)(threadIdx.x & 255);
+  const bool al = ((uintptr_t)x2 & 15u) == 0 && ((uintptr_t)g2 & 15u) == 0;
+  const float two_l = 2.0f * lam2, half_l = 0.5f * lam2;
+  float4 v[PER];
+  long off[PER];
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int e = t + 256 * u, row = e / F4, C = Cs + 4 * (e - row * F4);
+    const int b = row / CWS_CK, k = k0 + (row - b * CWS_CK);
+    off[u] = (tile && k < L && C < ld2) ? (long)(b * L + k) * ld2 + C : -1;
+    v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (off[u] >= 0) {
+      // streaming accesses (x and g here are touched once per evaluation):
+      // they leave the caches to the contractions' operands
+      const float* src = x2 + off[u];
+      if (al) {
+        v[u] = cws_load4(src);
+      } else {
+        v[u] = make_float4(__builtin_nontemporal_load(src), __builtin_nontemporal_load(src + 1),
+                           __builtin_nontemporal_load(src + 2), __builtin_nontemporal_load(src + 3));
+      }
+    }
+  }
+  double r = 0.0;
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    if (off[u] < 0) continue;
+    const int C = Cs + 4 * ((t + 256 * u) % F4);
+    const float q[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+    float o[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      o[c] = 0.f + two_l * q[c];
+      if (C + c >= C21) r += (double)(half_l * q[c] * q[c]);
+    }
+    float* dst = g2 + off[u];
+    if (C >= C21 && al) {
+      const f4v ov = {o[0], o[1], o[2], o[3]};
+      __builtin_nontemporal_store(ov, reinterpret_cast<f4v*>(dst));
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        if (C + c >= C21) __builtin_nontemporal_store(o[c], dst + c);
+    }
+  }
+  cws_block_sum(r, tile ? regpart : nullptr);
+}
+
+// the singles' regularisation lambda_single sum V^2 over the 20 L singles
+// (evaluate_cpu.c:153-156), one block
+__device__ __forceinline__ void convert_ws_singles_reg(const float* __restrict__ x1, int L, float lam1,
+                                                       double* __restrict__ out) {
+  double r = 0.0;
+  for (int v = (int)threadIdx.x; v < 20 * L; v += 256) r += (double)(lam1 * x1[v] * x1[v]);
+  cws_block_sum(r, out);
+}
+
 template <int PASSES>
 struct cws_shape {
   static constexpr int CW = PASSES == 1 ? 128 : 64;  // x-layout columns (Ws rows) per tile
@@ -99,7 +194,8 @@ struct cws_shape {
 
 template <int PASSES>
 __device__ __forceinline__ void convert_ws_block(const float* __restrict__ x2, int L, int Lp,
-                                                 uint16_t* __restrict__ ws, long lds, int bx, int by) {
+                                                 uint16_t* __restrict__ ws, long lds, int bx, int by,
+                                                 float half_l, double* __restrict__ regpart) {
   constexpr int CW = cws_shape<PASSES>::CW;
   constexpr int F4 = CW / 4;                  // 16-byte loads per row segment
   constexpr int NE = CWS_R * F4;              // loads of the tile
@@ -126,6 +222,23 @@ __device__ __forceinline__ void convert_ws_block(const float* __restrict__ x2, i
         v[u] = al ? *reinterpret_cast<const float4*>(src) : make_float4(src[0], src[1], src[2], src[3]);
       }
     }
+  }
+  if (regpart) {
+    // the objective's regularisation over the values this block read
+    // (each coupling of the a < 21 block is read by one block; a 16-byte
+    // load at the last columns also holds up to three padding columns
+    // C >= 21 L, which the padding blocks count)
+    double r = 0.0;
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int e = (int)threadIdx.x + 256 * u;
+      const int C = C0 + 4 * (e % F4);
+      const float q0 = v[u].x, q1 = C + 1 < C21 ? v[u].y : 0.f;
+      const float q2 = C + 2 < C21 ? v[u].z : 0.f, q3 = C + 3 < C21 ? v[u].w : 0.f;
+      r += ((double)(half_l * q0 * q0) + (double)(half_l * q1 * q1)) +
+           ((double)(half_l * q2 * q2) + (double)(half_l * q3 * q3));
+    }
+    cws_block_sum(r, regpart);
   }
 #pragma unroll
   for (int u = 0; u < PER; u++) {
@@ -177,14 +290,45 @@ __host__ __device__ __forceinline__ int convert_ws_tiles_c(int L, int passes) {
   return (CCM_NA * L + cw - 1) / cw;
 }
 __host__ __device__ __forceinline__ int convert_ws_rows(int Lp) { return Lp / CWS_CK; }
-
-template <int PASSES>
-__global__ __launch_bounds__(256) void k_convert_ws(const float* __restrict__ x2, int L, int Lp,
-                                                    uint16_t* __restrict__ ws, long lds) {
-  convert_ws_block<PASSES>(x2, L, Lp, ws, lds, blockIdx.x, blockIdx.y);
+// padding-column tiles of the form that also writes the padding gradient
+__host__ __device__ __forceinline__ int convert_ws_tiles_p(int L) {
+  return (CCM_NAP * L - ((CCM_NA * L) & ~3) + CWP - 1) / CWP;
 }
 
-// every alignment of a group (x buffers of a batch are whole alignments)
+// One block of the conversion grid (bx, by).  Plain form (regpart null):
+// bx < tiles_c, the coupling transpose.  With the regularisation: the
+// transpose blocks bx < tiles_c also write the regularisation partial
+// regpart[by * (tiles_c + tiles_p) + bx]; the padding tiles bx - tiles_c
+// (partials at the same index) run here when tp_run = tiles_p (else, with
+// tp_run = 0, in the forward's launch); block (tiles_c + tp_run, 0) writes
+// the singles' regularisation into *regsingle (the other blocks of that
+// column return).
+template <int PASSES>
+__device__ __forceinline__ void convert_ws_any(const float* __restrict__ x, long nsp, int L, int Lp,
+                                               uint16_t* __restrict__ ws, long lds, float* __restrict__ g,
+                                               float lam1, float lam2, double* __restrict__ regpart,
+                                               double* __restrict__ regsingle, int tp_run, int bx, int by) {
+  const int tc = convert_ws_tiles_c(L, PASSES), tp = convert_ws_tiles_p(L);
+  if (bx < tc)
+    convert_ws_block<PASSES>(x + nsp, L, Lp, ws, lds, bx, by, 0.5f * lam2,
+                             regpart ? regpart + (long)by * (tc + tp) + bx : nullptr);
+  else if (bx < tc + tp_run)
+    convert_ws_pad_tile(x + nsp, g + nsp, L, lam2, bx - tc, by, true, regpart + (long)by * (tc + tp) + bx);
+  else if (by == 0)
+    convert_ws_singles_reg(x, L, lam1, regsingle);
+}
+
+template <int PASSES>
+__global__ __launch_bounds__(256) void k_convert_ws(const float* __restrict__ x, long nsp, int L, int Lp,
+                                                    uint16_t* __restrict__ ws, long lds, float* __restrict__ g,
+                                                    float lam1, float lam2, double* __restrict__ regpart,
+                                                    double* __restrict__ regsingle, int tp_run) {
+  convert_ws_any<PASSES>(x, nsp, L, Lp, ws, lds, g, lam1, lam2, regpart, regsingle, tp_run, blockIdx.x,
+                         blockIdx.y);
+}
+
+// every alignment of a group (x buffers of a batch are whole alignments),
+// always with the regularisation and the padding gradient
 template <int PASSES>
 __global__ __launch_bounds__(256) void k_convert_ws_grouped(const ccm_bprob* __restrict__ probs,
                                                             const uint32_t* __restrict__ offs,
@@ -192,8 +336,10 @@ __global__ __launch_bounds__(256) void k_convert_ws_grouped(const ccm_bprob* __r
   const int gi = ccm_group_find(offs, np, blockIdx.x);
   const ccm_bprob& B = probs[pidx[gi]];
   if (!B.ctl->eval_on) return;
-  const int t = (int)(blockIdx.x - offs[gi]), tc = convert_ws_tiles_c(B.L, PASSES);
-  convert_ws_block<PASSES>(B.x + B.nsp, B.L, B.Lp, B.wt, B.ldwt, t % tc, t / tc);
+  const int t = (int)(blockIdx.x - offs[gi]);
+  const int tp = convert_ws_tiles_p(B.L), tcx = convert_ws_tiles_c(B.L, PASSES) + tp + 1;
+  convert_ws_any<PASSES>(B.x, B.nsp, B.L, B.Lp, B.wt, B.ldwt, B.g, B.lam1, B.lam2, B.regpair, B.regsingle,
+                         tp, t % tcx, t / tcx);
 }
 
 // One workgroup: TM = WM*RB*32 sequences x TN = WN*CB*32 gathered rows of Ws
@@ -388,10 +534,31 @@ __device__ __forceinline__ void sparse_fwd_tile(const uint16_t* __restrict__ Ws,
   fwd_softmax_partials<JG, NW * 64, ROUNDS>(gsum, fsum, epi, mt, n0, Ncol, tid);
 }
 
+// The padding tiles of the coupling conversion (convert_ws_pad_tile), run
+// by workgroups past the forward's tiles, two tiles per workgroup: they are
+// dispatched after every tile and take CUs that the forward's last, partial
+// round of tiles leaves idle.
+struct ccm_pad_job {
+  const float* x2;  // x past the singles
+  float* g2;        // g past the singles
+  double* regpart;  // the conversion's partials: tile (bxp, by) at by * stride + tc + bxp
+  float lam2;
+  int tp, rows, stride, tc;
+};
+
 template <int WM, int WN, int RB, int CB>
 __global__ __launch_bounds__(64 * WM * WN, 1) void k_sparse_fwd(
     const uint16_t* __restrict__ Ws, long lds, int Lp, int nsteps, const uint32_t* __restrict__ idx,
-    long ldidx, int Ncol, uint32_t tiles_n, uint32_t tiles, uint32_t total, ccm_fwd_epi epi) {
+    long ldidx, int Ncol, uint32_t tiles_n, uint32_t tiles, uint32_t total, ccm_fwd_epi epi, ccm_pad_job pj) {
+  static_assert(64 * WM * WN == 512, "two padding tiles per workgroup");
+  if (blockIdx.x >= total) {
+    const uint32_t q = 2u * (blockIdx.x - total) + (threadIdx.x >> 8);
+    const bool ok = q < (uint32_t)(pj.tp * pj.rows);
+    const int bxp = ok ? (int)(q % (uint32_t)pj.tp) : 0, by = ok ? (int)(q / (uint32_t)pj.tp) : 0;
+    convert_ws_pad_tile(pj.x2, pj.g2, epi.L, pj.lam2, bxp, by, ok,
+                        pj.regpart + (long)by * pj.stride + pj.tc + bxp);
+    return;
+  }
   const uint32_t qid = ccm_xcd_remap(blockIdx.x, total);
   if (qid >= tiles) return;
   sparse_fwd_tile<WM, WN, RB, CB>(Ws, lds, Lp, nsteps, idx, ldidx, Ncol, qid / tiles_n, qid % tiles_n, epi);
@@ -456,20 +623,59 @@ int ccm_launch_convert_ws(const float* d_x, int L, int Lp, uint16_t* d_ws, long 
       lds % 8)
     return CCM_ERR_ARG;
   const dim3 grid(convert_ws_tiles_c(L, passes), convert_ws_rows(Lp));
+  const long nsp = ccm_nsp(L);
+  float* const g = nullptr;
+  double* const rp = nullptr;
   if (passes == 1)
-    hipLaunchKernelGGL(k_convert_ws<1>, grid, dim3(256), 0, s, d_x + ccm_nsp(L), L, Lp, d_ws, lds);
+    hipLaunchKernelGGL(k_convert_ws<1>, grid, dim3(256), 0, s, d_x, nsp, L, Lp, d_ws, lds, g, 0.f, 0.f, rp, rp, 0);
   else
-    hipLaunchKernelGGL(k_convert_ws<2>, grid, dim3(256), 0, s, d_x + ccm_nsp(L), L, Lp, d_ws, lds);
+    hipLaunchKernelGGL(k_convert_ws<2>, grid, dim3(256), 0, s, d_x, nsp, L, Lp, d_ws, lds, g, 0.f, 0.f, rp, rp, 0);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }
 
-int ccm_launch_sparse_fwd_softmax(const uint16_t* d_ws, long lds, int Lp, int passes_fwd,
-                                  const uint32_t* d_idx, long ldidx, int ncols, const float* d_x1,
-                                  const uint8_t* d_msapt, long ldmt, const float* d_w, int L, int Nc,
-                                  uint16_t* d_dt, long lddt, long lo_off, int passes, int mode, int blk0,
-                                  double* d_fpart, double* d_g1part, ccm_stream_t stream) {
+int ccm_convert_ws_reg_parts(int L, int Lp, int passes) {
+  if (L < 1 || Lp % 64 || Lp < L || passes < 1 || passes > 2) return 0;
+  return (convert_ws_tiles_c(L, passes) + convert_ws_tiles_p(L)) * convert_ws_rows(Lp);
+}
+
+int ccm_launch_convert_ws_reg(const float* d_x, float* d_g, int L, int Lp, uint16_t* d_ws, long lds, int passes,
+                              float lam1, float lam2, double* d_regpair, double* d_regsingle, int pad,
+                              ccm_stream_t stream) {
   hipStream_t s = (hipStream_t)stream;
+  if (Lp % 64 || Lp < L || L < 1 || passes < 1 || passes > 2 || lds < ccm_sparse_ws_len(Lp, passes) ||
+      lds % 8 || !d_x || !d_ws || !d_regpair || !d_regsingle || (pad && !d_g))
+    return CCM_ERR_ARG;
+  const int tp_run = pad ? convert_ws_tiles_p(L) : 0;
+  const dim3 grid(convert_ws_tiles_c(L, passes) + tp_run + 1, convert_ws_rows(Lp));
+  const long nsp = ccm_nsp(L);
+  if (passes == 1)
+    hipLaunchKernelGGL(k_convert_ws<1>, grid, dim3(256), 0, s, d_x, nsp, L, Lp, d_ws, lds, d_g, lam1, lam2,
+                       d_regpair, d_regsingle, tp_run);
+  else
+    hipLaunchKernelGGL(k_convert_ws<2>, grid, dim3(256), 0, s, d_x, nsp, L, Lp, d_ws, lds, d_g, lam1, lam2,
+                       d_regpair, d_regsingle, tp_run);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
+
+int ccm_launch_sparse_fwd_softmax_pad(const uint16_t* d_ws, long lds, int Lp, int passes_fwd,
+                                      const uint32_t* d_idx, long ldidx, int ncols, const float* d_x1,
+                                      const uint8_t* d_msapt, long ldmt, const float* d_w, int L, int Nc,
+                                      uint16_t* d_dt, long lddt, long lo_off, int passes, int mode, int blk0,
+                                      double* d_fpart, double* d_g1part, const float* d_x, float* d_g,
+                                      float lam2, double* d_regpair, ccm_stream_t stream) {
+  hipStream_t s = (hipStream_t)stream;
+  // the padding tiles of the conversion, when asked for
+  ccm_pad_job pj{};
+  uint32_t npad = 0;
+  if (d_x || d_g || d_regpair) {
+    if (!d_x || !d_g || !d_regpair || !(Lp % 64 == 0 && Lp >= L && L >= 1)) return CCM_ERR_ARG;
+    const long nsp = ccm_nsp(L);
+    const int tc = convert_ws_tiles_c(L, passes_fwd == 1 ? 1 : 2), tp = convert_ws_tiles_p(L);
+    pj = ccm_pad_job{d_x + nsp, d_g + nsp, d_regpair, lam2, tp, convert_ws_rows(Lp), tc + tp, tc};
+    npad = (uint32_t)((tp * convert_ws_rows(Lp) + 1) / 2);
+  }
   const long P = (long)CCM_NA * Lp;
   const int nsteps = ccm_sparse_steps(Lp, passes_fwd);
   if (Lp % 64 || Lp < L || L < 1 || ncols % 128 || ncols <= 0 || Nc < 1 || Nc > ncols || nsteps <= 0 ||
@@ -487,14 +693,26 @@ int ccm_launch_sparse_fwd_softmax(const uint16_t* d_ws, long lds, int Lp, int pa
   const uint32_t tiles = (uint32_t)((L + JG - 1) / JG) * tiles_n, total = (tiles + 7u) & ~7u;
   // the sequence tiles read positions up to the next multiple of 256
   if (ldidx < (long)tiles_n * SF_TM) return CCM_ERR_ARG;
-  hipLaunchKernelGGL((k_sparse_fwd<2, 4, 4, 2>), dim3(total), dim3(512), 0, s, d_ws, lds, Lp, nsteps, d_idx,
-                     ldidx, ncols, tiles_n, tiles, total, epi);
+  hipLaunchKernelGGL((k_sparse_fwd<2, 4, 4, 2>), dim3(total + npad), dim3(512), 0, s, d_ws, lds, Lp, nsteps,
+                     d_idx, ldidx, ncols, tiles_n, tiles, total, epi, pj);
   CCM_LAUNCH_CHECK();
   return CCM_OK;
 }
 
+int ccm_launch_sparse_fwd_softmax(const uint16_t* d_ws, long lds, int Lp, int passes_fwd,
+                                  const uint32_t* d_idx, long ldidx, int ncols, const float* d_x1,
+                                  const uint8_t* d_msapt, long ldmt, const float* d_w, int L, int Nc,
+                                  uint16_t* d_dt, long lddt, long lo_off, int passes, int mode, int blk0,
+                                  double* d_fpart, double* d_g1part, ccm_stream_t stream) {
+  return ccm_launch_sparse_fwd_softmax_pad(d_ws, lds, Lp, passes_fwd, d_idx, ldidx, ncols, d_x1, d_msapt, ldmt,
+                                           d_w, L, Nc, d_dt, lddt, lo_off, passes, mode, blk0, d_fpart, d_g1part,
+                                           nullptr, nullptr, 0.f, nullptr, stream);
+}
+
 // grouped forms (batched alignments; descriptors in ccmpred_amd_batch.h)
-int ccm_convert_ws_blocks(int L, int Lp, int passes) { return convert_ws_tiles_c(L, passes) * convert_ws_rows(Lp); }
+int ccm_convert_ws_blocks(int L, int Lp, int passes) {
+  return (convert_ws_tiles_c(L, passes) + convert_ws_tiles_p(L) + 1) * convert_ws_rows(Lp);
+}
 
 uint32_t ccm_sparse_fwd_tile_rows(int L) { return L < 1 ? 0u : (uint32_t)((L + SF_TN / CCM_NA - 1) / (SF_TN / CCM_NA)); }
 

@@ -103,10 +103,34 @@ long ccm_sparse_index_ld(int Np);
 int ccm_sparse_fwd_fits(int Lp, int passes_fwd, int passes, int ncols);
 int ccm_launch_convert_ws(const float *d_x, int L, int Lp, uint16_t *d_ws, long lds, int passes,
                           ccm_stream_t s);
+/* convert_ws that also writes the a >= 21 padding of the pair gradient
+ * (2 lambda_pair x, evaluate_cpu.c:158-161) into d_g and the objective's
+ * regularisation (evaluate_cpu.c:151-163) as ccm_convert_ws_reg_parts(L, Lp,
+ * passes) partials into d_regpair and one into d_regsingle, so that the
+ * assembly (ccm_launch_assemble_fs) leaves both out and forms f itself
+ * (the reference's CUDA backend likewise forms the regularisation and
+ * starts g before the contractions: evaluate_cuda.c:48-51,
+ * gpu_compute_regularization at cu:146-219,691-740) */
+int ccm_convert_ws_reg_parts(int L, int Lp, int passes);
+int ccm_launch_convert_ws_reg(const float *d_x, float *d_g, int L, int Lp, uint16_t *d_ws, long lds,
+                              int passes, float lam1, float lam2, double *d_regpair,
+                              double *d_regsingle, int pad, ccm_stream_t s);
+/* pad = 0 leaves the padding tiles (their g and partials) to the forward:
+ * ccm_launch_sparse_fwd_softmax_pad with d_x, d_g, lam2 and d_regpair runs
+ * them in workgroups after its tiles (two per 512-thread workgroup), on the
+ * CUs its last, partial round leaves idle */
+int ccm_launch_sparse_fwd_softmax_pad(const uint16_t *d_ws, long lds, int Lp, int passes_fwd,
+                                      const uint32_t *d_idx, long ldidx, int ncols, const float *d_x1,
+                                      const uint8_t *d_msapt, long ldmt, const float *d_w, int L, int Nc,
+                                      uint16_t *d_dt, long lddt, long lo_off, int passes, int mode,
+                                      int blk0, double *d_fpart, double *d_g1part, const float *d_x,
+                                      float *d_g, float lam2, double *d_regpair, ccm_stream_t s);
 int ccm_launch_sparse_index(const uint8_t *d_msap, int Lp, int Np, int passes, uint32_t *d_idx,
                             ccm_stream_t s);
 /* grouped forms over a batch: one alignment per group entry, descriptors
- * with sparse set (wt = Ws, sidx / ldsidx the positions) */
+ * with sparse set (wt = Ws, sidx / ldsidx the positions); the grouped
+ * conversion is the ccm_launch_convert_ws_reg form (regpair: the
+ * ccm_convert_ws_reg_parts partials, regsingle: one) */
 int ccm_convert_ws_blocks(int L, int Lp, int passes);
 uint32_t ccm_sparse_fwd_tile_rows(int L);
 uint32_t ccm_sparse_fwd_tile_count(int L, int ncols);
@@ -142,6 +166,17 @@ int ccm_launch_assemble(const float *d_m, long ldm, int Lm, const float *d_hsym,
                         float *d_g, int L, float lam1, float lam2, const double *d_g1part, int n_g1blk,
                         const double *d_fpart, int n_fpart, double *d_regpair, double *d_regsingle,
                         double *d_f, ccm_stream_t s);
+/* after ccm_launch_convert_ws_reg: the singles and the a < 21 pair columns
+ * of g, and in the same launch *d_f = the sum of d_fpart | d_regpair |
+ * d_regsingle in k_sum3's order (the grouped assembly of a sparse
+ * descriptor takes this form, ccm_assemble_fs_blocks(L) blocks, and leaves
+ * the sum to ccm_launch_sum3_grouped) */
+int ccm_assemble_fs_blocks(int L);
+int ccm_launch_assemble_fs(const float *d_m, long ldm, int Lm, const float *d_hsym, const float *d_x,
+                           float *d_g, int L, float lam1, float lam2, const double *d_g1part,
+                           int n_g1blk, const double *d_fpart, int n_fpart, const double *d_regpair,
+                           int n_regpair, const double *d_regsingle, int n_regsingle, double *d_f,
+                           ccm_stream_t s);
 
 /* conjugate-gradient vector arithmetic */
 int ccm_dots_partial_len(void);

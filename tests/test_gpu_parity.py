@@ -271,6 +271,37 @@ def test_tiny_shapes(prec, tol, L, N):
     assert e <= tol
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "bf16f"])
+@pytest.mark.parametrize("L,N,chunk", [(37, 300, 0), (130, 700, 0), (61, 1100, 512), (250, 600, 0)])
+def test_padding_regularisation(prec, L, N, chunk):
+    """Nonzero a >= 21 padding couplings and nonzero singles against the FP64
+    oracle: the padding enters f through 0.5 lam2 x^2 and g through
+    2 lam2 x (evaluate_cpu.c:151-163).  The sparse BF16 forms write both in
+    the coupling conversion and form f inside the assembly; the padding
+    holds about 8 % of f here, so a missing or doubled term fails the f
+    bound.  L = 37, 61, 130 make 21 L no multiple of four (the padding
+    columns start unaligned); one case runs in sequence chunks."""
+    say(f"padding reg {prec} L={L} N={N} chunk={chunk}")
+    rng = np.random.default_rng(L + 7 * N)
+    msa = rng.integers(0, 21, (N, L), dtype=np.uint8)
+    x = make_x(msa, L + N, 0.02)
+    nsp = nsingle_padded(L)
+    x[nsp:].reshape(NA, L, NAP, L)[:, :, NA:, :] = rng.normal(0, 0.05, (NA, L, NAP - NA, L))
+    with P.PLL(msa, precision=prec, chunk_rows=chunk) as p:
+        f64, g64 = ora.evaluate_f64(msa, p.weights.astype(np.float64), x.astype(np.float64),
+                                    p.lambda_single, p.lambda_pair)
+        xd = torch.from_numpy(x).cuda()
+        f, g = p.evaluate(xd)
+        G = g[nsp:].view(NA, L, NAP, L)
+        X = xd[nsp:].view(NA, L, NAP, L)
+        assert torch.equal(G[:, :, NA:, :], 0.0 + (2.0 * np.float32(p.lambda_pair)) * X[:, :, NA:, :])
+        g = g.cpu().numpy()
+    e = rel_l2(g, g64)
+    say(f"  f={f:.12g} f64={f64:.12g} rel={abs(f - f64) / abs(f64):.2e} g rel_l2={e:.2e}")
+    assert abs(f - f64) / abs(f64) <= (1e-5 if prec == "fp32" else 1e-4)
+    assert e <= (1e-4 if prec == "fp32" else 1e-3)
+
+
 # ---- device CG end to end ------------------------------------------------
 
 def _spearman(a, b):
