@@ -387,8 +387,8 @@ def main():
     gemm = max((fwd_key, "gemm_grad"), key=lambda k: bf16["step_ms"][k])
     gemm_ms = bf16["step_ms"][gemm]
     achieved = flops / (gemm_ms * 1e-3) / 1e12
-    prefix = {"sparse_fwd_softmax": "k_sparse_fwd<", "gemm_fwd_softmax": "void k_gemm_onehot_dtl<2,",
-              "gemm_fwd": "void k_gemm_onehot_dtl<0,", "gemm_grad": "void k_gemm_onehot_dtl<1,"}[gemm]
+    prefix = {"sparse_fwd_softmax": "k_sparse_fwd<", "gemm_fwd_softmax": "k_gemm_onehot_dtl<2,",
+              "gemm_fwd": "k_gemm_onehot_dtl<0,", "gemm_grad": "k_gemm_onehot_dtl<1,"}[gemm]
     traffic, mfma_busy = measured_counters(prefix)
     fwd_roofline = None
     if fwd_key == "sparse_fwd_softmax":

@@ -742,7 +742,9 @@ static int enqueue_eval(ccm_batch *b)
 		KCHK(ccm_launch_softmax_cols_grouped(b->d_probs, b->g_soft.g, b->s));
 	KCHK(enqueue_gemms(b, 1));
 	KCHK(ccm_launch_assemble_grouped(b->d_probs, b->g_asm.g, b->s));
-	KCHK(ccm_launch_sum3_grouped(b->d_probs, b->g_all.g, b->s));
+	/* (the sparse form's assembly forms f itself) */
+	if (!b->sparse)
+		KCHK(ccm_launch_sum3_grouped(b->d_probs, b->g_all.g, b->s));
 	return 0;
 fail:
 	return -1;
@@ -985,12 +987,16 @@ int ccm_batch_solve(ccm_batch *b, const ccm_conjugrad_parameter *param, int roun
 		h[i].eval_on = 1;
 		h[i].phase = CCM_CG_INIT;
 		h[i].hi = INFINITY;
+		h[i].pad_scanned = 1;
 	}
 	HIPCHK(hipMemcpy(b->d_ctl, h, sizeof(ccm_cg_ctl) * (size_t)b->n, hipMemcpyHostToDevice));
+	/* g too: where x's padding is zero the sparse form's evaluation leaves
+	 * g's padding (2 lambda_pair x = 0 there) as it is */
 	for (int i = 0; i < b->n; i++) {
 		const size_t vb = sizeof(float) * (size_t)b->h_probs[i].nvp;
 		HIPCHK(hipMemsetAsync(b->h_probs[i].s, 0, vb, b->s));
 		HIPCHK(hipMemsetAsync(b->h_probs[i].gprev, 0, vb, b->s));
+		HIPCHK(hipMemsetAsync(b->h_probs[i].g, 0, vb, b->s));
 	}
 	/* the vector step and dots skip the a >= 21 padding of every x that
 	 * starts with zeros there */

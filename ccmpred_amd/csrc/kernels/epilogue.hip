@@ -297,11 +297,19 @@ __global__ __launch_bounds__(EP_NTH) void k_assemble_grouped(const ccm_bprob* __
   const int gi = ccm_group_find(offs, np, blockIdx.x);
   const ccm_bprob& B = probs[pidx[gi]];
   if (!B.ctl->eval_on) return;
-  // the sparse form's conversion wrote the regularisation partials and the padding
-  if (B.sparse)
+  // the sparse form's conversion wrote the regularisation partials and the
+  // padding; the alignment's first block forms f as k_assemble_fs does
+  if (B.sparse) {
+    const uint32_t blk = blockIdx.x - offs[gi];
+    if (blk == 0) {
+      ep_lds& sh = ep_shared();
+      sum3_block_lds<EP_NTH>(B.fpart, B.n_fpart, B.regpair, B.n_regpair, B.regsingle, B.n_regsingle, &B.ctl->fx,
+                             reinterpret_cast<double*>(sh.D1s), reinterpret_cast<double*>(sh.D2s));
+      return;
+    }
     assemble_block_fs(B.gt, B.P, B.Lp, B.hsym, B.x, B.g, B.L, B.nsp, B.lam1, B.lam2, B.g1part, B.n_g1blk,
-                      blockIdx.x - offs[gi]);
-  else
+                      blk - 1);
+  } else
     assemble_block(B.gt, B.P, B.Lp, B.hsym, B.x, B.g, B.L, B.nsp, B.lam1, B.lam2, B.g1part, B.n_g1blk,
                    B.regpair, B.regsingle, blockIdx.x - offs[gi]);
 }
@@ -311,9 +319,9 @@ extern "C" {
 // blocks of the fused assembly: pair tiles, then singles
 int ccm_assemble_blocks(int L) { return ccm_pair_epilogue_blocks(L) + ccm_singles_blocks(L); }
 
-// blocks of the FS assembly of one alignment in a grouped launch (the
-// single-alignment launch has one more, the objective's sum)
-int ccm_assemble_fs_blocks(int L) { return ccm_pair_sym_blocks_fs(L) + ccm_singles_blocks(L); }
+// blocks of the FS assembly of one alignment: the objective's sum, the
+// singles, the tile pairs
+int ccm_assemble_fs_blocks(int L) { return 1 + ccm_pair_sym_blocks_fs(L) + ccm_singles_blocks(L); }
 
 // FS form: g from the raw pair gradient, hsym and the node-gradient
 // partials for the singles and the a < 21 pair columns (the coupling
@@ -325,7 +333,7 @@ int ccm_launch_assemble_fs(const float* d_m, long ldm, int Lm, const float* d_hs
                            const double* d_fpart, int n_fpart, const double* d_regpair, int n_regpair,
                            const double* d_regsingle, int n_regsingle, double* d_f, hipStream_t s) {
   if (L <= 0 || !d_m || !d_x || !d_g || !d_f || n_fpart < 0 || n_regpair < 0 || n_regsingle < 0) return CCM_ERR_ARG;
-  hipLaunchKernelGGL(k_assemble_fs, dim3(1 + ccm_assemble_fs_blocks(L)), dim3(EP_NTH), 0, s, d_m, ldm, Lm, d_hsym,
+  hipLaunchKernelGGL(k_assemble_fs, dim3(ccm_assemble_fs_blocks(L)), dim3(EP_NTH), 0, s, d_m, ldm, Lm, d_hsym,
                      d_x, d_g, L, ccm_nsp(L), lam1, lam2, d_g1part, n_g1blk, d_fpart, n_fpart, d_regpair,
                      n_regpair, d_regsingle, n_regsingle, d_f);
   CCM_LAUNCH_CHECK();

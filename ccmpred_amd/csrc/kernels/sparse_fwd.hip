@@ -303,19 +303,28 @@ __host__ __device__ __forceinline__ int convert_ws_tiles_p(int L) {
 // tp_run = 0, in the forward's launch); block (tiles_c + tp_run, 0) writes
 // the singles' regularisation into *regsingle (the other blocks of that
 // column return).
+// pad_zero: x's padding is known to be zero and g's to hold zeros (a CG
+// solve after its padding scan): the padding tiles write zero partials only.
 template <int PASSES>
 __device__ __forceinline__ void convert_ws_any(const float* __restrict__ x, long nsp, int L, int Lp,
                                                uint16_t* __restrict__ ws, long lds, float* __restrict__ g,
                                                float lam1, float lam2, double* __restrict__ regpart,
-                                               double* __restrict__ regsingle, int tp_run, int bx, int by) {
+                                               double* __restrict__ regsingle, int tp_run, bool pad_zero,
+                                               int bx, int by) {
   const int tc = convert_ws_tiles_c(L, PASSES), tp = convert_ws_tiles_p(L);
-  if (bx < tc)
+  if (bx < tc) {
     convert_ws_block<PASSES>(x + nsp, L, Lp, ws, lds, bx, by, 0.5f * lam2,
                              regpart ? regpart + (long)by * (tc + tp) + bx : nullptr);
-  else if (bx < tc + tp_run)
-    convert_ws_pad_tile(x + nsp, g + nsp, L, lam2, bx - tc, by, true, regpart + (long)by * (tc + tp) + bx);
-  else if (by == 0)
+  } else if (bx < tc + tp_run) {
+    double* const part = regpart + (long)by * (tc + tp) + bx;
+    if (pad_zero) {
+      if (threadIdx.x == 0) *part = 0.0;
+    } else {
+      convert_ws_pad_tile(x + nsp, g + nsp, L, lam2, bx - tc, by, true, part);
+    }
+  } else if (by == 0) {
     convert_ws_singles_reg(x, L, lam1, regsingle);
+  }
 }
 
 template <int PASSES>
@@ -323,7 +332,7 @@ __global__ __launch_bounds__(256) void k_convert_ws(const float* __restrict__ x,
                                                     uint16_t* __restrict__ ws, long lds, float* __restrict__ g,
                                                     float lam1, float lam2, double* __restrict__ regpart,
                                                     double* __restrict__ regsingle, int tp_run) {
-  convert_ws_any<PASSES>(x, nsp, L, Lp, ws, lds, g, lam1, lam2, regpart, regsingle, tp_run, blockIdx.x,
+  convert_ws_any<PASSES>(x, nsp, L, Lp, ws, lds, g, lam1, lam2, regpart, regsingle, tp_run, false, blockIdx.x,
                          blockIdx.y);
 }
 
@@ -339,7 +348,7 @@ __global__ __launch_bounds__(256) void k_convert_ws_grouped(const ccm_bprob* __r
   const int t = (int)(blockIdx.x - offs[gi]);
   const int tp = convert_ws_tiles_p(B.L), tcx = convert_ws_tiles_c(B.L, PASSES) + tp + 1;
   convert_ws_any<PASSES>(B.x, B.nsp, B.L, B.Lp, B.wt, B.ldwt, B.g, B.lam1, B.lam2, B.regpair, B.regsingle,
-                         tp, t % tcx, t / tcx);
+                         tp, B.ctl->pad_scanned && !B.ctl->x_pad_nz, t % tcx, t / tcx);
 }
 
 // One workgroup: TM = WM*RB*32 sequences x TN = WN*CB*32 gathered rows of Ws

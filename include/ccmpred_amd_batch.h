@@ -37,7 +37,8 @@ typedef struct ccm_cg_ctl {
 	int32_t phase, it, ls, steepest;
 	int32_t status, nevals;
 	int32_t x_pad_nz; /* set by ccm_launch_pad_scan: x holds nonzero a >= 21 padding */
-	int32_t pad1;
+	int32_t pad_scanned; /* set before ccm_launch_pad_scan: x_pad_nz == 0 then means zero padding
+	                      * (and g's padding, cleared at the solve's start, stays zero) */
 	double fx;      /* objective of the last evaluation (FP64 sum) */
 	double dots[3]; /* <g,s>, <g,g>, <g,gprev> at the last evaluation */
 } ccm_cg_ctl;

@@ -169,8 +169,8 @@ int ccm_launch_assemble(const float *d_m, long ldm, int Lm, const float *d_hsym,
 /* after ccm_launch_convert_ws_reg: the singles and the a < 21 pair columns
  * of g, and in the same launch *d_f = the sum of d_fpart | d_regpair |
  * d_regsingle in k_sum3's order (the grouped assembly of a sparse
- * descriptor takes this form, ccm_assemble_fs_blocks(L) blocks, and leaves
- * the sum to ccm_launch_sum3_grouped) */
+ * descriptor takes this form, ccm_assemble_fs_blocks(L) blocks per
+ * alignment, its f into ctl->fx) */
 int ccm_assemble_fs_blocks(int L);
 int ccm_launch_assemble_fs(const float *d_m, long ldm, int Lm, const float *d_hsym, const float *d_x,
                            float *d_g, int L, float lam1, float lam2, const double *d_g1part,
