@@ -1,7 +1,8 @@
 """The batched workload itself: 64 synthetic MSAs, L uniform in [50, 600],
 N log-uniform in [200, 20000] (ccmpred_amd.msa.batched_workload), solved
-together for 50 CG iterations (epsilon 0) from init_bias in the bf16f form
-with autotuned tiles and graph-captured rounds.
+together for 50 CG iterations (epsilon 0) from init_bias in the single-pass
+bf16 and the two-pass bf16f form, with autotuned tiles and graph-captured
+rounds.
 
 Checked:
   * every alignment runs all 50 iterations;
@@ -11,8 +12,9 @@ Checked:
   * f and g of one grouped evaluation against the FP64 oracle
     (evaluate_cpu.c:16-166 restated) for the three smallest alignments: at
     init_bias + symmetric N(0, 0.01^2) couplings before the solve (f 1e-5,
-    g 1e-3 relative L2), and f at the solved x (near the optimum g itself
-    is small, so its relative error there measures nothing);
+    g 1e-3 relative L2), and f at the solved x (1e-5 bf16f, 1e-4 bf16; near
+    the optimum g itself is small, so its relative error there measures
+    nothing);
   * the contact map of alignment 40 (L=418, N=348) against the FP64-oracle
     CG map (tests/golden/cg_maps/batch64_40.npy) under the criteria of
     tests/cg_fixtures.py.
@@ -61,9 +63,9 @@ def fp64(msa, x):
     return ora.evaluate_f64_par(msa, w, x.astype(np.float64), 0.01, float(np.float32(0.2 * (L - 1))))
 
 
-def test_batched_workload_64():
+@pytest.mark.parametrize("prec", ["bf16", "bf16f"])
+def test_batched_workload_64(prec):
     msas = batched_workload(64)
-    prec = "bf16f"
     say(f"batched workload: {len(msas)} MSAs, {prec}")
     work = [m.shape[0] * m.shape[1] ** 2 for m in msas]
     small = [int(i) for i in np.argsort(work)[:3]]
@@ -104,13 +106,15 @@ def test_batched_workload_64():
         assert c["rho"] >= 0.999
         assert c["identical"] or c["swaps_are_near_ties"]
 
-        # the objective at the solved x against FP64
+        # the objective at the solved x against FP64: the single-pass form
+        # rounds the larger solved couplings to BF16 in its logits (measured
+        # 2.9e-5 on alignment 33), so it has the 1e-4 of the tiny-shape tests
         fb = b.evaluate()
         for i in small:
             f64, _ = fp64(msas[i], b.get_x(i))
             ef = abs(fb[i] - f64) / abs(f64)
             say(f"  solved x, alignment {i}: f rel err {ef:.2e}")
-            assert ef <= 1e-5
+            assert ef <= (1e-4 if prec == "bf16" else 1e-5)
 
         # the same solves one alignment at a time: bit for bit
         same = 0
