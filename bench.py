@@ -339,9 +339,9 @@ def main():
     ap.add_argument("--batch-size", type=int, default=64)
     ap.add_argument("--iterations", type=int, default=50)
     ap.add_argument("--rounds-per-graph", type=int, default=8)
-    ap.add_argument("--batch-precision", default="bf16f", choices=["bf16", "bf16f", "bf16x2"],
-                    help="BF16 form of the batched solves (bf16f: two-pass forward, whose "
-                         "objective keeps the line search reliable near the optimum)")
+    ap.add_argument("--batch-precision", default="bf16", choices=["bf16", "bf16f", "bf16x2"],
+                    help="BF16 form of the batched solves (bf16: single-pass forward, the headline "
+                         "form; bf16f: two-pass forward)")
     ap.add_argument("--sequential", action="store_true",
                     help="with --batched: also time the same solves one MSA at a time")
     args = ap.parse_args()
