@@ -118,9 +118,11 @@ def time_gpu(msa, x_host, precision, steps, warmup, warmup_seconds):
 
 
 def time_cg(msa, precision, iterations):
-    """The device CG of config C2: `iterations` iterations (epsilon 0) from
-    init_bias with the reference's other parameters, the line search on the
-    device; the second of two runs is timed (the first warms the clocks)."""
+    """The device CG at L=250, N=5000: `iterations` iterations (epsilon 0)
+    from init_bias with the reference's other parameters, vectors on the
+    device (the line search on the device for the BF16 forms, on the host
+    between evaluations for FP32); the second of two runs is timed (the
+    first warms the clocks)."""
     import torch
     from ccmpred_amd.pll import PLL, init_bias
     x0 = init_bias(msa)
@@ -377,6 +379,7 @@ def main():
     fp32 = (time_gpu(msa, x, "fp32", max(10, args.steps // 4), max(2, args.warmup // 2),
                      args.warmup_seconds) if "fp32" in forms else None)
     cg = time_cg(msa, "bf16", args.cg_iterations) if args.cg_iterations > 0 else None
+    cg32 = time_cg(msa, "fp32", args.cg_iterations) if args.cg_iterations > 0 and "fp32" in forms else None
     cpu, serial = (None, None) if args.no_cpu else cpu_legs(args)
 
     L, N = L_COLS, N_SEQS
@@ -437,6 +440,8 @@ def main():
     }
     if cg is not None:
         line["cg_bf16"] = cg
+    if cg32 is not None:
+        line["cg_fp32"] = cg32
     if bf16f is not None:
         line["bf16f_evals_per_s"] = round(bf16f["evals_per_s"], 2)
         line["bf16f_step_ms"] = {k: round(v, 4) for k, v in bf16f["step_ms"].items()}
