@@ -93,9 +93,11 @@ def test_cg_against_fp64(name, prec):
     df = (r["fx"] - meta["f"]) / abs(meta["f"])
     say(f"  ret={r['ret']} it={r['iterations']} evals={r['evaluations']} f={r['fx']:.10g} "
         f"(FP64 CG {meta['f']:.10g}, rel {df:+.2e}, bound {f_gap_bound(prec, meta):.2e}) {dt:.1f} s")
+    iu = np.triu_indices(L, 1)
+    map_err = float(np.linalg.norm(m[iu].astype(np.float64) - m64[iu]) / np.linalg.norm(m64[iu]))
     say(f"  spearman={c['rho']:.6f} top-L/5 identical={c['identical']} swapped={c['swapped']} "
         f"rel to FP64 cut-off={['%.2e' % v for v in c['swap_rel']]} "
-        f"FP64 pairs within 1% of cut-off={c['fp64_pairs_in_band']}")
+        f"FP64 pairs within 1% of cut-off={c['fp64_pairs_in_band']} APC map rel L2 {map_err:.2e}")
     assert r["iterations"] == 50
     assert c["rho"] >= 0.999
     assert c["identical"] or c["swaps_are_near_ties"]
