@@ -390,9 +390,16 @@ def main():
     gemm = max((fwd_key, "gemm_grad"), key=lambda k: bf16["step_ms"][k])
     gemm_ms = bf16["step_ms"][gemm]
     achieved = flops / (gemm_ms * 1e-3) / 1e12
-    prefix = {"sparse_fwd_softmax": "k_sparse_fwd<", "gemm_fwd_softmax": "k_gemm_onehot_dtl<2,",
-              "gemm_fwd": "k_gemm_onehot_dtl<0,", "gemm_grad": "k_gemm_onehot_dtl<1,"}[gemm]
-    traffic, mfma_busy = measured_counters(prefix)
+    # (the gradient at this size runs as the banded launch, 256-row then
+    # 192-row tiles; the single-tile kernel's row where the summary has it)
+    prefixes = {"sparse_fwd_softmax": ["k_sparse_fwd<"], "gemm_fwd_softmax": ["k_gemm_onehot_dtl<2,"],
+                "gemm_fwd": ["k_gemm_onehot_dtl<0,"],
+                "gemm_grad": ["k_gemm_grad_banded<", "k_gemm_onehot_dtl<1,"]}[gemm]
+    traffic, mfma_busy = None, None
+    for prefix in prefixes:
+        traffic, mfma_busy = measured_counters(prefix)
+        if traffic is not None:
+            break
     fwd_roofline = None
     if fwd_key == "sparse_fwd_softmax":
         # the forward on structured-sparse MFMA: the dense algorithmic FLOPs

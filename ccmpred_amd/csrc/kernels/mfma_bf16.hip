@@ -396,8 +396,10 @@ __device__ __forceinline__ void gemm_onehot_tile(const uint16_t* __restrict__ A,
 // looks it up through the same XOR).  Waits are explicit vmcnt counts and the
 // step barrier carries no memory fence, so later stages stay in flight across
 // it.  Same K order as gemm_onehot_tile: the results are bit-identical.
+// SB > 0 sizes the stages (at least the tile's own), so that tiles of two
+// shapes in one kernel share the same three LDS arrays.
 
-template <int MODE, int WM, int WN, int RB, int CB, int TK, int S>
+template <int MODE, int WM, int WN, int RB, int CB, int TK, int S, int SB = 0>
 __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict__ A, long lda, int M,
                                                      int Kb, int passes,
                                                      const uint8_t* __restrict__ bytes, long ldbytes,
@@ -634,7 +636,8 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
     }
   };
 
-  constexpr int STAGE_BYTES = A_BYTES + TNB;
+  static_assert(SB == 0 || SB >= A_BYTES + TNB, "stage size");
+  constexpr int STAGE_BYTES = SB > 0 ? SB : A_BYTES + TNB;
   uint8_t* s0 = dtl_stage<STAGE_BYTES, 0>();
   uint8_t* s1 = dtl_stage<STAGE_BYTES, 1>();
   uint8_t* s2 = dtl_stage<STAGE_BYTES, 2>();
@@ -773,6 +776,37 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot(
                                                  accumulate, qid / tiles_n, qid % tiles_n);
 }
 
+// The banded gradient (MODE 1) in one launch: tiles_a tiles of 256 x 256
+// over rows [0, rows_a), then tiles_b tiles of 192 x 256 over rows
+// [rows_a, M).  Each XCD takes a contiguous run of the 256-row tiles (the
+// remainder on the low XCDs) and then a contiguous run of the 192-row tiles
+// (the remainder on the high XCDs), so that every XCD runs about as many of
+// each; within an XCD the 256-row tiles are dispatched first, and a CU that
+// finishes one takes a 192-row tile without waiting for a kernel boundary.
+// Both shapes use the 256-row tile's three LDS stages.
+template <int TK, int S>
+__global__ __launch_bounds__(512, 1) void k_gemm_grad_banded(
+    const uint16_t* __restrict__ A, long lda, int M, int Kb, int passes,
+    const uint8_t* __restrict__ bytes, long ldbytes, int Lp, int Ncol, float* __restrict__ C,
+    long ldc, int accumulate, uint32_t tiles_n, int rows_a, uint32_t tiles_a, uint32_t tiles_b) {
+  constexpr int SB = 256 * TK * 2 + 256 * TK;
+  const uint32_t x = blockIdx.x & 7u, i = blockIdx.x >> 3;
+  const uint32_t qa = tiles_a >> 3, ra = tiles_a & 7u, qb = tiles_b >> 3, rb = tiles_b & 7u;
+  const uint32_t cnt_a = qa + (x < ra ? 1u : 0u), start_a = x * qa + (x < ra ? x : ra);
+  const uint32_t cnt_b = qb + (x + rb >= 8u ? 1u : 0u), start_b = x * qb + (x + rb > 8u ? x + rb - 8u : 0u);
+  const ccm_fwd_epi epi{};
+  if (i < cnt_a) {
+    const uint32_t t = start_a + i;
+    gemm_onehot_tile_dtl<1, 2, 4, 4, 2, TK, S, SB>(A, lda, rows_a, Kb, passes, bytes, ldbytes, Lp, Ncol, C,
+                                                   ldc, accumulate, t / tiles_n, t % tiles_n, epi);
+  } else if (i - cnt_a < cnt_b) {
+    const uint32_t t = start_b + (i - cnt_a);
+    gemm_onehot_tile_dtl<1, 2, 4, 3, 2, TK, S, SB>(A + (long)rows_a * lda, lda, M - rows_a, Kb, passes, bytes,
+                                                   ldbytes, Lp, Ncol, C + (long)rows_a * ldc, ldc, accumulate,
+                                                   t / tiles_n, t % tiles_n, epi);
+  }
+}
+
 // The same tiles over a group of alignments, all with this tile variant:
 // the XCD remap runs over the whole grid, so that each XCD takes a run of
 // consecutive tiles (mostly of one alignment).
@@ -872,30 +906,42 @@ int ccm_launch_convert_wt(const float* d_x, int L, int Lp, uint16_t* d_wt, long 
 //   3: 2x4, 4x2, 1, direct    -> 256 x 256, 8 waves
 //   4: 2x2, 4x2, 1, direct    -> 256 x 128, 4 waves
 //   5: 2x2, 2x2, 2, direct    -> 128 x 128, 4 waves
+//   6: 2x4, 3x2, 1, direct    -> 192 x 256, 8 waves (gradient only)
 // "registers": next step prefetched through registers, two LDS stages;
 // "direct": direct-to-LDS loads, three stages in flight.
+// Variant 6 is the short row band of the gradient's banded launch (a band
+// of 256-row tiles, then one of 192-row tiles, so that each CU runs one
+// tile of each instead of two rounds of 256-row tiles); the forward
+// launchers take variant 3 in its place.
 #define GM_VARIANTS(X)   \
   X(0, 2, 4, 4, 2, 1, 0) \
   X(1, 2, 2, 4, 2, 2, 0) \
   X(2, 2, 2, 2, 2, 2, 0) \
   X(3, 2, 4, 4, 2, 1, 1) \
   X(4, 2, 2, 4, 2, 1, 1) \
-  X(5, 2, 2, 2, 2, 2, 1)
-#define GM_NVAR 6
+  X(5, 2, 2, 2, 2, 2, 1) \
+  X(6, 2, 4, 3, 2, 1, 1)
+#define GM_NVAR 7
+#define GM_GRAD_ONLY 6
 #define GM_TK 64
-static const int gm_tm[GM_NVAR] = {256, 256, 128, 256, 256, 128};
-static const int gm_tn[GM_NVAR] = {256, 128, 128, 256, 128, 128};
-static const int gm_wgpc[GM_NVAR] = {1, 2, 2, 1, 1, 2};
-static const int gm_direct[GM_NVAR] = {0, 0, 0, 1, 1, 1};
+static const int gm_tm[GM_NVAR] = {256, 256, 128, 256, 256, 128, 192};
+static const int gm_tn[GM_NVAR] = {256, 128, 128, 256, 128, 128, 256};
+static const int gm_wgpc[GM_NVAR] = {1, 2, 2, 1, 1, 2, 1};
+static const int gm_direct[GM_NVAR] = {0, 0, 0, 1, 1, 1, 1};
 // throughput of a fully loaded CU relative to variant 3 (measured at
 // L = 250, N = 5000 and L = 300, N = 3000 on MI355X)
-static const float gm_eff[GM_NVAR] = {0.66f, 0.70f, 0.70f, 1.0f, 0.65f, 0.88f};
+static const float gm_eff[GM_NVAR] = {0.66f, 0.70f, 0.70f, 1.0f, 0.65f, 0.88f, 0.90f};
+
+// the variant a forward launch runs for a requested one
+static int fwd_variant(int variant) { return variant == GM_GRAD_ONLY ? 3 : variant; }
 
 // kernel of a variant (only the staging form in use is instantiated)
 template <int MODE, int WM, int WN, int RB, int CB, int OCC, int DTL, bool GROUPED>
 static const void* gm_kernel() {
   if constexpr (MODE == 2 && !DTL) {
     return nullptr;  // the fused forward exists in the direct-to-LDS form only
+  } else if constexpr (MODE != 1 && RB == 3) {
+    return nullptr;  // the 192-row tile is the gradient's
   } else if constexpr (GROUPED) {
     if constexpr (DTL)
       return (const void*)k_gemm_onehot_dtl_grouped<MODE, WM, WN, RB, CB, GM_TK, 3, OCC>;
@@ -927,6 +973,7 @@ static int launch_gemm(int variant, const uint16_t* A, long lda, int M, int Kb, 
     break;
   switch (variant) { GM_VARIANTS(GM_CASE) }
 #undef GM_CASE
+  if (!fn) return CCM_ERR_ARG;
   ccm_fwd_epi epi{};  // (the register-staged kernels take no epilogue argument)
   void* args[] = {(void*)&A,     (void*)&lda,   (void*)&M,  (void*)&Kb,   (void*)&passes,
                   (void*)&bytes, (void*)&ldbytes, (void*)&Lp, (void*)&Ncol, (void*)&C,
@@ -1002,6 +1049,34 @@ static int gemm_default_variant(int M, int Ncol) {
   return best;
 }
 
+// Banded gradient launch: rows [0, rows_a) in 256-row tiles (variant 3),
+// the rest in 192-row tiles (variant 6), two launches, when the estimated
+// time (rounds of tiles over the CUs, a 192-row tile costing 0.75 of a
+// 256-row one over its relative throughput) beats the best single variant
+// by more than 2 %.  At P = 5376 (L = 250): 252 tiles of each shape, one
+// of each per CU, against 441 tiles of 256 x 256 in two rounds.  Returns
+// rows_a, or 0 for a single-variant launch.
+static int grad_band_rows(int P) {
+  const int cus = device_cus();
+  const int cols = (P + 255) / 256;
+  const int v = gemm_default_variant(P, P);
+  const uint32_t slots = (uint32_t)(cus * gm_wgpc[v]);
+  const float plain = (float)((gemm_tiles(v, P, P) + slots - 1) / slots) *
+                      (float)(gm_tm[v] * gm_tn[v] * gm_wgpc[v]) / (gm_eff[v] * 65536.f);
+  const float rel_b = 0.75f / gm_eff[GM_GRAD_ONLY];
+  float best = plain * 0.98f;
+  int rows_a = 0;
+  for (int a = 1; 256 * a < P; a++) {
+    const int b = (P - 256 * a + 191) / 192;
+    const float t = (float)((a * cols + cus - 1) / cus) + (float)((b * cols + cus - 1) / cus) * rel_b;
+    if (t < best) {
+      best = t;
+      rows_a = 256 * a;
+    }
+  }
+  return rows_a;
+}
+
 extern "C" {
 
 int ccm_gemm_variants(void) { return GM_NVAR; }
@@ -1030,6 +1105,7 @@ uint32_t ccm_gemm_tile_count(int variant, int M, int Ncol) {
 // gradient (mode 1) kcols, a multiple of the K step 64
 int ccm_gemm_variant_ok(int variant, int mode, int Lp, int kcols) {
   if (variant < 0 || variant >= GM_NVAR) return 0;
+  if (mode == 0 && variant == GM_GRAD_ONLY) return 0;
   return mode == 0 ? Lp % GM_TK == 0 : kcols % GM_TK == 0;
 }
 
@@ -1066,7 +1142,10 @@ int ccm_gemm_default_variant(int M, int Ncol) { return gemm_default_variant(M, N
 // Tiles of the direct-to-LDS variants only; a register-staged variant maps
 // to the direct one of the same shape.  Tile rows cover TM / 21 alignment
 // columns, so the row tiles number ceil(L / (TM / 21)).
-static int fused_variant(int variant) { return variant >= 0 && variant < 3 ? variant + 3 : variant; }
+static int fused_variant(int variant) {
+  variant = fwd_variant(variant);
+  return variant >= 0 && variant < 3 ? variant + 3 : variant;
+}
 
 static uint32_t fused_tiles(int v, int L, int ncols) {
   const int jg = gm_tm[v] / CCM_NA;
@@ -1077,13 +1156,13 @@ int ccm_fwd_softmax_blocks(int N) { return N <= 0 ? 0 : (N + 127) / 128; }
 
 uint32_t ccm_fwd_softmax_tile_count(int variant, int L, int ncols) {
   variant = fused_variant(variant);
-  if (variant < 3 || variant >= GM_NVAR || L < 1 || ncols < 1) return 0;
+  if (variant < 3 || variant >= GM_GRAD_ONLY || L < 1 || ncols < 1) return 0;
   return fused_tiles(variant, L, ncols);
 }
 
 int ccm_fwd_softmax_tile_rows(int variant, int L) {
   variant = fused_variant(variant);
-  if (variant < 3 || variant >= GM_NVAR || L < 1) return 0;
+  if (variant < 3 || variant >= GM_GRAD_ONLY || L < 1) return 0;
   const int jg = gm_tm[variant] / CCM_NA;
   return (L + jg - 1) / jg;
 }
@@ -1095,7 +1174,7 @@ int ccm_fwd_softmax_fits(int Lp, int passes_fwd, int passes, int ncols) {
 
 int ccm_launch_gemm_fwd_softmax_grouped(const ccm_bprob* d_probs, ccm_group g, int variant, hipStream_t s) {
   variant = fused_variant(variant);
-  if (variant < 3 || variant >= GM_NVAR) return CCM_ERR_ARG;
+  if (variant < 3 || variant >= GM_GRAD_ONLY) return CCM_ERR_ARG;
   return launch_gemm_grouped<2>(variant, d_probs, g, s);
 }
 
@@ -1138,7 +1217,7 @@ int ccm_launch_gemm_fwd_softmax(const uint16_t* d_wt, long lda, int Lp, int pass
   // whole of Dt, through 32-bit offsets
   if ((P + 1) * lda * 2 > 0x7FFFFFFFL || (mode && P * lddt * 2 > 0x7FFFFFFFL)) return CCM_ERR_ARG;
   variant = variant < 0 ? ccm_fwd_softmax_default_variant(L, ncols) : fused_variant(variant);
-  if (variant < 3 || variant >= GM_NVAR) return CCM_ERR_ARG;
+  if (variant < 3 || variant >= GM_GRAD_ONLY) return CCM_ERR_ARG;
   ccm_fwd_epi epi{d_x1, d_msapt, ldmt, d_w, L, Nc, d_dt, lddt, lo_off, passes, mode, blk0, d_fpart, d_g1part};
   const uint32_t tiles_n = (uint32_t)((ncols + gm_tn[variant] - 1) / gm_tn[variant]);
   const uint32_t tiles = fused_tiles(variant, L, ncols), total = round8(tiles);
@@ -1169,7 +1248,7 @@ int ccm_launch_gemm_fwd(const uint16_t* d_wt, long lda, int Lp, int passes, cons
   const int P = CCM_NA * Lp;
   if (Lp % 64 || ncols % 128 || ncols <= 0 || lda < (long)passes * P || ldc < ncols || (lda % 8))
     return CCM_ERR_ARG;
-  if (variant < 0) variant = gemm_default_variant(P, ncols);
+  variant = variant < 0 ? gemm_default_variant(P, ncols) : fwd_variant(variant);
   return launch_gemm<0>(variant, d_wt, lda, P, P, passes, d_msap, (long)Lp, Lp, ncols, d_zt, ldc, 0, s);
 }
 
@@ -1182,7 +1261,33 @@ int ccm_launch_gemm_grad(const uint16_t* d_dt, long lda, int Lp, int passes, con
   const int P = CCM_NA * Lp;
   if (Lp % 64 || kcols % 64 || kcols <= 0 || lda < (long)passes * kcols || ldc < P || (lda % 8))
     return CCM_ERR_ARG;
-  if (variant < 0) variant = gemm_default_variant(P, P);
+  if (variant < 0) {
+    const int rows_a = grad_band_rows(P);
+    if (rows_a > 0) {
+      const uint32_t tiles_n = (uint32_t)((P + 255) / 256);
+      const uint32_t tiles_a = (uint32_t)(rows_a / 256) * tiles_n;
+      const uint32_t tiles_b = (uint32_t)((P - rows_a + 191) / 192) * tiles_n;
+      // workgroups per XCD: its 256-row tiles (remainder on the low XCDs)
+      // and its 192-row tiles (remainder on the high XCDs); the grid covers
+      // the XCD with the most
+      uint32_t per = 0;
+      for (uint32_t x = 0; x < 8; x++) {
+        const uint32_t c = tiles_a / 8 + (x < tiles_a % 8 ? 1u : 0u) + tiles_b / 8 + (x + tiles_b % 8 >= 8 ? 1u : 0u);
+        per = c > per ? c : per;
+      }
+      const int M = P, Ncol = P;
+      void* args[] = {(void*)&d_dt,  (void*)&lda,    (void*)&M,  (void*)&kcols, (void*)&passes,
+                      (void*)&d_msapt, (void*)&ldmt, (void*)&Lp, (void*)&Ncol,  (void*)&d_gt,
+                      (void*)&ldc,   (void*)&accumulate, (void*)&tiles_n, (void*)&rows_a,
+                      (void*)&tiles_a, (void*)&tiles_b};
+      if (hipLaunchKernel((const void*)k_gemm_grad_banded<GM_TK, 3>, dim3(8 * per), dim3(512), args, 0, s) !=
+          hipSuccess)
+        return CCM_ERR_LAUNCH;
+      CCM_LAUNCH_CHECK();
+      return CCM_OK;
+    }
+    variant = gemm_default_variant(P, P);
+  }
   return launch_gemm<1>(variant, d_dt, lda, P, kcols, passes, d_msapt, ldmt, Lp, P, d_gt, ldc,
                         accumulate, s);
 }

@@ -15,7 +15,7 @@ import pytest
 torch = pytest.importorskip("torch")
 
 from oracle import oracle as ora  # noqa: E402
-from ccmpred_amd import pll as P  # noqa: E402
+from ccmpred_amd import _lib, pll as P  # noqa: E402
 from ccmpred_amd.msa import read_psicov, synthetic_msa  # noqa: E402
 from tests.pll_bruteforce import nsingle_padded, NA, NAP  # noqa: E402
 
@@ -150,23 +150,43 @@ def test_bf16_deterministic_and_structured(prec):
 
 
 @pytest.mark.parametrize("prec", ["bf16", "bf16x2"])
-@pytest.mark.parametrize("L,N", [(75, 1000), (37, 300), (130, 700)])
+@pytest.mark.parametrize("L,N", [(75, 1000), (37, 300), (130, 700), (250, 700), (418, 300)])
 def test_tile_variants_bit_identical(prec, L, N):
     """Every MFMA tile variant (register-staged or direct-to-LDS, 256x256,
-    256x128, 128x128; the forward's softmax epilogue at 12 or 6 alignment
-    columns per tile) sums in the same order: f and g equal the default
-    variant's bit for bit.  L = 75, 37, 130 leave the last tile rows partly
-    past L; N = 300, 700 leave partly filled column tiles."""
+    256x128, 128x128, the gradient's 192x256; the forward's softmax epilogue
+    at 12 or 6 alignment columns per tile) sums in the same order: f and g
+    equal the default launch's bit for bit.  L = 75, 37, 130 leave the last
+    tile rows partly past L; N = 300, 700 leave partly filled column tiles.
+    At L = 250 and 418 the default gradient launch is banded (256-row tiles
+    over the first rows, 192-row tiles over the rest; 250: rows 0..3071 of
+    5376, 418: rows 0..6911 of 9408, whose last 192-row tile is partial)."""
     say(f"tile variants {prec} L={L} N={N}")
     msa, _ = synthetic_msa(L, N, 41, 6)
     xd = torch.from_numpy(make_x(msa, 42, 0.02)).cuda()
     with P.PLL(msa, precision=prec) as p:
         f0, g0 = p.evaluate(xd)
-    for v in range(6):
+    nv = _lib.lib().ccm_gemm_variants()
+    assert nv == 7
+    for v in range(nv):
         with P.PLL(msa, precision=prec, gemm_variant=v) as p:
             f, g = p.evaluate(xd)
         say(f"  variant {v}: f={f:.17g} (default {f0:.17g}) g equal={torch.equal(g, g0)}")
         assert f == f0 and torch.equal(g, g0)
+
+
+def test_gradient_only_variant_with_dense_forward():
+    """The gradient-only tile (variant 6) requested with the dense fused
+    forward: the forward runs variant 3 in its place, the gradient variant
+    6; f and g equal the default dense-forward launch bit for bit."""
+    L, N = 130, 700
+    say(f"variant 6 with the dense forward L={L} N={N}")
+    msa, _ = synthetic_msa(L, N, 43, 6)
+    xd = torch.from_numpy(make_x(msa, 44, 0.02)).cuda()
+    with P.PLL(msa, precision="bf16", sparse=False) as p:
+        f0, g0 = p.evaluate(xd)
+    with P.PLL(msa, precision="bf16", sparse=False, gemm_variant=6) as p:
+        f, g = p.evaluate(xd)
+    assert f == f0 and torch.equal(g, g0)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "bf16f"])
