@@ -81,7 +81,7 @@ EXPORTED = [
     "ccm_batch_solve", "ccm_batch_result",
     "ccm_convert_wt_blocks", "ccm_gemm_tile_count", "ccm_gemm_variant_ok",
     "ccm_gemm_default_variant_for", "ccm_gemm_variant_direct", "ccm_gemm_tile_shape", "ccm_dots_blocks", "ccm_cg_vec_blocks",
-    "ccm_launch_convert_wt_grouped", "ccm_launch_gemm_fwd_grouped", "ccm_launch_gemm_grad_grouped",
+    "ccm_launch_convert_wt_grouped", "ccm_launch_gemm_fwd_grouped", "ccm_launch_gemm_grad_grouped", "ccm_launch_gemm_grad_banded_desc",
     "ccm_launch_softmax_cols_grouped", "ccm_launch_pair_epilogue_grouped",
     "ccm_launch_singles_finalize_grouped", "ccm_launch_sum3_grouped", "ccm_launch_dots_grouped",
     "ccm_launch_cg_vec_grouped", "ccm_launch_cg_ctrl", "ccm_launch_pad_scan",

@@ -720,6 +720,15 @@ static int enqueue_gemms(ccm_batch *b, int mode)
 {
 	const int ng = mode == 0 ? b->n_fwd : b->n_grad;
 	const bgroup *gs = mode == 0 ? b->g_fwd : b->g_grad;
+	/* one alignment with the default gradient tile: the banded launch of
+	 * the single-alignment evaluation where its size takes one (a grouped
+	 * launch has one tile shape per alignment) */
+	if (mode == 1 && b->n == 1 && ng == 1 &&
+	    gs[0].variant == ccm_gemm_default_variant_for(1, b->h_probs[0].Lp, b->h_probs[0].ncols)) {
+		const int r = ccm_launch_gemm_grad_banded_desc(b->d_probs, 0, b->h_probs[0].P, b->s);
+		if (r != 1)
+			return r;
+	}
 	for (int i = 0; i < ng; i++) {
 		int r = mode == 1   ? ccm_launch_gemm_grad_grouped(b->d_probs, gs[i].g, gs[i].variant, b->s)
 		        : b->sparse ? ccm_launch_sparse_fwd_softmax_grouped(b->d_probs, gs[i].g, b->s)

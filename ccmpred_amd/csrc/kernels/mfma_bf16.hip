@@ -785,10 +785,11 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void k_gemm_onehot(
 // finishes one takes a 192-row tile without waiting for a kernel boundary.
 // Both shapes use the 256-row tile's three LDS stages.
 template <int TK, int S>
-__global__ __launch_bounds__(512, 1) void k_gemm_grad_banded(
-    const uint16_t* __restrict__ A, long lda, int M, int Kb, int passes,
-    const uint8_t* __restrict__ bytes, long ldbytes, int Lp, int Ncol, float* __restrict__ C,
-    long ldc, int accumulate, uint32_t tiles_n, int rows_a, uint32_t tiles_a, uint32_t tiles_b) {
+__device__ __forceinline__ void grad_banded_tile(const uint16_t* __restrict__ A, long lda, int M, int Kb,
+                                                 int passes, const uint8_t* __restrict__ bytes, long ldbytes,
+                                                 int Lp, int Ncol, float* __restrict__ C, long ldc,
+                                                 int accumulate, uint32_t tiles_n, int rows_a,
+                                                 uint32_t tiles_a, uint32_t tiles_b) {
   constexpr int SB = 256 * TK * 2 + 256 * TK;
   const uint32_t x = blockIdx.x & 7u, i = blockIdx.x >> 3;
   const uint32_t qa = tiles_a >> 3, ra = tiles_a & 7u, qb = tiles_b >> 3, rb = tiles_b & 7u;
@@ -805,6 +806,28 @@ __global__ __launch_bounds__(512, 1) void k_gemm_grad_banded(
                                                    ldbytes, Lp, Ncol, C + (long)rows_a * ldc, ldc, accumulate,
                                                    t / tiles_n, t % tiles_n, epi);
   }
+}
+
+template <int TK, int S>
+__global__ __launch_bounds__(512, 1) void k_gemm_grad_banded(
+    const uint16_t* __restrict__ A, long lda, int M, int Kb, int passes,
+    const uint8_t* __restrict__ bytes, long ldbytes, int Lp, int Ncol, float* __restrict__ C,
+    long ldc, int accumulate, uint32_t tiles_n, int rows_a, uint32_t tiles_a, uint32_t tiles_b) {
+  grad_banded_tile<TK, S>(A, lda, M, Kb, passes, bytes, ldbytes, Lp, Ncol, C, ldc, accumulate, tiles_n, rows_a,
+                          tiles_a, tiles_b);
+}
+
+// The same for alignment pi of a batch, its buffers from its descriptor
+// (the whole alignment as one chunk, as the grouped gradient takes it); no
+// work once the alignment's solve has ended.
+template <int TK, int S>
+__global__ __launch_bounds__(512, 1) void k_gemm_grad_banded_desc(const ccm_bprob* __restrict__ probs, int pi,
+                                                                  uint32_t tiles_n, int rows_a,
+                                                                  uint32_t tiles_a, uint32_t tiles_b) {
+  const ccm_bprob& B = probs[pi];
+  if (!B.ctl->eval_on) return;
+  grad_banded_tile<TK, S>(B.dt, B.lddt, B.P, B.ncols, B.passes, B.msapt, B.ldmt, B.Lp, B.P, B.gt, B.P, 0,
+                          tiles_n, rows_a, tiles_a, tiles_b);
 }
 
 // The same tiles over a group of alignments, all with this tile variant:
@@ -1077,7 +1100,41 @@ static int grad_band_rows(int P) {
   return rows_a;
 }
 
+// The banded launch of a P x P gradient, where it applies (returns 0 where
+// not): the rows of 256-row tiles, the column tiles, the tiles of each band
+// and the grid (8 XCDs times the most workgroups one XCD takes: its
+// 256-row tiles, remainder on the low XCDs, and its 192-row tiles,
+// remainder on the high XCDs).
+static int grad_band_plan(int P, int* rows_a, uint32_t* tiles_n, uint32_t* tiles_a, uint32_t* tiles_b,
+                          uint32_t* grid) {
+  *rows_a = grad_band_rows(P);
+  if (*rows_a <= 0) return 0;
+  *tiles_n = (uint32_t)((P + 255) / 256);
+  *tiles_a = (uint32_t)(*rows_a / 256) * *tiles_n;
+  *tiles_b = (uint32_t)((P - *rows_a + 191) / 192) * *tiles_n;
+  uint32_t per = 0;
+  for (uint32_t x = 0; x < 8; x++) {
+    const uint32_t c = *tiles_a / 8 + (x < *tiles_a % 8 ? 1u : 0u) + *tiles_b / 8 + (x + *tiles_b % 8 >= 8 ? 1u : 0u);
+    per = c > per ? c : per;
+  }
+  *grid = 8 * per;
+  return 1;
+}
+
 extern "C" {
+
+// The banded gradient of alignment pi of a batch (descriptors at d_probs,
+// P its coupling rows); 1 (nothing launched) where P takes no banding
+int ccm_launch_gemm_grad_banded_desc(const ccm_bprob* d_probs, int pi, int P, hipStream_t s) {
+  int rows_a;
+  uint32_t tiles_n, tiles_a, tiles_b, grid;
+  if (!d_probs || pi < 0 || P <= 0) return CCM_ERR_ARG;
+  if (!grad_band_plan(P, &rows_a, &tiles_n, &tiles_a, &tiles_b, &grid)) return 1;
+  hipLaunchKernelGGL((k_gemm_grad_banded_desc<GM_TK, 3>), dim3(grid), dim3(512), 0, s, d_probs, pi, tiles_n, rows_a,
+                     tiles_a, tiles_b);
+  CCM_LAUNCH_CHECK();
+  return CCM_OK;
+}
 
 int ccm_gemm_variants(void) { return GM_NVAR; }
 
@@ -1262,25 +1319,15 @@ int ccm_launch_gemm_grad(const uint16_t* d_dt, long lda, int Lp, int passes, con
   if (Lp % 64 || kcols % 64 || kcols <= 0 || lda < (long)passes * kcols || ldc < P || (lda % 8))
     return CCM_ERR_ARG;
   if (variant < 0) {
-    const int rows_a = grad_band_rows(P);
-    if (rows_a > 0) {
-      const uint32_t tiles_n = (uint32_t)((P + 255) / 256);
-      const uint32_t tiles_a = (uint32_t)(rows_a / 256) * tiles_n;
-      const uint32_t tiles_b = (uint32_t)((P - rows_a + 191) / 192) * tiles_n;
-      // workgroups per XCD: its 256-row tiles (remainder on the low XCDs)
-      // and its 192-row tiles (remainder on the high XCDs); the grid covers
-      // the XCD with the most
-      uint32_t per = 0;
-      for (uint32_t x = 0; x < 8; x++) {
-        const uint32_t c = tiles_a / 8 + (x < tiles_a % 8 ? 1u : 0u) + tiles_b / 8 + (x + tiles_b % 8 >= 8 ? 1u : 0u);
-        per = c > per ? c : per;
-      }
+    int rows_a;
+    uint32_t tiles_n, tiles_a, tiles_b, grid;
+    if (grad_band_plan(P, &rows_a, &tiles_n, &tiles_a, &tiles_b, &grid)) {
       const int M = P, Ncol = P;
       void* args[] = {(void*)&d_dt,  (void*)&lda,    (void*)&M,  (void*)&kcols, (void*)&passes,
                       (void*)&d_msapt, (void*)&ldmt, (void*)&Lp, (void*)&Ncol,  (void*)&d_gt,
                       (void*)&ldc,   (void*)&accumulate, (void*)&tiles_n, (void*)&rows_a,
                       (void*)&tiles_a, (void*)&tiles_b};
-      if (hipLaunchKernel((const void*)k_gemm_grad_banded<GM_TK, 3>, dim3(8 * per), dim3(512), args, 0, s) !=
+      if (hipLaunchKernel((const void*)k_gemm_grad_banded<GM_TK, 3>, dim3(grid), dim3(512), args, 0, s) !=
           hipSuccess)
         return CCM_ERR_LAUNCH;
       CCM_LAUNCH_CHECK();

@@ -249,6 +249,12 @@ int ccm_fwd_softmax_tile_rows(int variant, int L);
 int ccm_launch_gemm_fwd_softmax_grouped(const ccm_bprob *d_probs, ccm_group g, int variant,
                                         ccm_stream_t s);
 int ccm_launch_gemm_grad_grouped(const ccm_bprob *d_probs, ccm_group g, int variant, ccm_stream_t s);
+/* the gradient contraction of one alignment of a batch (descriptor pi) as
+ * the banded launch that ccm_launch_gemm_grad makes by default (256-row
+ * tiles, then 192-row tiles; no work once the alignment's solve has
+ * ended); returns 1, launching nothing, where its P = 21 Lp takes no
+ * banding */
+int ccm_launch_gemm_grad_banded_desc(const ccm_bprob *d_probs, int pi, int P, ccm_stream_t s);
 int ccm_launch_softmax_cols_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
 int ccm_launch_pair_epilogue_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);
 int ccm_launch_assemble_grouped(const ccm_bprob *d_probs, ccm_group g, ccm_stream_t s);

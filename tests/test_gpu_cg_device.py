@@ -44,6 +44,31 @@ def test_device_line_search_equals_host(prec, sparse, chunk, L, N, seed):
     assert np.array_equal(xd_, xh)
 
 
+@pytest.mark.parametrize("prec", ["bf16", "bf16f"])
+@pytest.mark.parametrize("L,N,seed", [(250, 600, 7), (418, 300, 8)])
+def test_device_line_search_banded_gradient(prec, L, N, seed):
+    """At L = 250 and 418 the gradient contraction runs as the banded launch
+    (256-row, then 192-row tiles): the device CG's batch of one launches it
+    from its descriptor, the host loop through evaluate_hip; the two runs
+    agree bit for bit."""
+    msa, _ = synthetic_msa(L, N, seed, L // 12)
+    print(f"device CG, banded gradient: {prec} L={L} N={N}", flush=True)
+    out = {}
+    with P.PLL(msa, precision=prec) as p:
+        for ls in ("device", "host"):
+            xd = torch.from_numpy(P.init_bias(msa)).cuda()
+            r = p.conjugrad(xd, max_iterations=20, epsilon=0.0, line_search=ls)
+            out[ls] = (r, xd.cpu().numpy())
+    (rd, xd_), (rh, xh) = out["device"], out["host"]
+    print(f"  device ret={rd['ret']} it={rd['iterations']} ev={rd['evaluations']} f={rd['fx']!r} | "
+          f"host ret={rh['ret']} it={rh['iterations']} ev={rh['evaluations']} f={rh['fx']!r}", flush=True)
+    assert rh["iterations"] == 20
+    assert (rd["ret"], rd["iterations"], rd["evaluations"]) == (rh["ret"], rh["iterations"], rh["evaluations"])
+    assert rd["fx"] == rh["fx"]
+    assert np.array_equal(rd["fhist"], rh["fhist"])
+    assert np.array_equal(xd_, xh)
+
+
 def test_device_line_search_unaligned_x_takes_host_loop():
     """x not 16-byte aligned: the host loop runs and gives the aligned copy's
     results (the work vectors are zeroed and synchronised on every path)."""
