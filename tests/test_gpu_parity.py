@@ -189,6 +189,24 @@ def test_gradient_only_variant_with_dense_forward():
     assert f == f0 and torch.equal(g, g0)
 
 
+@pytest.mark.parametrize("prec", ["bf16", "bf16x2"])
+def test_chunked_banded_gradient(prec):
+    """Chunks accumulate into the stored pair gradient (the launch's
+    accumulate mode): at L = 250 the default gradient launch is banded, and
+    its chunked f and g equal the single-shape (variant 3) chunked ones bit
+    for bit; N = 1100 in chunks of 512 leaves a ragged last chunk."""
+    L, N = 250, 1100
+    say(f"chunked banded gradient {prec} L={L} N={N}")
+    msa, _ = synthetic_msa(L, N, 33, 20)
+    xd = torch.from_numpy(make_x(msa, 34, 0.02)).cuda()
+    with P.PLL(msa, precision=prec, chunk_rows=512) as p:
+        f0, g0 = p.evaluate(xd)
+    with P.PLL(msa, precision=prec, chunk_rows=512, gemm_variant=3) as p:
+        f, g = p.evaluate(xd)
+    say(f"  f banded={f0:.17g} variant 3={f:.17g} g equal={torch.equal(g, g0)}")
+    assert f == f0 and torch.equal(g, g0)
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "bf16f"])
 @pytest.mark.parametrize("chunk", [512, 1000, 3000])
 def test_chunked_evaluate_matches_unchunked(prec, chunk):
