@@ -141,6 +141,14 @@ int ccm_launch_sparse_fwd_softmax(const uint16_t *d_ws, long lds, int Lp, int pa
                                   const uint8_t *d_msapt, long ldmt, const float *d_w, int L, int Nc,
                                   uint16_t *d_dt, long lddt, long lo_off, int passes, int mode,
                                   int blk0, double *d_fpart, double *d_g1part, ccm_stream_t s);
+/* The pair-gradient contraction Gt[c][r] (+)= sum_n Dt[c][n] [x_nk = b]
+ * (cu:279-312 without the one-hot term).  Tile variants (ccm_gemm_variants):
+ * 0-5 serve both contractions, 6 (192 x 256) the gradient only (the
+ * forward launchers run variant 3 for it); every variant sums in the same
+ * order, so all give the same bits.  variant < 0: the default, which is
+ * one banded launch (256-row tiles over the first rows, 192-row tiles over
+ * the rest, one of each per CU) where the CU count favours it, else the
+ * cost model's single variant. */
 int ccm_launch_gemm_grad(const uint16_t *d_dt, long lda, int Lp, int passes,
                          const uint8_t *d_msapt, long ldmt, int kcols, float *d_gt, long ldc,
                          int accumulate, int variant, ccm_stream_t s);
