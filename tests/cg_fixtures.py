@@ -4,7 +4,8 @@ tests/golden/make_cg_maps.py) and the criteria a device run is held to.
 A device CG run of 50 iterations from init_bias (epsilon 0) is compared with
 the FP64-evaluated oracle CG of the same algorithm on the same alignment:
   * APC map Spearman correlation over i < j >= 0.999;
-  * identical top-L/5 pairs at separation j - i >= 6 (ties by (i, j)), where
+  * identical top-L/5 pairs at separation j - i >= 6 (ties by (i, j)), and
+    at j - i >= 7 (the separation of the reference's top_couplings.py), where
     the only swaps allowed are between pairs whose FP64 scores lie within
     1 % of the FP64 cut-off score (the L/5-th), i.e. pairs the FP64 map
     itself does not separate by more than 1 %.
@@ -54,12 +55,13 @@ def spearman(a, b):
     return float(np.corrcoef(ra, rb)[0, 1])
 
 
-def compare(m_dev, m64):
-    """Spearman, top-L/5 sets and the FP64 near-tie test of the swaps."""
+def compare(m_dev, m64, min_sep=6):
+    """Spearman, top-L/5 sets at separation j - i >= min_sep and the FP64
+    near-tie test of the swaps."""
     L = m64.shape[0]
     k = L // 5
-    t_dev, t64 = set(top_pairs(m_dev, k)), set(top_pairs(m64, k))
-    iu = np.triu_indices(L, k=6)
+    t_dev, t64 = set(top_pairs(m_dev, k, min_sep)), set(top_pairs(m64, k, min_sep))
+    iu = np.triu_indices(L, k=min_sep)
     cut = float(np.sort(m64[iu])[::-1][k - 1])
     swapped = sorted(t_dev ^ t64)
     rel = [abs(float(m64[p]) - cut) / abs(cut) for p in swapped]

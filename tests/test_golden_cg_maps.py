@@ -42,6 +42,7 @@ def test_fixture_consistent(name):
     assert msa.shape == (meta["N"], L)
     assert meta["iterations"] == 50 and meta["ret"] == 2  # max iterations reached
     assert [tuple(p) for p in meta["top_l5"]] == F.top_pairs(m, L // 5)
+    assert [tuple(p) for p in meta["top_l5_sep7"]] == F.top_pairs(m, L // 5, 7)
     assert np.isclose(meta["cutoff_score"], F.compare(m, m)["cut"], rtol=0, atol=0)
     assert all(a >= b for a, b in zip(meta["fhist"], meta["fhist"][1:]))
 

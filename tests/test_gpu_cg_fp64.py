@@ -98,7 +98,12 @@ def test_cg_against_fp64(name, prec):
     say(f"  spearman={c['rho']:.6f} top-L/5 identical={c['identical']} swapped={c['swapped']} "
         f"rel to FP64 cut-off={['%.2e' % v for v in c['swap_rel']]} "
         f"FP64 pairs within 1% of cut-off={c['fp64_pairs_in_band']} APC map rel L2 {map_err:.2e}")
+    c7 = F.compare(m, m64, min_sep=7)
+    assert [list(p) for p in F.top_pairs(m64, c7["k"], 7)] == [list(p) for p in meta["top_l5_sep7"]]
+    say(f"  separation >= 7: top-L/5 identical={c7['identical']} swapped={c7['swapped']} "
+        f"rel to FP64 cut-off={['%.2e' % v for v in c7['swap_rel']]}")
     assert r["iterations"] == 50
     assert c["rho"] >= 0.999
     assert c["identical"] or c["swaps_are_near_ties"]
+    assert c7["identical"] or c7["swaps_are_near_ties"]
     assert abs(df) <= f_gap_bound(prec, meta)
