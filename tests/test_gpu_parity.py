@@ -150,16 +150,17 @@ def test_bf16_deterministic_and_structured(prec):
 
 
 @pytest.mark.parametrize("prec", ["bf16", "bf16x2"])
-@pytest.mark.parametrize("L,N", [(75, 1000), (37, 300), (130, 700), (250, 700), (418, 300)])
+@pytest.mark.parametrize("L,N", [(75, 1000), (37, 300), (130, 700), (250, 700), (418, 300), (700, 200)])
 def test_tile_variants_bit_identical(prec, L, N):
     """Every MFMA tile variant (register-staged or direct-to-LDS, 256x256,
     256x128, 128x128, the gradient's 192x256; the forward's softmax epilogue
     at 12 or 6 alignment columns per tile) sums in the same order: f and g
     equal the default launch's bit for bit.  L = 75, 37, 130 leave the last
     tile rows partly past L; N = 300, 700 leave partly filled column tiles.
-    At L = 250 and 418 the default gradient launch is banded (256-row tiles
-    over the first rows, 192-row tiles over the rest; 250: rows 0..3071 of
-    5376, 418: rows 0..6911 of 9408, whose last 192-row tile is partial)."""
+    At L = 250, 418 and 700 the default gradient launch is banded (256-row
+    tiles over the first rows, 192-row tiles over the rest; 250: rows
+    0..3071 of 5376, 418: rows 0..6911 of 9408, 700: rows 0..12287 of
+    14784, whose last column tile is 192 wide)."""
     say(f"tile variants {prec} L={L} N={N}")
     msa, _ = synthetic_msa(L, N, 41, 6)
     xd = torch.from_numpy(make_x(msa, 42, 0.02)).cuda()
