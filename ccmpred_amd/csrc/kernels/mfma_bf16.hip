@@ -575,7 +575,12 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
   // wait for the next stage's loads, barrier (after which no wave reads cur
   // any more), refill cur with step st+3 and request the next step's first
   // fragments from nxt.  Neither the barrier nor the next step's LDS latency
-  // then leaves the matrix pipe without queued work.
+  // then leaves the matrix pipe without queued work.  The gradient issues a
+  // sub-step's MFMAs column block by column block and builds the next
+  // sub-step's column-block-0 one-hot fragment (bnext0) between them, so
+  // that its one-hot arithmetic overlaps MFMAs instead of preceding them
+  // (every accumulator still sums its K steps in the same order).
+  bf16x8_t bnext0;
   auto step = [&](uint8_t* cur, const uint8_t* nxt, int st) {
     uint32_t bsel = 0;
     if constexpr (FWD) {
@@ -589,17 +594,17 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
         if (((sub + 1) & 1) == 0) ld_bytes(cur, bw[((sub + 1) >> 1) & 1], (sub + 1) >> 1);
       }
       bf16x8_t bfrag[CB];
+      if constexpr (FWD) {
 #pragma unroll
-      for (int cb = 0; cb < CB; cb++) {
-        if constexpr (FWD) {
+        for (int cb = 0; cb < CB; cb++) {
           const int d = (sub >> 1) * 8 + (sub & 1) * 4;
           bfrag[cb] = onehot8_spread(bsp[cb][d], bsp[cb][d + 1], bsp[cb][d + 2], bsp[cb][d + 3],
                                      bsel * 0x00010001u);
-        } else {
-          const uint4 w = bw[(sub >> 1) & 1][cb];
-          const uint2 v = (sub & 1) ? make_uint2(w.z, w.w) : make_uint2(w.x, w.y);
-          bfrag[cb] = onehot8(v, bval[cb]);
         }
+      } else {
+        bfrag[0] = bnext0;
+        const uint4 w = bw[(sub >> 1) & 1][1];
+        bfrag[1] = onehot8((sub & 1) ? make_uint2(w.z, w.w) : make_uint2(w.x, w.y), bval[1]);
       }
       if (sub == SUBS - 1) {
         // loads in flight: those of steps st+1 and st+2 (and, in the forward, a
@@ -626,17 +631,38 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
         __builtin_amdgcn_sched_barrier(0);
         issue(cur, st + 3 < nsteps ? st + 3 : nsteps - 1);
       }
+      if constexpr (FWD) {
 #pragma unroll
-      for (int rb = 0; rb < RB; rb++) {
-        const bf16x8_t afrag = __builtin_bit_cast(bf16x8_t, af[sub & 1][rb]);
+        for (int rb = 0; rb < RB; rb++) {
+          const bf16x8_t afrag = __builtin_bit_cast(bf16x8_t, af[sub & 1][rb]);
 #pragma unroll
-        for (int cb = 0; cb < CB; cb++)
-          acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afrag, bfrag[cb], acc[rb][cb], 0, 0, 0);
+          for (int cb = 0; cb < CB; cb++)
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(afrag, bfrag[cb], acc[rb][cb], 0, 0, 0);
+        }
+      } else {
+        // column block 0's MFMAs, the next sub-step's column-block-0
+        // fragment (into the registers bfrag[0] leaves: past the step's
+        // last sub-step from the next stage's bytes, read at the hand-over),
+        // then column block 1's MFMAs, which the one-hot arithmetic overlaps
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++)
+          acc[rb][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, af[sub & 1][rb]),
+                                                               bfrag[0], acc[rb][0], 0, 0, 0);
+        {
+          const int ns = sub + 1 < SUBS ? sub + 1 : 0;
+          const uint4 w = bw[(ns >> 1) & 1][0];
+          bnext0 = onehot8((ns & 1) ? make_uint2(w.z, w.w) : make_uint2(w.x, w.y), bval[0]);
+        }
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++)
+          acc[rb][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, af[sub & 1][rb]),
+                                                               bfrag[1], acc[rb][1], 0, 0, 0);
       }
     }
   };
 
   static_assert(SB == 0 || SB >= A_BYTES + TNB, "stage size");
+  static_assert(FWD || CB == 2, "gradient: two column blocks per wave");
   constexpr int STAGE_BYTES = SB > 0 ? SB : A_BYTES + TNB;
   uint8_t* s0 = dtl_stage<STAGE_BYTES, 0>();
   uint8_t* s1 = dtl_stage<STAGE_BYTES, 1>();
@@ -654,6 +680,7 @@ __device__ __forceinline__ void gemm_onehot_tile_dtl(const uint16_t* __restrict_
   }
   lda_frags(s0, af[0], 0);
   ld_bytes(s0, bw[0], 0);
+  if constexpr (!FWD) bnext0 = onehot8(make_uint2(bw[0][0].x, bw[0][0].y), bval[0]);
   // one flat loop over the steps, three (one per stage) per iteration: a
   // nested loop over K blocks made the compiler lose track of which stage
   // the loads in flight at the inner loop's head write, and wait for all of
