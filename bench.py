@@ -50,6 +50,33 @@ def measured_counters(kernel_prefix):
     return None, None
 
 
+PEAK_FILE = os.path.join(ROOT, "profiles", "mfma_peak_microbench.md")
+
+
+def measured_mfma_peaks():
+    """The register-only MFMA throughputs of the committed microbenchmark
+    summary (ccmpred_amd/bin/mfma_peak's JSON line), or {}."""
+    try:
+        for ln in open(PEAK_FILE).read().splitlines():
+            ln = ln.strip()
+            if ln.startswith('{"cus"'):
+                return json.loads(ln)
+    except (OSError, ValueError):
+        pass
+    return {}
+
+
+def measured_dense_peak(kernel, achieved):
+    """Context beside the published peak for a dense contraction kernel: the
+    measured register-only rate of its instruction with operands like its
+    own (0/1 against small BF16 values), and the fraction of it achieved."""
+    pk = measured_mfma_peaks().get("mfma_32x32x16_bf16_contraction_like_tflops")
+    if not pk or kernel == "sparse_fwd_softmax":
+        return {}
+    return {"peak_measured": pk, "frac_measured": round(achieved / pk, 4),
+            "peak_measured_source": os.path.relpath(PEAK_FILE, ROOT)}
+
+
 LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clock x 2.4 GHz = 78.6 TB/s
 
 
@@ -418,6 +445,10 @@ def main():
                         "peak_sparse": MFMA_BF16_SPARSE_PEAK_TFLOPS,
                         "frac_sparse": round(executed / (fms * 1e-3) / 1e12 / MFMA_BF16_SPARSE_PEAK_TFLOPS, 4),
                         "unit": "TFLOP/s", "traffic": f_traffic, "mfma_busy": f_busy}
+        sp = measured_mfma_peaks().get("smfmac_32x32x32_bf16_contraction_like_tflops")
+        if sp:
+            fwd_roofline["peak_sparse_measured"] = sp
+            fwd_roofline["frac_sparse_measured"] = round(executed / (fms * 1e-3) / 1e12 / sp, 4)
     line = {
         "metric": "PLL f+grad evals/s at L=250,N=5000: BF16-MFMA vs FP32 GPU vs host CPU",
         "value": round(bf16["evals_per_s"], 2),
@@ -439,7 +470,8 @@ def main():
                      "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": traffic,
                      "algorithmic_flops": flops, "avg_ms": round(gemm_ms, 4),
                      "mfma_busy": mfma_busy,
-                     "counters": os.path.relpath(COUNTERS_FILE, ROOT) if traffic else None},
+                     "counters": os.path.relpath(COUNTERS_FILE, ROOT) if traffic else None,
+                     **measured_dense_peak(gemm, achieved)},
         "roofline_forward": fwd_roofline,
         "warmup_steps_run": bf16["warmup_steps"],
         "bf16_step_ms": {k: round(v, 4) for k, v in bf16["step_ms"].items()},
