@@ -6,7 +6,8 @@
 // varying.  No memory is read in the loop; each wave writes one float at the
 // end so that no MFMA is dead.  Prints one JSON line: TFLOP/s per
 // instruction and operand kind (dense-equivalent flops: 2*32*32*16 per
-// dense and 2*32*32*32 per sparse instruction).
+// dense and 2*32*32*32 per sparse instruction) at 8 waves per SIMD, and
+// the same at 1, 2 and 4 waves per SIMD (keys ending _1w, _2w, _4w).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -84,15 +85,36 @@ int main() {
   int dev = 0, cus = 0;
   CK(hipGetDevice(&dev));
   CK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const int blocks = cus * 8;  // 8 workgroups of 4 waves per CU: 8 waves per SIMD
   float* out = nullptr;
-  CK(hipMalloc(&out, sizeof(float) * (size_t)blocks * 4));
+  CK(hipMalloc(&out, sizeof(float) * (size_t)cus * 8 * 4));
   hipEvent_t e0, e1;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
-  const double waves = (double)blocks * 4;
   const char* names[2] = {"contraction_like", "random"};
+  // the clocks rise over the first second or so of load: 1.5 s of the dense
+  // loop before anything is timed (one warm-up launch per measurement is
+  // not enough; the first configuration measured would read low)
+  {
+    float spent = 0.f;
+    while (spent < 1500.f) {
+      CK(hipEventRecord(e0, 0));
+      hipLaunchKernelGGL(k_peak_dense, dim3(cus * 8), dim3(256), 0, 0, 0, out);
+      CK(hipGetLastError());
+      CK(hipEventRecord(e1, 0));
+      CK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      CK(hipEventElapsedTime(&ms, e0, e1));
+      spent += ms;
+    }
+  }
+  // waves per SIMD: 8 (the headline figures), then 1, 2 and 4 (the
+  // contraction kernels run two per SIMD)
   printf("{\"cus\": %d", cus);
+  const int wps_list[4] = {8, 1, 2, 4};
+  for (int w = 0; w < 4; w++) {
+  const int wps = wps_list[w];
+  const int blocks = cus * wps;  // wps workgroups of 4 waves per CU
+  const double waves = (double)blocks * 4;
   for (int sparse = 0; sparse < 2; sparse++)
     for (int kind = 0; kind < 2; kind++) {
       float best = 1e30f;
@@ -107,12 +129,17 @@ int main() {
         CK(hipEventSynchronize(e1));
         float ms = 0.f;
         CK(hipEventElapsedTime(&ms, e0, e1));
-        if (rep > 0 && ms < best) best = ms;  // the first launch warms the clocks
+        if (rep > 0 && ms < best) best = ms;
       }
       const double flops = waves * ITERS * ACC * (sparse ? 65536.0 : 32768.0);
-      printf(", \"%s_%s_tflops\": %.1f", sparse ? "smfmac_32x32x32_bf16" : "mfma_32x32x16_bf16", names[kind],
-             flops / (best * 1e-3) / 1e12);
+      if (wps == 8)
+        printf(", \"%s_%s_tflops\": %.1f", sparse ? "smfmac_32x32x32_bf16" : "mfma_32x32x16_bf16", names[kind],
+               flops / (best * 1e-3) / 1e12);
+      else
+        printf(", \"%s_%s_tflops_%dw\": %.1f", sparse ? "smfmac_32x32x32_bf16" : "mfma_32x32x16_bf16",
+               names[kind], wps, flops / (best * 1e-3) / 1e12);
     }
+  }
   printf("}\n");
   CK(hipFree(out));
   return 0;
